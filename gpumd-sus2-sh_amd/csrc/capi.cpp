@@ -1,0 +1,275 @@
+// C ABI (include/sus2sh.h): status codes instead of exit(1), messages kept per handle.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/sus2sh.h"
+#include "engine.hpp"
+#include "model.hpp"
+
+struct sus2sh_handle {
+  std::unique_ptr<sus2sh::Engine> engine;
+  std::string error;
+};
+
+struct sus2sh_graph {
+  sus2sh::StandardGraph g;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <typename F>
+int guarded(std::string* err, F&& fn)
+{
+  try {
+    fn();
+    return SUS2SH_OK;
+  } catch (const sus2sh::InputError& e) {
+    *err = e.what();
+    return SUS2SH_ERR_INPUT;
+  } catch (const sus2sh::ArgumentError& e) {
+    *err = e.what();
+    return SUS2SH_ERR_ARGUMENT;
+  } catch (const sus2sh::UnsupportedError& e) {
+    *err = e.what();
+    return SUS2SH_ERR_UNSUPPORTED;
+  } catch (const sus2sh::DeviceError& e) {
+    *err = e.what();
+    return SUS2SH_ERR_DEVICE;
+  } catch (const std::bad_alloc&) {
+    *err = "host allocation failed";
+    return SUS2SH_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    *err = e.what();
+    return SUS2SH_ERR_INPUT;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sus2sh_probe(const char* path)
+{
+  if (!path) return 0;
+  try {
+    return sus2sh::probe_model_file(path) ? 1 : 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
+                  sus2sh_handle** out)
+{
+  if (!out) {
+    g_create_error = "sus2sh_create: out is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  *out = nullptr;
+  if (!mtp_path || num_atoms < 0 || nopts < 0 || (nopts > 0 && !opts)) {
+    g_create_error = "sus2sh_create: invalid arguments";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  std::unique_ptr<sus2sh_handle> h(new sus2sh_handle());
+  const int rc = guarded(&g_create_error, [&] {
+    h->engine.reset(new sus2sh::Engine(mtp_path, num_atoms, nopts, opts));
+  });
+  if (rc != SUS2SH_OK) return rc;
+  g_create_error.clear();
+  *out = h.release();
+  return SUS2SH_OK;
+}
+
+void sus2sh_destroy(sus2sh_handle* handle) { delete handle; }
+
+int sus2sh_compute(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                   const int* d_type, const double* d_position, double* d_potential,
+                   double* d_force, double* d_virial, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  if (!box) {
+    handle->error = "sus2sh_compute: box is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  return guarded(&handle->error, [&] {
+    handle->engine->compute(*box, num_atoms, d_type, d_position, d_potential, d_force, d_virial,
+                            static_cast<hipStream_t>(stream), nullptr);
+  });
+}
+
+int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                              const int* d_type, const double* d_position,
+                              double* d_potential, double* d_force, double* d_virial,
+                              double* d_param_grad, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  if (!box || !d_param_grad) {
+    handle->error = "sus2sh_compute_param_grad: box or d_param_grad is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  return guarded(&handle->error, [&] {
+    handle->engine->compute(*box, num_atoms, d_type, d_position, d_potential, d_force, d_virial,
+                            static_cast<hipStream_t>(stream), d_param_grad);
+  });
+}
+
+const char* sus2sh_last_error(const sus2sh_handle* handle)
+{
+  if (!handle) return g_create_error.c_str();
+  return handle->error.c_str();
+}
+
+double sus2sh_cutoff(const sus2sh_handle* handle)
+{
+  return handle ? handle->engine->model().max_dist : 0.0;
+}
+
+int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out)
+{
+  if (!handle || !out) return SUS2SH_ERR_ARGUMENT;
+  const sus2sh::Model& m = handle->engine->model();
+  const sus2sh::GraphProgram& g = handle->engine->program();
+  std::memset(out, 0, sizeof(*out));
+  out->species_count = m.species_count;
+  out->l_max = m.l_max;
+  out->k_max = m.k_max;
+  out->body_order = m.body_order;
+  out->radial_basis_size = m.rb_size;
+  out->radial_funcs_count = m.radial_funcs_count;
+  out->moments_count = m.moments_count;
+  out->basic_count = m.basic_count;
+  out->scalar_count = m.scalar_count;
+  out->product_count = static_cast<int>(m.products.size());
+  out->layer_count = m.layer_count;
+  out->row_count = static_cast<int>(m.rows.size());
+  out->back_row_count = m.back_row_count;
+  out->back_term_count = m.back_term_count;
+  out->standard_graph = m.standard_graph_matched ? 1 : 0;
+  out->use_float = handle->engine->options().use_float ? 1 : 0;
+  out->param_count = handle->engine->param_count();
+  out->stored_slots = g.slot_count;
+  out->graph_tile = handle->engine->graph_tile();
+  out->neighbor_capacity = handle->engine->neighbor_capacity();
+  out->max_dist = m.max_dist;
+  return SUS2SH_OK;
+}
+
+int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n)
+{
+  if (!handle || !ms || n < 0) return 0;
+  const std::vector<float>& st = handle->engine->stage_ms();
+  const int k = std::min<int>(n, static_cast<int>(st.size()));
+  for (int i = 0; i < k; ++i) ms[i] = st[i];
+  return k;
+}
+
+double sus2sh_mean_neighbors(const sus2sh_handle* handle)
+{
+  return handle ? handle->engine->mean_neighbors() : 0.0;
+}
+
+double sus2sh_clebsch_gordan(int j1, int m1, int j2, int m2, int j, int m)
+{
+  return sus2sh::clebsch_gordan(j1, m1, j2, m2, j, m);
+}
+
+double sus2sh_real_cg(int l1, int m1, int l2, int m2, int L, int M)
+{
+  try {
+    return sus2sh::real_cg(l1, m1, l2, m2, L, M);
+  } catch (...) {
+    return 0.0;
+  }
+}
+
+int sus2sh_standard_graph(int l_max, int k_max, int body_order, const int* body_l_max,
+                          int n_body_l_max, sus2sh_graph** out)
+{
+  if (!out) return SUS2SH_ERR_ARGUMENT;
+  *out = nullptr;
+  std::unique_ptr<sus2sh_graph> g(new sus2sh_graph());
+  const int rc = guarded(&g_create_error, [&] {
+    std::vector<int> bl(7, l_max);
+    if (body_l_max) {
+      if (n_body_l_max < 0 || n_body_l_max > 5)
+        throw sus2sh::ArgumentError("n_body_l_max must be in [0,5]");
+      for (int i = 0; i < n_body_l_max; ++i) bl[2 + i] = body_l_max[i];
+    }
+    for (int b = 2; b <= body_order && b < 7; ++b)
+      if (bl[b] < 0 || bl[b] > l_max) throw sus2sh::ArgumentError("body_l_max entry out of range");
+    g->g = sus2sh::build_standard_graph(l_max, k_max, body_order, bl);
+  });
+  if (rc != SUS2SH_OK) return rc;
+  *out = g.release();
+  return SUS2SH_OK;
+}
+
+int sus2sh_graph_dims(const sus2sh_graph* graph, int* dims)
+{
+  if (!graph || !dims) return SUS2SH_ERR_ARGUMENT;
+  dims[0] = graph->g.node_count;
+  dims[1] = static_cast<int>(graph->g.basic_k.size());
+  dims[2] = static_cast<int>(graph->g.products.size());
+  dims[3] = static_cast<int>(graph->g.scalars.size());
+  dims[4] = graph->g.max_layer;
+  return SUS2SH_OK;
+}
+
+int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_idx,
+                        double* product_coeff, int* scalars, int* scalar_body)
+{
+  if (!graph) return SUS2SH_ERR_ARGUMENT;
+  const sus2sh::StandardGraph& g = graph->g;
+  if (basic_klm)
+    for (size_t b = 0; b < g.basic_k.size(); ++b) {
+      basic_klm[3 * b] = g.basic_k[b];
+      basic_klm[3 * b + 1] = g.basic_l[b];
+      basic_klm[3 * b + 2] = g.basic_m[b];
+    }
+  for (size_t p = 0; p < g.products.size(); ++p) {
+    if (product_idx) {
+      product_idx[3 * p] = g.products[p].left;
+      product_idx[3 * p + 1] = g.products[p].right;
+      product_idx[3 * p + 2] = g.products[p].target;
+    }
+    if (product_coeff) product_coeff[p] = g.products[p].coeff;
+  }
+  for (size_t s = 0; s < g.scalars.size(); ++s) {
+    if (scalars) scalars[s] = g.scalars[s];
+    if (scalar_body) scalar_body[s] = g.scalar_body[s];
+  }
+  return SUS2SH_OK;
+}
+
+void sus2sh_graph_free(sus2sh_graph* graph) { delete graph; }
+
+int sus2sh_model_check(const char* mtp_path, int* dims)
+{
+  if (!mtp_path) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&g_create_error, [&] {
+    const sus2sh::Model m = sus2sh::load_model(mtp_path);
+    if (dims) {
+      dims[0] = m.species_count;
+      dims[1] = m.l_max;
+      dims[2] = m.k_max;
+      dims[3] = m.body_order;
+      dims[4] = m.rb_size;
+      dims[5] = m.radial_funcs_count;
+      dims[6] = m.moments_count;
+      dims[7] = m.basic_count;
+      dims[8] = m.scalar_count;
+      dims[9] = static_cast<int>(m.products.size());
+      dims[10] = m.layer_count;
+      dims[11] = static_cast<int>(m.rows.size());
+      dims[12] = static_cast<int>(m.row_terms.size());
+      dims[13] = m.back_row_count;
+      dims[14] = m.back_term_count;
+      dims[15] = m.standard_graph_matched ? 1 : 0;
+    }
+  });
+}
+
+}  // extern "C"

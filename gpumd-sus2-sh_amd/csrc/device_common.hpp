@@ -1,0 +1,241 @@
+// Device-side building blocks shared by the SUS2-SH kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sus2sh {
+
+// One neighbour edge, stored k-major (edge (k, p) at k * N + p) so that a wavefront whose
+// lanes are 64 consecutive atoms reads 64 consecutive edges.
+template <typename Real>
+struct alignas(16) Edge {
+  Real dx, dy, dz;  // r_q(image) - r_p
+  int j;            // neighbour in cell-sorted order
+};
+template <>
+struct alignas(16) Edge<double> {
+  double dx, dy, dz;
+  int j;
+  int pad;
+};
+
+template <typename Real>
+struct Term {
+  uint32_t idx;
+  Real c;
+};
+template <>
+struct alignas(16) Term<double> {
+  uint32_t idx;
+  uint32_t pad;
+  double c;
+};
+
+template <typename T, int N>
+using vec_t = T __attribute__((ext_vector_type(N)));
+
+// Radial tables for pair p = t_i S + t_j (built from the model on the host).
+template <typename Real>
+struct RadialTable {
+  const Real* coeff;  // [S*S][Mr][rb]  scaling * w_ti * w_tj * radial_coeffs[mu][n]
+  const Real* scal;   // [S*S][Mr][2]   (a, s)
+  int S;
+  int rb;
+  Real rc;
+  int kind;     // 0: xi = tanh(z), 1: xi = z / sqrt(1 + z^2); z = a (r - s) / 2
+  int uniform;  // 1 when (a, s) does not depend on mu for every pair
+};
+
+struct CellGrid {
+  double h[9];
+  double hi[9];
+  int pbc[3];
+  int n[3];  // cells per direction
+  int R[3];  // cell search half-range per direction
+};
+
+template <typename Real>
+__device__ __forceinline__ Real dev_sqrt(Real x)
+{
+  return sqrt(x);
+}
+
+// xi(r) and dxi/dr for one (a, s)
+template <typename Real>
+__device__ __forceinline__ void radial_map(int kind, Real a, Real s, Real r, Real& xi, Real& dxi)
+{
+  const Real z = Real(0.5) * a * (r - s);
+  if (kind == 1) {
+    const Real q = Real(1) / sqrt(Real(1) + z * z);
+    xi = z * q;
+    dxi = Real(0.5) * a * q * q * q;
+  } else {
+    xi = tanh(z);
+    dxi = Real(0.5) * a * (Real(1) - xi * xi);
+  }
+}
+
+// R_mu(r) (and R'_mu(r)) for all MR radial functions of one pair: Chebyshev recurrence on
+// xi with the (r - rc)^2 envelope (src/force/sus2_sh.cu:3171-3220).  `coeff` and `scal`
+// point at the pair's block of the tables (usually in LDS).
+template <typename Real, int MR, bool DERIV>
+__device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
+                                            const Real* __restrict__ scal, int rb, Real rc,
+                                            int kind, int uniform, Real r, Real (&R)[MR],
+                                            Real (&dR)[MR])
+{
+  const Real dr = r - rc;
+  const Real env = dr * dr;
+  const Real denv = Real(2) * dr;
+  Real xi = 0, dxi = 0;
+  if (uniform) radial_map<Real>(kind, scal[0], scal[1], r, xi, dxi);
+#pragma unroll
+  for (int mu = 0; mu < MR; ++mu) {
+    if (!uniform) radial_map<Real>(kind, scal[2 * mu], scal[2 * mu + 1], r, xi, dxi);
+    const Real* c = coeff + mu * rb;
+    Real t0 = Real(1), t1 = xi, d0 = Real(0), d1 = Real(1);
+    Real acc = c[0];
+    Real accd = Real(0);
+    if (rb > 1) {
+      acc += c[1] * xi;
+      accd = c[1];
+    }
+    for (int n = 2; n < rb; ++n) {
+      const Real t2 = Real(2) * xi * t1 - t0;
+      const Real d2 = Real(2) * (t1 + xi * d1) - d0;
+      acc += c[n] * t2;
+      if (DERIV) accd += c[n] * d2;
+      t0 = t1;
+      t1 = t2;
+      d0 = d1;
+      d1 = d2;
+    }
+    R[mu] = env * acc;
+    if (DERIV) dR[mu] = denv * acc + env * dxi * accd;
+  }
+}
+
+constexpr __host__ __device__ int sh_index(int l, int m) { return l * l + m + l; }
+
+// Racah-normalised real regular solid harmonics S_lm at a unit vector u (values only):
+//   S_{l+1,l+1}  = c_l (x S_ll - y S_l,-l),  S_{l+1,-l-1} = c_l (y S_ll + x S_l,-l)
+//   S_{l+1,m}    = ((2l+1) z S_lm - sqrt((l+m)(l-m)) r^2 S_{l-1,m}) / sqrt((l+m+1)(l-m+1))
+// with c_l = sqrt(2^{delta_l0} (2l+1)/(2l+2)); Y_lm = sqrt((2l+1)/4pi) S_lm(u).  This is the
+// same basis (no Condon-Shortley phase) as the explicit table of src/force/sus2_sh.cu:3058-3168.
+template <typename Real, int L>
+__device__ __forceinline__ void solid_values(Real x, Real y, Real z, Real (&S)[(L + 1) * (L + 1)])
+{
+  S[0] = Real(1);
+  if constexpr (L >= 1) {
+    S[sh_index(1, -1)] = y;
+    S[sh_index(1, 0)] = z;
+    S[sh_index(1, 1)] = x;
+  }
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    const Real c = Real(__builtin_sqrt((2.0 * l + 1.0) / (2.0 * l + 2.0)));
+    const Real sll = S[sh_index(l, l)], slm = S[sh_index(l, -l)];
+    S[sh_index(l + 1, l + 1)] = c * (x * sll - y * slm);
+    S[sh_index(l + 1, -l - 1)] = c * (y * sll + x * slm);
+#pragma unroll
+    for (int m = -l; m <= l; ++m) {
+      const double den = __builtin_sqrt(double((l + m + 1) * (l - m + 1)));
+      const Real A = Real((2.0 * l + 1.0) / den);
+      Real v = A * z * S[sh_index(l, m)];
+      if (m > -l && m < l) {
+        const Real Bc = Real(__builtin_sqrt(double((l + m) * (l - m))) / den);
+        v -= Bc * S[sh_index(l - 1, m)];  // r^2 = 1 at the unit vector
+      }
+      S[sh_index(l + 1, m)] = v;
+    }
+  }
+}
+
+// Values and polynomial gradients dS_lm/d(x,y,z) at the unit vector u.
+template <typename Real, int L>
+__device__ __forceinline__ void solid_values_grads(Real x, Real y, Real z,
+                                                   Real (&S)[(L + 1) * (L + 1)],
+                                                   Real (&Gx)[(L + 1) * (L + 1)],
+                                                   Real (&Gy)[(L + 1) * (L + 1)],
+                                                   Real (&Gz)[(L + 1) * (L + 1)])
+{
+  S[0] = Real(1);
+  Gx[0] = Gy[0] = Gz[0] = Real(0);
+  if constexpr (L >= 1) {
+    S[sh_index(1, -1)] = y;
+    S[sh_index(1, 0)] = z;
+    S[sh_index(1, 1)] = x;
+    Gx[sh_index(1, -1)] = 0;
+    Gy[sh_index(1, -1)] = 1;
+    Gz[sh_index(1, -1)] = 0;
+    Gx[sh_index(1, 0)] = 0;
+    Gy[sh_index(1, 0)] = 0;
+    Gz[sh_index(1, 0)] = 1;
+    Gx[sh_index(1, 1)] = 1;
+    Gy[sh_index(1, 1)] = 0;
+    Gz[sh_index(1, 1)] = 0;
+  }
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    const Real c = Real(__builtin_sqrt((2.0 * l + 1.0) / (2.0 * l + 2.0)));
+    const int ip = sh_index(l, l), im = sh_index(l, -l);
+    const Real sp = S[ip], sm = S[im];
+    const int op = sh_index(l + 1, l + 1), om = sh_index(l + 1, -l - 1);
+    S[op] = c * (x * sp - y * sm);
+    Gx[op] = c * (sp + x * Gx[ip] - y * Gx[im]);
+    Gy[op] = c * (x * Gy[ip] - sm - y * Gy[im]);
+    Gz[op] = c * (x * Gz[ip] - y * Gz[im]);
+    S[om] = c * (y * sp + x * sm);
+    Gx[om] = c * (y * Gx[ip] + sm + x * Gx[im]);
+    Gy[om] = c * (sp + y * Gy[ip] + x * Gy[im]);
+    Gz[om] = c * (y * Gz[ip] + x * Gz[im]);
+#pragma unroll
+    for (int m = -l; m <= l; ++m) {
+      const double den = __builtin_sqrt(double((l + m + 1) * (l - m + 1)));
+      const Real A = Real((2.0 * l + 1.0) / den);
+      const int i0 = sh_index(l, m), o = sh_index(l + 1, m);
+      Real v = A * z * S[i0];
+      Real gx = A * z * Gx[i0];
+      Real gy = A * z * Gy[i0];
+      Real gz = A * (S[i0] + z * Gz[i0]);
+      if (m > -l && m < l) {
+        const Real Bc = Real(__builtin_sqrt(double((l + m) * (l - m))) / den);
+        const int i1 = sh_index(l - 1, m);
+        const Real s1 = S[i1];
+        v -= Bc * s1;
+        gx -= Bc * (Real(2) * x * s1 + Gx[i1]);
+        gy -= Bc * (Real(2) * y * s1 + Gy[i1]);
+        gz -= Bc * (Real(2) * z * s1 + Gz[i1]);
+      }
+      S[o] = v;
+      Gx[o] = gx;
+      Gy[o] = gy;
+      Gz[o] = gz;
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ constexpr double ylm_norm()
+{
+  return __builtin_sqrt((2.0 * L + 1.0) / (4.0 * 3.14159265358979323846264338327950288));
+}
+
+__device__ __forceinline__ double norm_of_l(int l)
+{
+  return __builtin_sqrt((2.0 * l + 1.0) / (4.0 * 3.14159265358979323846264338327950288));
+}
+
+// Full (k, l, m) grid index in the standard basic order: l = L..0, k = K-1..0, m = -l..l
+// (src/force/sus2_sh.cu:1451-1465, 2519-2551).
+template <int L, int K>
+__host__ __device__ constexpr int full_index(int l, int k, int m)
+{
+  int off = 0;
+  for (int lp = L; lp > l; --lp) off += K * (2 * lp + 1);
+  return off + (K - 1 - k) * (2 * l + 1) + (m + l);
+}
+
+}  // namespace sus2sh

@@ -1,0 +1,225 @@
+// Per-edge kernels: basic moments and forces/virial.  One lane per atom in cell-sorted
+// order, one wavefront per 64-atom tile; each lane walks its atom's neighbours and keeps
+// the full (k, l, m) accumulator grid in VGPRs, so nothing per-edge is ever stored.
+//
+//   basic moments   B_i[k,l,m] = sum_j R_{k(L+1)+l}(r_ij) Y_lm(r_ij)
+//                   (src/force/sus2_sh.cu:3486-3550, 3650-3724)
+//   forces/virial   g_ij = sum_b G_b (R'_mu r_hat Y_y + R_mu grad Y_y); F_i += g, F_j -= g,
+//                   W_i -= r_ij (x) g   (src/force/sus2_sh.cu:5996-6035, 6347-6446)
+// The backward pass recomputes R, R', Y and grad Y per edge instead of storing them.
+// Radial tables are staged in LDS (pair type varies per lane).
+#include "kernels.hpp"
+
+namespace sus2sh {
+
+namespace {
+
+template <typename Real>
+__device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, Real* s_coeff,
+                                             Real* s_scal)
+{
+  const int nc = rt.S * rt.S * Mr * rt.rb;
+  const int ns = rt.S * rt.S * Mr * 2;
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) s_coeff[i] = rt.coeff[i];
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) s_scal[i] = rt.scal[i];
+  __syncthreads();
+}
+
+template <typename Real, int L, int K>
+__global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __restrict__ out)
+{
+  constexpr int MR = K * (L + 1);
+  constexpr int NC = (L + 1) * (L + 1);
+  constexpr int FG = K * NC;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
+  Real* s_scal = s_coeff + a.rt.S * a.rt.S * MR * a.rt.rb;
+  stage_tables(a.rt, MR, s_coeff, s_scal);
+
+  const int N = a.N;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  Real acc[FG];
+#pragma unroll
+  for (int f = 0; f < FG; ++f) acc[f] = Real(0);
+  const int ti = a.type_s[p];
+  const int cnt = a.count[p];
+  const int S = a.rt.S;
+  const int rb = a.rt.rb;
+  const Real rc = a.rt.rc;
+  for (int k = 0; k < cnt; ++k) {
+    const Edge<Real> e = a.nl[static_cast<size_t>(k) * N + p];
+    const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
+    const Real r = sqrt(r2);
+    const Real inv = Real(1) / r;
+    const int pair = ti * S + a.type_s[e.j];
+    Real R[MR], dR[MR];
+    eval_radial<Real, MR, false>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+                                 a.rt.kind, a.rt.uniform, r, R, dR);
+    Real Sv[NC];
+    solid_values<Real, L>(e.dx * inv, e.dy * inv, e.dz * inv, Sv);
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+      const Real nl = Real(norm_of_l(l));
+#pragma unroll
+      for (int m = -l; m <= l; ++m) {
+        const Real y = nl * Sv[sh_index(l, m)];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) acc[full_index<L, K>(l, kk, m)] += R[kk * (L + 1) + l] * y;
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < FG; ++f) {
+    const int b = a.fmap[f];
+    if (b >= 0) out[static_cast<size_t>(b) * N + p] = acc[f];
+  }
+}
+
+template <typename Real, int L, int K>
+__global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
+                                                double* __restrict__ f_s,
+                                                double* __restrict__ v_s)
+{
+  constexpr int MR = K * (L + 1);
+  constexpr int NC = (L + 1) * (L + 1);
+  constexpr int FG = K * NC;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
+  Real* s_scal = s_coeff + a.rt.S * a.rt.S * MR * a.rt.rb;
+  stage_tables(a.rt, MR, s_coeff, s_scal);
+
+  const int N = a.N;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  Real G[FG];
+#pragma unroll
+  for (int f = 0; f < FG; ++f) {
+    const int b = a.fmap[f];
+    G[f] = b >= 0 ? grads[static_cast<size_t>(b) * N + p] : Real(0);
+  }
+  const int ti = a.type_s[p];
+  const int cnt = a.count[p];
+  const int S = a.rt.S;
+  const int rb = a.rt.rb;
+  const Real rc = a.rt.rc;
+  Real fx = 0, fy = 0, fz = 0;
+  Real w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < cnt; ++k) {
+    const Edge<Real> e = a.nl[static_cast<size_t>(k) * N + p];
+    const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
+    const Real r = sqrt(r2);
+    const Real inv = Real(1) / r;
+    const Real ux = e.dx * inv, uy = e.dy * inv, uz = e.dz * inv;
+    const int pair = ti * S + a.type_s[e.j];
+    Real R[MR], dR[MR];
+    eval_radial<Real, MR, true>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+                                a.rt.kind, a.rt.uniform, r, R, dR);
+    Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
+    solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
+    // g = u * sum (V - l U / r) Y + (1/r) sum U N_l grad S,  with U = sum_k G R, V = sum_k G R'
+    Real sv = 0, gx = 0, gy = 0, gz = 0;
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+      const Real nl = Real(norm_of_l(l));
+#pragma unroll
+      for (int m = -l; m <= l; ++m) {
+        Real U = 0, V = 0;
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+          const Real gb = G[full_index<L, K>(l, kk, m)];
+          U += gb * R[kk * (L + 1) + l];
+          V += gb * dR[kk * (L + 1) + l];
+        }
+        const int y = sh_index(l, m);
+        sv += (V - Real(l) * inv * U) * nl * Sv[y];
+        const Real un = U * nl;
+        gx += un * Gx[y];
+        gy += un * Gy[y];
+        gz += un * Gz[y];
+      }
+    }
+    const Real g0 = ux * sv + gx * inv;
+    const Real g1 = uy * sv + gy * inv;
+    const Real g2 = uz * sv + gz * inv;
+    fx += g0;
+    fy += g1;
+    fz += g2;
+    unsafeAtomicAdd(f_s + e.j, -static_cast<double>(g0));
+    unsafeAtomicAdd(f_s + N + e.j, -static_cast<double>(g1));
+    unsafeAtomicAdd(f_s + 2 * N + e.j, -static_cast<double>(g2));
+    w[0] -= e.dx * g0;
+    w[1] -= e.dy * g1;
+    w[2] -= e.dz * g2;
+    w[3] -= e.dx * g1;
+    w[4] -= e.dx * g2;
+    w[5] -= e.dy * g2;
+    w[6] -= e.dy * g0;
+    w[7] -= e.dz * g0;
+    w[8] -= e.dz * g1;
+  }
+  unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
+  unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
+  unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
+#pragma unroll
+  for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+}
+
+template <typename Real>
+size_t table_bytes(const EdgeLaunch<Real>& a)
+{
+  const int Mr = a.K * (a.L + 1);
+  return static_cast<size_t>(a.rt.S) * a.rt.S * Mr * (a.rt.rb + 2) * sizeof(Real);
+}
+
+#define SUS2SH_FOR_EACH_LK(X) \
+  X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(0, 5) X(0, 6) \
+  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) \
+  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6) \
+  X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(3, 5) X(3, 6) \
+  X(4, 1) X(4, 2) X(4, 3) X(4, 4) X(4, 5) X(4, 6)
+
+}  // namespace
+
+template <typename Real>
+bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
+{
+  if (a.N <= 0) return true;
+  const int blocks = (a.N + 255) / 256;
+  const size_t lds = table_bytes(a);
+#define SUS2SH_BASICS_CASE(LL, KK)                                           \
+  if (a.L == LL && a.K == KK) {                                              \
+    k_basics<Real, LL, KK><<<blocks, 256, lds, s>>>(a, basics);               \
+    return true;                                                             \
+  }
+  SUS2SH_FOR_EACH_LK(SUS2SH_BASICS_CASE)
+#undef SUS2SH_BASICS_CASE
+  return false;
+}
+
+template <typename Real>
+bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
+                   double* pg_rad, hipStream_t s)
+{
+  (void)pg_rad;
+  if (a.N <= 0) return true;
+  const int blocks = (a.N + 255) / 256;
+  const size_t lds = table_bytes(a);
+#define SUS2SH_FORCES_CASE(LL, KK)                                              \
+  if (a.L == LL && a.K == KK) {                                                 \
+    k_forces<Real, LL, KK><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s);         \
+    return true;                                                                \
+  }
+  SUS2SH_FOR_EACH_LK(SUS2SH_FORCES_CASE)
+#undef SUS2SH_FORCES_CASE
+  return false;
+}
+
+template bool launch_basics<float>(const EdgeLaunch<float>&, float*, hipStream_t);
+template bool launch_basics<double>(const EdgeLaunch<double>&, double*, hipStream_t);
+template bool launch_forces<float>(const EdgeLaunch<float>&, const float*, double*, double*,
+                                   double*, hipStream_t);
+template bool launch_forces<double>(const EdgeLaunch<double>&, const double*, double*, double*,
+                                    double*, hipStream_t);
+
+}  // namespace sus2sh

@@ -1,0 +1,429 @@
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+namespace sus2sh {
+
+namespace {
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool starts_with(const std::string& s, const std::string& p)
+{
+  return s.size() >= p.size() && s.compare(0, p.size(), p) == 0;
+}
+
+bool parse_bool(const std::string& v, const std::string& name)
+{
+  if (v == "1" || v == "true" || v == "TRUE" || v == "yes" || v == "on") return true;
+  if (v == "0" || v == "false" || v == "FALSE" || v == "no" || v == "off") return false;
+  throw InputError("Invalid boolean value for " + name + ": " + v);
+}
+
+// Options after the species symbols (src/force/sus2_sh.cu:520-535): sus2_float /
+// sus2_sh_float select the precision (env SUS2_GPUMD_FLOAT); the reference's other
+// sus2_* switches select CUDA micro-variants and are accepted and ignored here.
+EngineOptions parse_options(const Model& m, int nopts, const char** opts)
+{
+  EngineOptions o;
+  if (const char* env = std::getenv("SUS2_GPUMD_FLOAT")) o.use_float = parse_bool(env, "SUS2_GPUMD_FLOAT");
+  if (const char* env = std::getenv("SUS2_GPUMD_PROFILE")) o.profile = parse_bool(env, "SUS2_GPUMD_PROFILE");
+  const int begin = std::min(nopts, m.species_count);
+  for (int i = begin; i < nopts; ++i) {
+    const std::string opt = opts[i] ? opts[i] : "";
+    const size_t eq = opt.find('=');
+    const std::string key = eq == std::string::npos ? opt : opt.substr(0, eq);
+    const std::string val = eq == std::string::npos ? "" : opt.substr(eq + 1);
+    if (key == "sus2_float" || key == "sus2_sh_float") {
+      o.use_float = parse_bool(val, key);
+    } else if (key == "sus2_sh_profile") {
+      o.profile = parse_bool(val, key);
+    } else if (key == "sus2_sh_graph_tile") {
+      const int t = std::atoi(val.c_str());
+      if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
+        throw InputError("sus2_sh_graph_tile must be 1, 2, 4, 8 or 16: " + val);
+      o.graph_tile = t;
+    }
+  }
+  return o;
+}
+
+}  // namespace
+
+DeviceBuffer::~DeviceBuffer()
+{
+  if (ptr_) (void)hipFree(ptr_);
+}
+
+void DeviceBuffer::resize(size_t bytes)
+{
+  if (bytes <= bytes_ && ptr_) return;
+  if (ptr_) hip_check(hipFree(ptr_), "hipFree");
+  ptr_ = nullptr;
+  bytes_ = 0;
+  if (bytes == 0) return;
+  hip_check(hipMalloc(&ptr_, bytes), "hipMalloc");
+  bytes_ = bytes;
+}
+
+void DeviceBuffer::upload(const void* host, size_t bytes)
+{
+  resize(std::max<size_t>(bytes, 16));
+  if (bytes) hip_check(hipMemcpy(ptr_, host, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** opts)
+{
+  model_ = load_model(path);
+  if (model_.radial_kind == RadialKind::LaguerreLog1p)
+    throw UnsupportedError("SUS2_SH: RBLaguerre_log1p radial basis is not supported by this engine.");
+  if (model_.zbl_enabled)
+    throw UnsupportedError("SUS2_SH: ZBL-enabled models are not supported by this engine.");
+  if (model_.l_max > kMaxL)
+    throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 4 only.");
+  if (model_.k_max > kMaxK) throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 6.");
+  if (model_.species_count > kMaxSpecies)
+    throw UnsupportedError("SUS2_SH: this engine supports at most 16 species.");
+  opt_ = parse_options(model_, nopts, opts);
+  prog_ = compile_graph(model_);
+
+  hip_check(hipGetDevice(&device_), "hipGetDevice");
+  int lds = 0;
+  hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_),
+            "hipDeviceGetAttribute");
+  max_lds_ = static_cast<size_t>(lds);
+
+  const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
+                       model_.radial_funcs_count * (model_.rb_size + 2) * rsz;
+  if (table > 65536)
+    throw UnsupportedError("SUS2_SH: radial tables exceed 64 KiB of LDS (too many species).");
+
+  // graph tile: largest power of two whose LDS footprint leaves room for 2 workgroups/CU
+  const auto lds_of = [&](int t) {
+    return opt_.use_float ? graph_lds_bytes<float>(t, prog_.slot_count)
+                          : graph_lds_bytes<double>(t, prog_.slot_count);
+  };
+  if (opt_.graph_tile > 0) {
+    tile_ = opt_.graph_tile;
+    if (lds_of(tile_) > max_lds_)
+      throw InputError("sus2_sh_graph_tile too large for this model's on-chip graph state.");
+  } else {
+    const size_t budget = std::min<size_t>(max_lds_, 64 * 1024);
+    tile_ = 1;
+    for (int t : {16, 8, 4, 2, 1})
+      if (lds_of(t) <= budget) {
+        tile_ = t;
+        break;
+      }
+    if (lds_of(tile_) > max_lds_)
+      throw UnsupportedError("SUS2_SH: graph state of one atom exceeds the LDS of a workgroup.");
+  }
+
+  if (opt_.use_float)
+    upload_tables<float>();
+  else
+    upload_tables<double>();
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_max_), 4 * sizeof(int), 0),
+            "hipHostMalloc");
+  (void)num_atoms;
+}
+
+Engine::~Engine()
+{
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (host_max_) (void)hipHostFree(host_max_);
+}
+
+int Engine::param_count() const
+{
+  const int S = model_.species_count;
+  return 2 * S + model_.scalar_count + model_.radial_funcs_count * model_.rb_size + S +
+         2 * S * S * model_.radial_funcs_count;
+}
+
+template <typename Real>
+void Engine::upload_tables()
+{
+  const Model& m = model_;
+  const GraphProgram& g = prog_;
+  std::vector<double> coeff, scal;
+  build_radial_tables(m, coeff, scal);
+  std::vector<Real> cR(coeff.begin(), coeff.end()), sR(scal.begin(), scal.end());
+  radial_coeff_.upload(cR.data(), cR.size() * sizeof(Real));
+  radial_scal_.upload(sR.data(), sR.size() * sizeof(Real));
+  radial_uniform_ = true;
+  const int pairs = m.species_count * m.species_count;
+  for (int p = 0; p < pairs && radial_uniform_; ++p)
+    for (int mu = 1; mu < m.radial_funcs_count; ++mu)
+      if (scal[(p * m.radial_funcs_count + mu) * 2] != scal[(p * m.radial_funcs_count) * 2] ||
+          scal[(p * m.radial_funcs_count + mu) * 2 + 1] != scal[(p * m.radial_funcs_count) * 2 + 1]) {
+        radial_uniform_ = false;
+        break;
+      }
+
+  // full (k,l,m) grid -> basic index
+  const int L = m.l_max, K = m.k_max;
+  std::vector<int> fmap(K * (L + 1) * (L + 1), -1);
+  for (int b = 0; b < m.basic_count; ++b) {
+    int off = 0;
+    for (int lp = L; lp > m.basic_l[b]; --lp) off += K * (2 * lp + 1);
+    const int f = off + (K - 1 - m.basic_k[b]) * (2 * m.basic_l[b] + 1) + m.basic_m[b] + m.basic_l[b];
+    fmap[f] = b;
+  }
+  fmap_.upload(fmap.data(), fmap.size() * sizeof(int));
+
+  const int rows = static_cast<int>(g.fwd_row_target.size());
+  std::vector<int4> frows(rows);
+  std::vector<Real> fw(rows);
+  for (int r = 0; r < rows; ++r) {
+    frows[r] = make_int4(g.fwd_row_target[r], g.fwd_row_begin[r], g.fwd_row_count[r], 0);
+    fw[r] = static_cast<Real>(g.fwd_row_weight[r]);
+  }
+  fwd_rows_.upload(frows.data(), frows.size() * sizeof(int4));
+  fwd_w_.upload(fw.data(), fw.size() * sizeof(Real));
+  std::vector<Term<Real>> ft(g.fwd_term_idx.size());
+  for (size_t t = 0; t < ft.size(); ++t) {
+    ft[t] = Term<Real>{};
+    ft[t].idx = g.fwd_term_idx[t];
+    ft[t].c = static_cast<Real>(g.fwd_term_coeff[t]);
+  }
+  fwd_terms_.upload(ft.data(), ft.size() * sizeof(Term<Real>));
+  fwd_layer_.upload(g.fwd_layer_begin.data(), g.fwd_layer_begin.size() * sizeof(int));
+
+  const int chunks = static_cast<int>(g.bwd_chunk_src.size());
+  std::vector<int4> bch(chunks);
+  for (int c = 0; c < chunks; ++c)
+    bch[c] = make_int4(g.bwd_chunk_src[c], g.bwd_chunk_begin[c], g.bwd_chunk_count[c],
+                       g.bwd_chunk_atomic[c]);
+  bwd_chunks_.upload(bch.data(), bch.size() * sizeof(int4));
+  std::vector<Term<Real>> bt(g.bwd_term_idx.size());
+  for (size_t t = 0; t < bt.size(); ++t) {
+    bt[t] = Term<Real>{};
+    bt[t].idx = g.bwd_term_idx[t];
+    bt[t].c = static_cast<Real>(g.bwd_term_coeff[t]);
+  }
+  bwd_terms_.upload(bt.data(), bt.size() * sizeof(Term<Real>));
+  bwd_layer_.upload(g.bwd_layer_begin.data(), g.bwd_layer_begin.size() * sizeof(int));
+
+  e_slot_.upload(g.energy_slot.data(), g.energy_slot.size() * sizeof(int));
+  std::vector<Real> ew(g.energy_weight.begin(), g.energy_weight.end());
+  e_w_.upload(ew.data(), ew.size() * sizeof(Real));
+  std::vector<Real> seed(g.seed.begin(), g.seed.end());
+  seed_.upload(seed.data(), seed.size() * sizeof(Real));
+  shift_.upload(m.shift_coeffs.data(), m.shift_coeffs.size() * sizeof(double));
+  sigma_.upload(m.species_coeffs.data(), m.species_coeffs.size() * sizeof(double));
+  scalar_slot_.upload(g.scalar_slot.data(), g.scalar_slot.size() * sizeof(int));
+  scalar_row_.upload(g.scalar_row.data(), g.scalar_row.size() * sizeof(int));
+}
+
+void Engine::ensure_atoms(int N, size_t edge_bytes)
+{
+  const size_t n = static_cast<size_t>(N);
+  const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  if (N > atoms_alloc_) {
+    key_.resize(n * sizeof(int));
+    val_.resize(n * sizeof(int));
+    skey_.resize(n * sizeof(int));
+    satom_.resize(n * sizeof(int));
+    wrapped_.resize(3 * n * sizeof(double));
+    pos_s_.resize(3 * n * sizeof(double));
+    type_s_.resize(n * sizeof(int));
+    count_.resize(n * sizeof(int));
+    basics_.resize(static_cast<size_t>(model_.basic_count) * n * rsz);
+    grads_.resize(static_cast<size_t>(model_.basic_count) * n * rsz);
+    e_s_.resize(n * sizeof(double));
+    f_s_.resize(3 * n * sizeof(double));
+    v_s_.resize(9 * n * sizeof(double));
+    atoms_alloc_ = N;
+  }
+  max_count_.resize(64);
+  nl_.resize(static_cast<size_t>(cap_) * n * edge_bytes);
+}
+
+CellGrid Engine::make_grid(const sus2sh_box& box, int N, int* ncell) const
+{
+  CellGrid g{};
+  for (int k = 0; k < 9; ++k) {
+    g.h[k] = box.h[k];
+    g.hi[k] = box.h[9 + k];
+  }
+  const double* h = box.h;
+  const double a[3] = {h[0], h[3], h[6]}, b[3] = {h[1], h[4], h[7]}, c[3] = {h[2], h[5], h[8]};
+  const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+  const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+  const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double vol = std::fabs(a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2]);
+  if (!(vol > 0.0) || !std::isfinite(vol)) throw ArgumentError("box matrix is singular");
+  const double t[3] = {vol / std::sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]),
+                       vol / std::sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]),
+                       vol / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2])};
+  const double rc = model_.max_dist;
+  long total = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.pbc[d] = box.pbc[d] ? 1 : 0;
+    g.n[d] = std::max(1, static_cast<int>(std::floor(t[d] / rc)));
+    g.n[d] = std::min(g.n[d], 1024);
+    total *= g.n[d];
+  }
+  // keep the cell count bounded by the atom count (sparse systems)
+  const long limit = std::max<long>(27, 4L * std::max(N, 1));
+  while (total > limit) {
+    int dmax = 0;
+    for (int d = 1; d < 3; ++d)
+      if (g.n[d] > g.n[dmax]) dmax = d;
+    g.n[dmax] = std::max(1, g.n[dmax] / 2);
+    total = static_cast<long>(g.n[0]) * g.n[1] * g.n[2];
+  }
+  for (int d = 0; d < 3; ++d)
+    g.R[d] = std::max(1, static_cast<int>(std::ceil(rc * g.n[d] / t[d] - 1e-12)));
+  *ncell = static_cast<int>(total);
+  return g;
+}
+
+void Engine::compute(const sus2sh_box& box, int N, const int* type, const double* pos,
+                     double* pot, double* force, double* virial, hipStream_t stream,
+                     double* param_grad)
+{
+  if (N < 0) throw ArgumentError("num_atoms must be >= 0");
+  if (N == 0) return;
+  if (!type || !pos || !pot || !force || !virial)
+    throw ArgumentError("null device pointer passed to compute");
+  if (opt_.use_float)
+    run<float>(box, N, type, pos, pot, force, virial, stream, param_grad);
+  else
+    run<double>(box, N, type, pos, pot, force, virial, stream, param_grad);
+}
+
+template <typename Real>
+void Engine::run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
+                 double* force, double* virial, hipStream_t s, double* param_grad)
+{
+  if (param_grad)
+    throw UnsupportedError("SUS2_SH: parameter gradients are not available in this build.");
+  int ncell = 0;
+  const CellGrid grid = make_grid(box, N, &ncell);
+  if (cap_ == 0) cap_ = 64;
+  ensure_atoms(N, sizeof(Edge<Real>));
+  cell_begin_.resize(static_cast<size_t>(ncell) * sizeof(int));
+  cell_end_.resize(static_cast<size_t>(ncell) * sizeof(int));
+  int bits = 1;
+  while ((1L << bits) < ncell) ++bits;
+  const size_t temp = radix_sort_temp_bytes(N, bits);
+  sort_temp_.resize(std::max<size_t>(temp, 16));
+
+  if (opt_.profile && events_.empty()) {
+    events_.resize(6);
+    for (hipEvent_t& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  if (opt_.profile) hip_check(hipEventRecord(events_[0], s), "hipEventRecord");
+
+  launch_bin(N, pos, grid, key_.as<int>(), val_.as<int>(), wrapped_.as<double>(), s);
+  hip_check(radix_sort_pairs(sort_temp_.as<void>(), temp, key_.as<int>(), skey_.as<int>(),
+                             val_.as<int>(), satom_.as<int>(), N, bits, s),
+            "radix sort");
+  launch_fill_int(cell_begin_.as<int>(), 0, ncell, s);
+  launch_fill_int(cell_end_.as<int>(), 0, ncell, s);
+  launch_fill_int(max_count_.as<int>(), 0, 4, s);
+  launch_gather(N, model_.species_count, skey_.as<int>(), satom_.as<int>(), wrapped_.as<double>(),
+                type, pos_s_.as<double>(), type_s_.as<int>(), cell_begin_.as<int>(),
+                cell_end_.as<int>(), max_count_.as<int>() + 1, s);
+  NeighborArgs na{};
+  na.N = N;
+  na.grid = grid;
+  na.rc2 = model_.max_dist * model_.max_dist;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    na.cap = cap_;
+    if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
+    launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
+    launch_neighbors<Real>(na, pos_s_.as<double>(), skey_.as<int>(), cell_begin_.as<int>(),
+                           cell_end_.as<int>(), count_.as<int>(), nl_.as<Edge<Real>>(),
+                           max_count_.as<int>(), s);
+    hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const int maxc = host_max_[0];
+    const unsigned long long total =
+      static_cast<unsigned long long>(static_cast<unsigned>(host_max_[2])) |
+      (static_cast<unsigned long long>(static_cast<unsigned>(host_max_[3])) << 32);
+    mean_neighbors_ = static_cast<double>(total) / N;
+    if (host_max_[1] != 0)
+      throw ArgumentError("atom type outside [0, species_count) passed to compute");
+    if (maxc <= cap_) break;
+    cap_ = ((maxc + 8 + 7) / 8) * 8;
+    ensure_atoms(N, sizeof(Edge<Real>));
+  }
+  if (opt_.profile) hip_check(hipEventRecord(events_[1], s), "hipEventRecord");
+
+  EdgeLaunch<Real> el{};
+  el.N = N;
+  el.L = model_.l_max;
+  el.K = model_.k_max;
+  el.B = model_.basic_count;
+  el.count = count_.as<int>();
+  el.nl = nl_.as<Edge<Real>>();
+  el.type_s = type_s_.as<int>();
+  el.rt.coeff = radial_coeff_.as<Real>();
+  el.rt.scal = radial_scal_.as<Real>();
+  el.rt.S = model_.species_count;
+  el.rt.rb = model_.rb_size;
+  el.rt.rc = static_cast<Real>(model_.max_dist);
+  el.rt.kind = model_.radial_kind == RadialKind::ChebyshevSSSRational ? 1 : 0;
+  el.rt.uniform = radial_uniform_ ? 1 : 0;
+  el.fmap = fmap_.as<int>();
+  if (!launch_basics<Real>(el, basics_.as<Real>(), s))
+    throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
+  if (opt_.profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
+
+  GraphTable<Real> gt{};
+  gt.fwd_rows = fwd_rows_.as<int4>();
+  gt.fwd_w = fwd_w_.as<Real>();
+  gt.fwd_terms = fwd_terms_.as<Term<Real>>();
+  gt.fwd_layer = fwd_layer_.as<int>();
+  gt.bwd_chunks = bwd_chunks_.as<int4>();
+  gt.bwd_terms = bwd_terms_.as<Term<Real>>();
+  gt.bwd_layer = bwd_layer_.as<int>();
+  gt.e_slot = e_slot_.as<int>();
+  gt.e_w = e_w_.as<Real>();
+  gt.seed = seed_.as<Real>();
+  gt.shift = shift_.as<double>();
+  gt.sigma = sigma_.as<double>();
+  gt.n_e = static_cast<int>(prog_.energy_slot.size());
+  gt.slots = prog_.slot_count;
+  gt.basics = model_.basic_count;
+  gt.layers = prog_.layer_count;
+  if (!launch_graph<Real>(tile_, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
+                          grads_.as<Real>(), nullptr, s, max_lds_))
+    throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
+  if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
+
+  launch_fill_double(f_s_.as<double>(), 0.0, 3 * static_cast<size_t>(N), s);
+  if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(), nullptr, s))
+    throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+  if (opt_.profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
+
+  launch_scatter(N, satom_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
+                 pot, force, virial, s);
+  hip_check(hipGetLastError(), "kernel launch");
+  if (opt_.profile) {
+    hip_check(hipEventRecord(events_[5], s), "hipEventRecord");
+    hip_check(hipEventSynchronize(events_[5]), "hipEventSynchronize");
+    stage_ms_.assign(5, 0.0f);
+    for (int k = 0; k < 5; ++k)
+      hip_check(hipEventElapsedTime(&stage_ms_[k], events_[k], events_[k + 1]),
+                "hipEventElapsedTime");
+  }
+}
+
+template void Engine::upload_tables<float>();
+template void Engine::upload_tables<double>();
+
+}  // namespace sus2sh

@@ -1,0 +1,117 @@
+// Host engine behind the C ABI: owns the model, its device tables and the per-atom work
+// buffers, and runs one evaluation as a fixed kernel sequence on the caller's stream:
+//   bin -> radix sort -> cell ranges -> neighbours -> basic moments -> CG graph
+//   -> forces/virial -> scatter to caller order.
+// Mirrors SUS2_SH (src/force/sus2_sh.cuh:7-203): constructor = model load + table upload
+// (src/force/sus2_sh.cu:7782-9467), compute = src/force/sus2_sh.cu:9620-10675.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sus2sh.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+#include "program.hpp"
+
+namespace sus2sh {
+
+struct DeviceError : public std::runtime_error {
+  explicit DeviceError(const std::string& msg) : std::runtime_error(msg) {}
+};
+
+struct UnsupportedError : public std::runtime_error {
+  explicit UnsupportedError(const std::string& msg) : std::runtime_error(msg) {}
+};
+
+struct ArgumentError : public std::runtime_error {
+  explicit ArgumentError(const std::string& msg) : std::runtime_error(msg) {}
+};
+
+// Owning device allocation.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  void resize(size_t bytes);  // keeps nothing; grows only
+  void upload(const void* host, size_t bytes);
+  template <typename T>
+  T* as() const
+  {
+    return static_cast<T*>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+struct EngineOptions {
+  bool use_float = true;
+  int graph_tile = 0;  // 0: automatic
+  bool profile = false;
+};
+
+class Engine {
+ public:
+  Engine(const std::string& path, int num_atoms, int nopts, const char** opts);
+  ~Engine();
+
+  void compute(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
+               double* force, double* virial, hipStream_t stream, double* param_grad);
+
+  const Model& model() const { return model_; }
+  const GraphProgram& program() const { return prog_; }
+  const EngineOptions& options() const { return opt_; }
+  int graph_tile() const { return tile_; }
+  int neighbor_capacity() const { return cap_; }
+  int param_count() const;
+  const std::vector<float>& stage_ms() const { return stage_ms_; }
+  double mean_neighbors() const { return mean_neighbors_; }
+
+ private:
+  template <typename Real>
+  void upload_tables();
+  template <typename Real>
+  void run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
+           double* force, double* virial, hipStream_t stream, double* param_grad);
+  void ensure_atoms(int N, size_t edge_bytes);
+  CellGrid make_grid(const sus2sh_box& box, int N, int* ncell) const;
+
+  Model model_;
+  GraphProgram prog_;
+  EngineOptions opt_;
+  int tile_ = 1;
+  int cap_ = 0;
+  size_t max_lds_ = 65536;
+  int device_ = 0;
+
+  // model tables (precision-specific ones are uploaded for the active precision only)
+  DeviceBuffer radial_coeff_, radial_scal_, fmap_;
+  DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
+  DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;
+  DeviceBuffer e_slot_, e_w_, seed_, shift_, sigma_;
+  DeviceBuffer scalar_slot_, scalar_row_;
+  bool radial_uniform_ = false;
+
+  // per-atom work buffers
+  int atoms_alloc_ = 0;
+  DeviceBuffer key_, val_, skey_, satom_, wrapped_, pos_s_, type_s_, count_;
+  DeviceBuffer cell_begin_, cell_end_, sort_temp_, max_count_;
+  DeviceBuffer nl_, basics_, grads_, e_s_, f_s_, v_s_;
+  DeviceBuffer pg_rad_, pg_misc_;
+  int* host_max_ = nullptr;  // pinned
+
+  std::vector<float> stage_ms_;
+  std::vector<hipEvent_t> events_;
+  double mean_neighbors_ = 0.0;
+};
+
+}  // namespace sus2sh

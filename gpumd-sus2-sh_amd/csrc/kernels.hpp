@@ -1,0 +1,107 @@
+// Kernel launch interface of the SUS2-SH engine.  Implementations live in
+// neighbor.hip (cell list), edge_kernels.hip (basic moments, forces/virial),
+// graph_kernel.hip (CG product graph forward / energy / reverse).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace sus2sh {
+
+// Graph program tables on the device.
+template <typename Real>
+struct GraphTable {
+  const int4* fwd_rows;   // (target slot or -1, term begin, term count, 0)
+  const Real* fwd_w;      // energy weight of terminal rows
+  const Term<Real>* fwd_terms;
+  const int* fwd_layer;   // [layers + 1]
+  const int4* bwd_chunks; // (src slot, term begin, term count, atomic)
+  const Term<Real>* bwd_terms;
+  const int* bwd_layer;   // [layers + 1]
+  const int* e_slot;
+  const Real* e_w;
+  const Real* seed;       // [slots]
+  const double* shift;    // [S]
+  const double* sigma;    // [S]
+  int n_e;
+  int slots;
+  int basics;
+  int layers;
+};
+
+// Optional outputs for the parameter gradient.
+struct ParamGradOut {
+  double* d_species;   // [S]  += per-type sum of (1 + E') (nullptr: off)
+  double* d_shift;     // [S]  += per-type atom count
+  double* d_moment;    // [Q]  += sum_i sigma_i M_i[map[s]]
+  const int* scalar_slot;  // [Q]
+  const int* scalar_row;   // [Q]
+  int Q;
+};
+
+struct NeighborArgs {
+  int N;
+  CellGrid grid;
+  double rc2;
+  int cap;
+};
+
+// --- neighbour build (neighbor.hip) ---
+void launch_bin(int N, const double* pos, const CellGrid& g, int* cell_key, int* atom_val,
+                double* wrapped, hipStream_t s);
+void launch_fill_int(int* p, int value, size_t n, hipStream_t s);
+void launch_fill_double(double* p, double value, size_t n, hipStream_t s);
+// bad_type: set to 1 when a type is outside [0, S) (clamped to 0 for the computation)
+void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
+                   const double* wrapped, const int* type, double* pos_s, int* type_s,
+                   int* cell_begin, int* cell_end, int* bad_type, hipStream_t s);
+template <typename Real>
+void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sorted_key,
+                      const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,
+                      int* max_count, hipStream_t s);
+size_t radix_sort_temp_bytes(int N, int bits);
+hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
+                            const int* vals_in, int* vals_out, int N, int bits, hipStream_t s);
+
+// --- edge kernels (edge_kernels.hip) ---
+template <typename Real>
+struct EdgeLaunch {
+  int N;
+  int L;
+  int K;
+  int B;
+  const int* count;
+  const Edge<Real>* nl;
+  const int* type_s;
+  RadialTable<Real> rt;
+  const int* fmap;  // [K (L+1)^2] full-grid index -> basic index, or -1
+};
+
+template <typename Real>
+bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);
+
+// force/virial: grads[B][N] -> f_s[3][N] (atomic, double) and v_s[9][N].
+// pg_rad (optional): accumulates dE/d(radial coeff, type coeff, scal) per pair into
+// pg_rad[S*S][Mr*rb + Mr*2] (double, atomic).
+template <typename Real>
+bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
+                   double* pg_rad, hipStream_t s);
+
+// --- graph kernel (graph_kernel.hip) ---
+template <typename Real>
+bool launch_graph(int tile, int N, const Real* basics, const int* type_s,
+                  const GraphTable<Real>& gt, double* e_s, Real* grads, const ParamGradOut* pg,
+                  hipStream_t s, size_t max_lds);
+
+template <typename Real>
+size_t graph_lds_bytes(int tile, int slots);
+
+// --- scatter of per-sorted-atom results to the caller's arrays ---
+void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,
+                    const double* v_s, double* potential, double* force, double* virial,
+                    hipStream_t s);
+
+}  // namespace sus2sh

@@ -1,0 +1,251 @@
+// Cell-list neighbour build over all periodic images within rc, and the final scatter of
+// per-atom results.  Replaces SUS2_SH::build_neighbor_list (src/force/sus2_sh.cu:9547-9618):
+// the reference uses GPUMD's cell list when every periodic thickness is >= 2 rc and an
+// O(N^2) multi-image search otherwise (sus2_sh.cu:7668-7767).  One cell search with image
+// shifts covers both cases: with n_d = floor(t_d / rc) cells along a direction of thickness
+// t_d, every image within rc lies within R_d = ceil(rc n_d / t_d) cells of the home cell,
+// and a small cell (n_d < 2 R_d + 1) is simply visited once per image shift.
+//
+// Atoms are sorted by cell (hipCUB radix sort, deterministic), so every later kernel runs
+// in cell order: neighbours of a wavefront's atoms sit in the same few cache lines.
+#include <hipcub/hipcub.hpp>
+
+#include "kernels.hpp"
+
+namespace sus2sh {
+
+namespace {
+
+__global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g, int* __restrict__ key,
+                      int* __restrict__ val, double* __restrict__ wrapped)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double x = pos[i], y = pos[N + i], z = pos[2 * N + i];
+  double s[3] = {g.hi[0] * x + g.hi[1] * y + g.hi[2] * z, g.hi[3] * x + g.hi[4] * y + g.hi[5] * z,
+                 g.hi[6] * x + g.hi[7] * y + g.hi[8] * z};
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (g.pbc[d]) {
+      s[d] -= floor(s[d]);
+      if (s[d] >= 1.0) s[d] = 0.0;
+    }
+    int cd = static_cast<int>(floor(s[d] * g.n[d]));
+    cd = cd < 0 ? 0 : (cd >= g.n[d] ? g.n[d] - 1 : cd);
+    c[d] = cd;
+  }
+  wrapped[i] = g.h[0] * s[0] + g.h[1] * s[1] + g.h[2] * s[2];
+  wrapped[N + i] = g.h[3] * s[0] + g.h[4] * s[1] + g.h[5] * s[2];
+  wrapped[2 * N + i] = g.h[6] * s[0] + g.h[7] * s[1] + g.h[8] * s[2];
+  key[i] = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+  val[i] = i;
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int v, size_t n)
+{
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_double(double* __restrict__ p, double v, size_t n)
+{
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_gather(int N, int S, const int* __restrict__ skey, const int* __restrict__ satom,
+                         const double* __restrict__ wrapped, const int* __restrict__ type,
+                         double* __restrict__ pos_s, int* __restrict__ type_s,
+                         int* __restrict__ cbeg, int* __restrict__ cend, int* __restrict__ bad_type)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int i = satom[p];
+  pos_s[p] = wrapped[i];
+  pos_s[N + p] = wrapped[N + i];
+  pos_s[2 * N + p] = wrapped[2 * N + i];
+  int t = type[i];
+  if (t < 0 || t >= S) {  // reported to the host; clamped so that no table is overrun
+    atomicOr(bad_type, 1);
+    t = 0;
+  }
+  type_s[p] = t;
+  const int k = skey[p];
+  if (p == 0 || skey[p - 1] != k) cbeg[k] = p;
+  if (p == N - 1 || skey[p + 1] != k) cend[k] = p + 1;
+}
+
+__device__ __forceinline__ int wrap_cell(int c, int n, int& shift)
+{
+  // floor division for possibly negative c
+  shift = (c >= 0) ? c / n : -((-c + n - 1) / n);
+  return c - shift * n;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double* __restrict__ pos,
+                                                   const int* __restrict__ skey,
+                                                   const int* __restrict__ cbeg,
+                                                   const int* __restrict__ cend,
+                                                   int* __restrict__ count,
+                                                   Edge<Real>* __restrict__ nl,
+                                                   int* __restrict__ max_count)
+{
+  const int N = a.N;
+  const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = p0 < N;
+  const int p = valid ? p0 : N - 1;  // inactive lanes repeat the last atom, store nothing
+  const CellGrid& g = a.grid;
+  const int key = skey[p];
+  const int cx = key % g.n[0];
+  const int cy = (key / g.n[0]) % g.n[1];
+  const int cz = key / (g.n[0] * g.n[1]);
+  const double xp = pos[p], yp = pos[N + p], zp = pos[2 * N + p];
+  int c = 0;
+  for (int oz = -g.R[2]; oz <= g.R[2]; ++oz) {
+    int sz = 0, z = cz + oz;
+    if (g.pbc[2]) {
+      z = wrap_cell(z, g.n[2], sz);
+    } else if (z < 0 || z >= g.n[2]) {
+      continue;
+    }
+    for (int oy = -g.R[1]; oy <= g.R[1]; ++oy) {
+      int sy = 0, y = cy + oy;
+      if (g.pbc[1]) {
+        y = wrap_cell(y, g.n[1], sy);
+      } else if (y < 0 || y >= g.n[1]) {
+        continue;
+      }
+      for (int ox = -g.R[0]; ox <= g.R[0]; ++ox) {
+        int sx = 0, x = cx + ox;
+        if (g.pbc[0]) {
+          x = wrap_cell(x, g.n[0], sx);
+        } else if (x < 0 || x >= g.n[0]) {
+          continue;
+        }
+        const int cc = (z * g.n[1] + y) * g.n[0] + x;
+        const double shx = sx * g.h[0] + sy * g.h[1] + sz * g.h[2];
+        const double shy = sx * g.h[3] + sy * g.h[4] + sz * g.h[5];
+        const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
+        const bool self_image = (sx | sy | sz) == 0;
+        const int qe = cend[cc];
+        for (int q = cbeg[cc]; q < qe; ++q) {
+          if (self_image && q == p) continue;
+          const double dx = pos[q] - xp + shx;
+          const double dy = pos[N + q] - yp + shy;
+          const double dz = pos[2 * N + q] - zp + shz;
+          if (dx * dx + dy * dy + dz * dz < a.rc2) {
+            if (valid && c < a.cap) {
+              Edge<Real> e;
+              e.dx = static_cast<Real>(dx);
+              e.dy = static_cast<Real>(dy);
+              e.dz = static_cast<Real>(dz);
+              e.j = q;
+              nl[static_cast<size_t>(c) * N + p] = e;
+            }
+            ++c;
+          }
+        }
+      }
+    }
+  }
+  if (valid) count[p] = c < a.cap ? c : a.cap;
+  // wavefront reduction, then one atomic per wave: max count and total edge count
+  int cmax = valid ? c : 0;
+  unsigned long long csum = valid ? static_cast<unsigned long long>(c) : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cmax = max(cmax, __shfl_xor(cmax, off, 64));
+    csum += __shfl_xor(csum, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(max_count, cmax);
+    atomicAdd(reinterpret_cast<unsigned long long*>(max_count + 2), csum);
+  }
+}
+
+__global__ void k_scatter(int N, const int* __restrict__ satom, const double* __restrict__ e_s,
+                          const double* __restrict__ f_s, const double* __restrict__ v_s,
+                          double* __restrict__ pot, double* __restrict__ force,
+                          double* __restrict__ vir)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int i = satom[p];
+  pot[i] += e_s[p];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) force[static_cast<size_t>(c) * N + i] += f_s[static_cast<size_t>(c) * N + p];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) vir[static_cast<size_t>(c) * N + i] += v_s[static_cast<size_t>(c) * N + p];
+}
+
+inline int blocks_for(size_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_bin(int N, const double* pos, const CellGrid& g, int* cell_key, int* atom_val,
+                double* wrapped, hipStream_t s)
+{
+  k_bin<<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, cell_key, atom_val, wrapped);
+}
+
+void launch_fill_int(int* p, int value, size_t n, hipStream_t s)
+{
+  if (n == 0) return;
+  k_fill_int<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
+}
+
+void launch_fill_double(double* p, double value, size_t n, hipStream_t s)
+{
+  if (n == 0) return;
+  k_fill_double<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
+}
+
+void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
+                   const double* wrapped, const int* type, double* pos_s, int* type_s,
+                   int* cell_begin, int* cell_end, int* bad_type, hipStream_t s)
+{
+  k_gather<<<blocks_for(N, 256), 256, 0, s>>>(N, S, sorted_key, sorted_atom, wrapped, type, pos_s,
+                                               type_s, cell_begin, cell_end, bad_type);
+}
+
+template <typename Real>
+void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sorted_key,
+                      const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,
+                      int* max_count, hipStream_t s)
+{
+  k_neighbors<Real><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
+                                                         cell_end, count, nl, max_count);
+}
+
+template void launch_neighbors<float>(const NeighborArgs&, const double*, const int*, const int*,
+                                      const int*, int*, Edge<float>*, int*, hipStream_t);
+template void launch_neighbors<double>(const NeighborArgs&, const double*, const int*, const int*,
+                                       const int*, int*, Edge<double>*, int*, hipStream_t);
+
+size_t radix_sort_temp_bytes(int N, int bits)
+{
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const int*>(nullptr),
+                                     static_cast<int*>(nullptr), static_cast<const int*>(nullptr),
+                                     static_cast<int*>(nullptr), N, 0, bits);
+  return bytes;
+}
+
+hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
+                            const int* vals_in, int* vals_out, int N, int bits, hipStream_t s)
+{
+  return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, N, 0,
+                                     bits, s);
+}
+
+void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,
+                    const double* v_s, double* potential, double* force, double* virial,
+                    hipStream_t s)
+{
+  k_scatter<<<blocks_for(N, 256), 256, 0, s>>>(N, sorted_atom, e_s, f_s, v_s, potential, force,
+                                                virial);
+}
+
+}  // namespace sus2sh

@@ -1,0 +1,35 @@
+"""MI355X-native SUS2-SH potential engine (host-side Python mirror of the C ABI).
+
+The compute path is libsus2sh.so (HIP kernels for gfx950); this package only binds it.
+"""
+from ._capi import EXPORTED_SYMBOLS, LIB_PATH, Box, Sus2shError, lib, last_error  # noqa: F401
+from .model_gen import (ModelSpec, StandardGraph, param_vector, standard_graph,  # noqa: F401
+                        synthetic_model, with_params, write_mtp)
+from .potential import SUS2_SH, box_struct, evaluate  # noqa: F401
+from . import structures  # noqa: F401
+
+
+def probe(path):
+    """1 when `path` is a SUS2-SH `.mtp` file (first token MTP, potential_tag = SUS2-SH)."""
+    return bool(lib().sus2sh_probe(str(path).encode()))
+
+
+def clebsch_gordan(j1, m1, j2, m2, j, m):
+    return lib().sus2sh_clebsch_gordan(j1, m1, j2, m2, j, m)
+
+
+def real_cg(l1, m1, l2, m2, L, M):
+    return lib().sus2sh_real_cg(l1, m1, l2, m2, L, M)
+
+
+def model_check(path):
+    """Host-side load + validation; returns a dict of the parsed dimensions."""
+    import ctypes
+    from ._capi import check
+    dims = (ctypes.c_int * 16)()
+    check(lib().sus2sh_model_check(str(path).encode(), dims))
+    keys = ("species_count", "l_max", "k_max", "body_order", "radial_basis_size",
+            "radial_funcs_count", "moments_count", "basic_count", "scalar_count",
+            "product_count", "layer_count", "row_count", "row_term_count", "back_row_count",
+            "back_term_count", "standard_graph")
+    return dict(zip(keys, list(dims)))

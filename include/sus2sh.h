@@ -1,0 +1,154 @@
+/*
+ * sus2sh.h -- C ABI of the MI355X SUS2-SH potential engine.
+ *
+ * Drop-in boundary for the SUS2-SH backend of GPUMD-SUS2-SH.  Every entry point names
+ * the reference interface it replaces.  All device pointers are plain HIP device
+ * pointers; `stream` is a hipStream_t (NULL = the default stream).  No entry point
+ * exits the process: failures return a non-zero status and sus2sh_last_error() holds
+ * the message (the reference prints and calls exit(1), src/force/sus2_sh.cu:269-273).
+ */
+#ifndef SUS2SH_H
+#define SUS2SH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUS2SH_OK 0
+#define SUS2SH_ERR_INPUT 1       /* model file or option error */
+#define SUS2SH_ERR_ARGUMENT 2    /* invalid argument to an entry point */
+#define SUS2SH_ERR_DEVICE 3      /* HIP runtime or kernel launch failure */
+#define SUS2SH_ERR_UNSUPPORTED 4 /* valid model that this engine does not evaluate */
+
+typedef struct sus2sh_handle sus2sh_handle;
+
+/* GPUMD Box: h[0..8] is the row-major 3x3 matrix whose COLUMNS are the cell vectors
+ * a, b, c (x = h0 sa + h1 sb + h2 sc); h[9..17] is its inverse (row-major).
+ * pbc[d] = 1 for a periodic direction.
+ * Reference: GPUMD Box::cpu_h / pbc_x,y,z, used at src/force/sus2_sh.cu:7701-7703 and
+ * src/force/force.cu:439-465. */
+typedef struct sus2sh_box {
+  double h[18];
+  int pbc[3];
+} sus2sh_box;
+
+typedef struct sus2sh_model_info {
+  int species_count;
+  int l_max;
+  int k_max;
+  int body_order;
+  int radial_basis_size;
+  int radial_funcs_count;
+  int moments_count;      /* alpha_moments_count */
+  int basic_count;        /* alpha_index_basic_count */
+  int scalar_count;       /* alpha_scalar_moments */
+  int product_count;      /* sh_product_count */
+  int layer_count;
+  int row_count;          /* explicit rows (one per defined target) */
+  int back_row_count;
+  int back_term_count;
+  int standard_graph;     /* 1 when the file graph equals the regenerated standard graph */
+  int use_float;          /* 1: FP32 moments/adjoint, 0: FP64 throughout */
+  int param_count;        /* length of the dE/dtheta vector */
+  int stored_slots;       /* per-atom on-chip graph state (values and adjoints each) */
+  int graph_tile;         /* atoms per workgroup in the graph kernel */
+  int neighbor_capacity;  /* current per-atom neighbour capacity */
+  double max_dist;
+} sus2sh_model_info;
+
+/* Model identity: first token "MTP" and the substring "potential_tag = SUS2-SH".
+ * Replaces is_sus2_sh_potential_file (src/force/sus2_sh.cu:7771-7780) together with the
+ * "MTP" dispatch in Force::parse_potential (src/force/force.cu:82-87,112-114).
+ * Returns 1 for a SUS2-SH model file, 0 otherwise. */
+int sus2sh_probe(const char* path);
+
+/* Constructor.  Replaces SUS2_SH::SUS2_SH(file_potential, num_atoms,
+ * num_potential_options, potential_options) (src/force/sus2_sh.cuh:12-16, called from
+ * src/force/force.cu:114).  `opts` are the run.in tokens after the model file: the
+ * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
+ * Honoured options: sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
+ * sus2_sh_graph_tile=<n>, sus2_sh_profile=<0|1>; other sus2_* tokens are accepted and
+ * ignored.  Uses the current HIP device. */
+int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
+                  sus2sh_handle** out);
+
+/* Destructor.  Replaces SUS2_SH::~SUS2_SH (src/force/sus2_sh.cuh:17). */
+void sus2sh_destroy(sus2sh_handle* handle);
+
+/* One evaluation.  Replaces SUS2_SH::compute(Box&, type, position, potential, force,
+ * virial) (src/force/sus2_sh.cuh:19-25, src/force/sus2_sh.cu:9620-10675), called from
+ * Force::compute (src/force/force.cu:538-539).
+ *   d_type      int[N]       species index of each atom (order of symbols in run.in)
+ *   d_position  double[3N]   x[N], y[N], z[N]
+ *   d_potential double[N]    += per-atom energy
+ *   d_force     double[3N]   += fx[N], fy[N], fz[N]
+ *   d_virial    double[9N]   += per-atom virial rows xx yy zz xy xz yz yx zx zy
+ * Outputs are accumulated; the caller zeroes them (src/force/force.cu:513-519).
+ * The neighbour list is rebuilt on every call (src/force/sus2_sh.cu:9648). */
+int sus2sh_compute(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                   const int* d_type, const double* d_position, double* d_potential,
+                   double* d_force, double* d_virial, void* stream);
+
+/* Model-parameter gradient of the total energy, dE/dtheta, accumulated (+=) into
+ * d_param_grad (double[param_count]).  Parameter order:
+ *   [shift_coeffs S][species_coeffs S][moment_coeffs Q][radial_coeffs M_r*rb]
+ *   [radial type coefficients S][scal_coeffs 2 S^2 M_r]
+ * No counterpart in the reference backend (training lives in the separate SUS2-SH
+ * trainers).  Energy/forces/virial are accumulated exactly as in sus2sh_compute. */
+int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                              const int* d_type, const double* d_position,
+                              double* d_potential, double* d_force, double* d_virial,
+                              double* d_param_grad, void* stream);
+
+/* Last error message of a handle, or of the last failed sus2sh_create on this thread
+ * when handle is NULL.  Never NULL. */
+const char* sus2sh_last_error(const sus2sh_handle* handle);
+
+/* Neighbour cutoff (reference: Potential::rc = max_dist, src/force/sus2_sh.cu:7821). */
+double sus2sh_cutoff(const sus2sh_handle* handle);
+
+int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out);
+
+/* Per-stage GPU times of the last compute when profiling is on (sus2_sh_profile=1):
+ * ms[0..n) = neighbour, basic, graph, force, scatter.  Returns the number written.
+ * Replaces the reference's host-clock stage profile (src/force/sus2_sh.cu:9631-9645). */
+int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
+
+/* Mean number of neighbours per atom found by the last compute. */
+double sus2sh_mean_neighbors(const sus2sh_handle* handle);
+
+/* ---------------- host-only model utilities (no GPU needed) ---------------- */
+
+/* Clebsch-Gordan coefficient <j1 m1 j2 m2 | j m> (src/force/sus2_sh.cu:1107-1142). */
+double sus2sh_clebsch_gordan(int j1, int m1, int j2, int m2, int j, int m);
+
+/* Real-basis coupling coefficient (src/force/sus2_sh.cu:1186-1218). */
+double sus2sh_real_cg(int l1, int m1, int l2, int m2, int L, int M);
+
+typedef struct sus2sh_graph sus2sh_graph;
+
+/* Standard real-CG graph (SHStandardGraphBuilderHost::BuildGraph,
+ * src/force/sus2_sh.cu:1231-1249).  body_l_max holds the sh_body_l_max entries for body
+ * orders 2.. (4 values for body order <= 5, 5 for body order 6), or NULL for l_max. */
+int sus2sh_standard_graph(int l_max, int k_max, int body_order, const int* body_l_max,
+                          int n_body_l_max, sus2sh_graph** out);
+/* dims[0..5) = node_count, basic_count, product_count, scalar_count, max_layer */
+int sus2sh_graph_dims(const sus2sh_graph* graph, int* dims);
+/* basic_klm[3B] = (k,l,m); product_idx[3T] = (left,right,target); product_coeff[T];
+ * scalars[Q]; scalar_body[Q] (body order of each scalar). */
+int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_idx,
+                        double* product_coeff, int* scalars, int* scalar_body);
+void sus2sh_graph_free(sus2sh_graph* graph);
+
+/* Parse and validate a model file on the host only (the reference loader,
+ * src/force/sus2_sh.cu:2105-2482).  dims[0..16) = species_count, l_max, k_max,
+ * body_order, radial_basis_size, radial_funcs_count, moments_count, basic_count,
+ * scalar_count, product_count, layer_count, row_count, row_term_count,
+ * back_row_count, back_term_count, standard_graph. */
+int sus2sh_model_check(const char* mtp_path, int* dims);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUS2SH_H */

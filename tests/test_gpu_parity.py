@@ -1,0 +1,166 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (stated per precision):
+  FP64 engine  : |dE| <= 1e-9 eV/atom, |dF| <= 1e-8 eV/A, |dW| <= 1e-8 eV (the whole path is
+                 double; the north-star bound is 1e-6 eV/atom and 1e-5 eV/A)
+  FP32 engine  : |dE| <= 2e-5 eV/atom + 2e-6 |E|, |dF| <= 1e-4 eV/A + 2e-5 |F|max,
+                 |dW| <= 1e-4 eV + 2e-5 |W|max
+"""
+import numpy as np
+import pytest
+
+import sus2sh
+from sus2sh import structures as st
+from oracle.binding import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = dict(e=1e-9, f=1e-8, w=1e-8)
+
+
+def _run(path, cell, types, pos, options=(), pbc=(1, 1, 1)):
+    print(f"[case] model={path} N={len(types)} options={options}", flush=True)
+    return sus2sh.evaluate(path, cell, types, pos, options=options, pbc=pbc)
+
+
+def _compare(ref, got, fp32):
+    e0, f0, v0 = ref
+    e1, f1, v1 = got
+    de = np.abs(e1 - e0).max()
+    df = np.abs(f1 - f0).max()
+    dv = np.abs(v1 - v0).max()
+    print(f"  max|dE|={de:.3e} eV/atom  max|dF|={df:.3e} eV/A  max|dW|={dv:.3e}  "
+          f"(|F|max={np.abs(f0).max():.3f}, |E|max={np.abs(e0).max():.3f})", flush=True)
+    if fp32:
+        assert de <= 2e-5 + 2e-6 * np.abs(e0).max()
+        assert df <= 1e-4 + 2e-5 * np.abs(f0).max()
+        assert dv <= 1e-4 + 2e-5 * np.abs(v0).max()
+    else:
+        assert de <= TOL64["e"]
+        assert df <= TOL64["f"]
+        assert dv <= TOL64["w"]
+
+
+CASES = {
+    "l2k2_fcc": dict(model=dict(l_max=2, k_max=2, body_order=5, rb_size=8, max_dist=5.0, seed=11),
+                     struct=("fcc", 4, 3.80, 0.05)),
+    "l3k3_fcc": dict(model=dict(l_max=3, k_max=3, body_order=5, rb_size=8, max_dist=5.0, seed=12,
+                                scaling=0.5),
+                     struct=("fcc", 5, 3.85, 0.08)),
+    "l4k4_4422": dict(model=dict(l_max=4, k_max=4, body_order=5, body_l_max=(4, 4, 2, 2),
+                                 rb_size=10, max_dist=5.5, seed=13, species_count=3,
+                                 radial_basis_type="RBChebyshev_sss_rational", scaling=0.4),
+                      struct=("fcc", 4, 3.85, 0.08)),
+    "l3k2_nonuniform": dict(model=dict(l_max=3, k_max=2, body_order=4, rb_size=6, max_dist=4.6,
+                                       seed=14, scal_mode="random"),
+                            struct=("fcc", 4, 3.7, 0.1)),
+    "l2k3_body6": dict(model=dict(l_max=2, k_max=3, body_order=6, rb_size=5, max_dist=4.2,
+                                  seed=15, scaling=0.6),
+                       struct=("fcc", 4, 3.6, 0.05)),
+}
+
+
+def _structure(spec, S, seed=7):
+    kind, n, a, sigma = spec
+    fr = [1.0 / S] * S
+    return st.fcc_alloy(n, n, n, a, fr, sigma, seed, seed + 1)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("fp32", [False, True])
+def test_engine_matches_oracle(make_model, name, fp32):
+    case = CASES[name]
+    path = make_model(name, **case["model"])
+    S = case["model"].get("species_count", 2)
+    pos, types, cell = _structure(case["struct"], S)
+    ref = Oracle(path).compute(cell, types, pos)
+    opts = [f"T{s}" for s in range(S)] + ["sus2_float=%d" % (1 if fp32 else 0)]
+    got = _run(path, cell, types, pos, options=opts)
+    _compare(ref, got, fp32)
+
+
+def _separated_positions(cell, n, dmin, rng):
+    """Random positions in a periodic cell with every image pair at least dmin apart."""
+    shifts = np.array([[i, j, k] for i in range(-3, 4) for j in range(-3, 4)
+                       for k in range(-3, 4)], dtype=np.float64) @ cell
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(0, 1, size=3) @ cell
+        ok = True
+        for q in pts + [p]:
+            d = (q - p)[None, :] + shifts
+            r = np.sqrt((d * d).sum(1))
+            if q is p:
+                r = r[r > 1e-12]
+            if r.min() < dmin:
+                ok = False
+                break
+        if ok:
+            pts.append(p)
+    return np.array(pts).T.copy()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_small_triclinic_cells_multi_image(make_model, fp32):
+    """Cells thinner than 2 rc (down to < rc) need every periodic image within rc
+    (src/force/sus2_sh.cu:7668-7767)."""
+    path = make_model("l3k3_small", l_max=3, k_max=3, body_order=5, rb_size=8, max_dist=5.0,
+                      seed=21, scaling=0.5)
+    orc = Oracle(path)
+    rng = np.random.default_rng(5)
+    for n in (8, 17, 40):
+        cell = st.triclinic_cell(n, 0.065, rng)
+        pos = _separated_positions(cell, n, 1.8, rng)
+        types = rng.integers(0, 2, size=n).astype(np.int32)
+        ref = orc.compute(cell, types, pos, brute=True)
+        got = _run(path, cell, types, pos, options=["Cu", "Zr", "sus2_float=%d" % fp32])
+        _compare(ref, got, fp32)
+
+
+def test_open_boundaries(make_model):
+    path = make_model("l2k2_open", l_max=2, k_max=2, body_order=4, rb_size=6, max_dist=5.0, seed=22)
+    pos, types, cell = st.fcc_alloy(3, 3, 3, 3.8, [0.5, 0.5], 0.05, 3, 4)
+    for pbc in ((0, 0, 0), (1, 0, 1)):
+        ref = Oracle(path).compute(cell, types, pos, pbc=pbc, brute=True)
+        got = _run(path, cell, types, pos, options=["A", "B", "sus2_float=0"], pbc=pbc)
+        _compare(ref, got, False)
+
+
+def test_accumulate_and_resize(make_model):
+    """Outputs are accumulated (+=) and N may change between calls on one handle
+    (src/force/force.cu:513-539; src/force/sus2_sh.cu:9628-9629)."""
+    import torch
+    path = make_model("l2k2_acc", l_max=2, k_max=2, body_order=5, rb_size=8, max_dist=5.0, seed=23)
+    pot = sus2sh.SUS2_SH(path, 0, ["Cu", "Zr", "sus2_float=0"])
+    orc = Oracle(path)
+    for n in (3, 4, 3):
+        pos, types, cell = st.fcc_alloy(n, n, n, 3.8, [0.5, 0.5], 0.05, n, n + 1)
+        N = len(types)
+        print(f"[case] accumulate N={N}", flush=True)
+        ref = orc.compute(cell, types, pos)
+        t = torch.as_tensor(types, device="cuda")
+        x = torch.as_tensor(pos, device="cuda")
+        e = torch.ones(N, dtype=torch.float64, device="cuda")
+        f = torch.ones(3, N, dtype=torch.float64, device="cuda")
+        v = torch.ones(9, N, dtype=torch.float64, device="cuda")
+        box = sus2sh.box_struct(cell)
+        pot.compute(box, t, x, e, f, v)
+        pot.compute(box, t, x, e, f, v)
+        torch.cuda.synchronize()
+        assert np.abs(e.cpu().numpy() - (1 + 2 * ref[0])).max() < 1e-8
+        assert np.abs(f.cpu().numpy() - (1 + 2 * ref[1])).max() < 1e-8
+        assert np.abs(v.cpu().numpy() - (1 + 2 * ref[2])).max() < 1e-8
+    # empty system is a no-op
+    z = torch.zeros(0, dtype=torch.float64, device="cuda")
+    pot.compute(sus2sh.box_struct(np.eye(3) * 20), torch.zeros(0, dtype=torch.int32, device="cuda"),
+                z, z, z, z)
+    pot.close()
+
+
+def test_invalid_model_reports_error(tmp_path):
+    bad = tmp_path / "bad.mtp"
+    bad.write_text("MTP\nversion = 1.1.0\npotential_tag = SUS2-SH\n")
+    with pytest.raises(sus2sh.Sus2shError) as err:
+        sus2sh.SUS2_SH(str(bad), 10, ["Cu"])
+    assert err.value.code == 1
+    assert "Missing token" in err.value.message

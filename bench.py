@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the SUS2-SH potential path on one MI355X.
+
+Workload (BASELINE metric): Cu-Zr l3k3 at 1,024,000 atoms -- 80x80x40 fcc cells, a = 3.85 A,
+50/50 species (seed 3001), Gaussian jitter sigma = 0.08 A (seed 3002); SUS2-SH model
+sh_l_max = 3, sh_k_max = 3, body order 5 with sh_body_l_max = {3,3,3,3} (the reference's
+l3333 graph: 1349 moments, 5475 products, 572 scalars), 8 RBChebyshev_sss functions per
+radial channel, cutoff 5.0 A, coefficients N(0, 0.1) from seed 3003.  One step = one full
+potential evaluation (neighbour rebuild + basic moments + CG graph + forces/virial), as
+GPUMD calls SUS2_SH::compute once per MD step, with fixed positions (fixed-cell force loop).
+
+Prints ONE JSON line.  `value` is FP32 atom-steps/s; the FP64 path of the same engine, the
+FP32/FP64 speed ratio and the max force error of FP32 vs FP64 on the full system are extra
+fields.  Kernel times come from HIP events recorded by the engine on the stream it runs on.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpumd-sus2-sh_amd"))
+sys.path.insert(0, ROOT)
+
+REFERENCE_A100 = 5.23633e6  # l3k3 1.024M compact serial product, README.md:152 of the reference
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(nx, ny, nz, seed_model=3003):
+    import sus2sh
+    from sus2sh import structures as st
+    spec = sus2sh.synthetic_model(3, 3, 5, body_l_max=(3, 3, 3, 3), species_count=2, rb_size=8,
+                                  max_dist=5.0, seed=seed_model, scaling=0.5)
+    tmp = tempfile.mkdtemp(prefix="sus2sh_bench_")
+    path = sus2sh.write_mtp(spec, os.path.join(tmp, "cuzr_l3k3.mtp"))
+    pos, types, cell = st.fcc_alloy(nx, ny, nz, 3.85, [0.5, 0.5], 0.08, 3001, 3002)
+    return spec, path, pos, types, cell
+
+
+def kernel_flops_per_atom(info, nbar):
+    """Algorithmic FP32 flops per atom per kernel (FMA = 2; transcendentals not counted),
+    the per-atom formula of the survey's roofline accounting applied kernel by kernel."""
+    L, K = info["l_max"], info["k_max"]
+    B, Mr, rb = info["basic_count"], info["radial_funcs_count"], info["radial_basis_size"]
+    nc = (L + 1) ** 2
+    T = info["product_count"]
+    Q = info["scalar_count"]
+    basic = nbar * (2 * B + 4 * Mr * rb + 4 * nc)
+    force = nbar * (8 * B + 8 * Mr * rb + 16 * nc + 6 * Mr + 18)
+    graph = 3 * T + 3 * (2 * T) + 2 * Q
+    return {"basic": basic, "graph": graph, "force": force}
+
+
+def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
+    import torch
+    import sus2sh
+    n = len(types)
+    opts = ["Cu", "Zr", "sus2_float=%d" % (1 if precision == "fp32" else 0)]
+    if profile:
+        opts.append("sus2_sh_profile=1")
+    pot = sus2sh.SUS2_SH(path, n, opts)
+    dev = "cuda"
+    t = torch.as_tensor(types, device=dev)
+    x = torch.as_tensor(pos, device=dev).contiguous()
+    e = torch.zeros(n, dtype=torch.float64, device=dev)
+    f = torch.zeros(3, n, dtype=torch.float64, device=dev)
+    v = torch.zeros(9, n, dtype=torch.float64, device=dev)
+    box = sus2sh.box_struct(cell)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        e.zero_()
+        f.zero_()
+        v.zero_()
+        pot.compute(box, t, x, e, f, v)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stage_sum = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        if profile:
+            st = pot.stage_times()
+            stage_sum = st if stage_sum is None else [a + b for a, b in zip(stage_sum, st)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info = pot.info()
+    nbar = pot.mean_neighbors()
+    out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
+               info=info, nbar=nbar, energy=float(e.sum().item()),
+               force=f.cpu().numpy(), stream=stream.cuda_stream)
+    if stage_sum is not None:
+        out["stage_ms"] = dict(zip(["neighbor", "basic", "graph", "force", "scatter"],
+                                   [s / steps for s in stage_sum]))
+    pot.close()
+    return out
+
+
+def cpu_baseline(path, seconds=12.0):
+    """FP64 C/OpenMP oracle (a port of the same algorithm) on a bounded sample of the same
+    workload: a 20x20x20-cell (32,000 atom) piece with the same lattice, composition and
+    jitter, evaluated repeatedly for about `seconds` of wall time."""
+    from oracle.binding import Oracle, lib
+    import sus2sh.structures as st
+    pos, types, cell = st.fcc_alloy(20, 20, 20, 3.85, [0.5, 0.5], 0.08, 3001, 3002)
+    o = Oracle(path)
+    threads = lib().oracle_num_threads()
+    o.compute(cell, types, pos)
+    n_eval, t0 = 0, time.perf_counter()
+    while True:
+        o.compute(cell, types, pos)
+        n_eval += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n_eval >= 50:
+            break
+    return {"value": len(types) * n_eval / el, "unit": "atom-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n_eval} FP64 evaluations of a 32,000-atom 20x20x20 fcc Cu-Zr piece "
+                      f"(same model, lattice, composition, jitter) in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, nargs=3, default=[80, 80, 40])
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part "
+                                   "of this build (spatial decomposition is a design note only)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+
+    nx, ny, nz = args.cells
+    log(f"[bench] building workload {nx}x{ny}x{nz} fcc")
+    spec, path, pos, types, cell = build_workload(nx, ny, nz)
+    N = len(types)
+    log(f"[bench] N={N}; FP32 run: {args.warmup} warmup + {args.steps} timed steps")
+    r32 = run_engine(path, cell, types, pos, "fp32", args.steps, args.warmup, profile=True)
+    log(f"[bench] fp32 {r32['atom_steps_per_s']:.4e} atom-steps/s, stages {r32.get('stage_ms')}")
+    log(f"[bench] FP64 run: 2 warmup + {args.fp64_steps} timed steps")
+    r64 = run_engine(path, cell, types, pos, "fp64", args.fp64_steps, 2, profile=True)
+    log(f"[bench] fp64 {r64['atom_steps_per_s']:.4e} atom-steps/s, stages {r64.get('stage_ms')}")
+    ferr = float(np.abs(r32["force"] - r64["force"]).max())
+    frms = float(np.sqrt(((r32["force"] - r64["force"]) ** 2).mean()))
+
+    info, nbar = r32["info"], r32["nbar"]
+    flops = kernel_flops_per_atom(info, nbar)
+    stages = r32["stage_ms"]
+    dominant = max(("basic", "graph", "force"), key=lambda k: stages[k])
+    achieved = flops[dominant] * N / (stages[dominant] * 1e-3) / 1e12
+    total_flops = (flops["basic"] + flops["graph"] + flops["force"]) * N
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (FP64 oracle, bounded sample)")
+        cpu = cpu_baseline(path, args.cpu_seconds)
+        log(f"[bench] cpu {cpu}")
+
+    result = {
+        "metric": "atom-steps/s at Cu-Zr l3k3 1.024M (FP32 vs FP64 path); max force err vs FP64",
+        "value": r32["atom_steps_per_s"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": r32["atom_steps_per_s"] / REFERENCE_A100,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"CuZr_l3k3_fcc_{N}_atoms_fixed_cell_force_loop",
+                   "atoms": N, "cells": [nx, ny, nz], "lattice_a": 3.85, "jitter": 0.08,
+                   "model": "l3333 body5 rb8 rc5.0 seed3003", "mean_neighbors": nbar},
+        "fp64_value": r64["atom_steps_per_s"],
+        "fp64_ms_per_step": r64["ms_per_step"],
+        "fp32_over_fp64": r32["atom_steps_per_s"] / r64["atom_steps_per_s"],
+        "max_force_err_fp32_vs_fp64": ferr,
+        "rms_force_err_fp32_vs_fp64": frms,
+        "stage_ms_fp32": stages,
+        "stage_ms_fp64": r64["stage_ms"],
+        "step_tflops_fp32": total_flops / (r32["ms_per_step"] * 1e-3) / 1e12,
+        "roofline": {"bound": "valu", "kernel": dominant, "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flops_per_atom": flops[dominant]},
+        "cpu_baseline": cpu,
+        "vs_baseline_note": "reference A100 figure 5.23633e6 is a whole GPUMD NPT step with an "
+                            "rc=7.5 rb=10 l3333 glass model",
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,23 @@ int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int 
   });
 }
 
+int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                  const int* d_type, const double* masses, double* d_position,
+                  double* d_velocity, double dt_fs, int nsteps, double* d_potential,
+                  double* d_force, double* d_virial, double* energy_trace, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  if (!box) {
+    handle->error = "sus2sh_md_nve: box is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  return guarded(&handle->error, [&] {
+    handle->engine->md_nve(*box, num_atoms, d_type, masses, d_position, d_velocity, dt_fs, nsteps,
+                           d_potential, d_force, d_virial, energy_trace,
+                           static_cast<hipStream_t>(stream));
+  });
+}
+
 const char* sus2sh_last_error(const sus2sh_handle* handle)
 {
   if (!handle) return g_create_error.c_str();

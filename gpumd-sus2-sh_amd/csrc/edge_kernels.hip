@@ -76,32 +76,91 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   }
 }
 
-template <typename Real, int L, int K>
+// Adjoint of the radial coefficients for one edge: with A_mu = sum_m G_{k,l,m} Y_lm,
+//   dE/dc^p_{mu n} += A_mu (r - rc)^2 T_n(xi_mu),
+//   dE/da^p_mu     += A_mu (r - rc)^2 sum_n c_n T'_n(xi_mu) dxi/da   (same for s),
+// accumulated per pair p in LDS (double).  No reference counterpart (training path).
+template <typename Real, int MR>
+__device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ coeff,
+                                                     const Real* __restrict__ scal, int rb,
+                                                     Real rc, int kind, Real r,
+                                                     const Real (&A)[MR], double* __restrict__ D,
+                                                     double* __restrict__ Da,
+                                                     double* __restrict__ Ds)
+{
+  const Real env = (r - rc) * (r - rc);
+#pragma unroll
+  for (int mu = 0; mu < MR; ++mu) {
+    const Real a = scal[2 * mu], s = scal[2 * mu + 1];
+    const Real z = Real(0.5) * a * (r - s);
+    Real xi, dxdz;
+    if (kind == 1) {
+      const Real q = Real(1) / sqrt(Real(1) + z * z);
+      xi = z * q;
+      dxdz = q * q * q;
+    } else {
+      xi = tanh(z);
+      dxdz = Real(1) - xi * xi;
+    }
+    const Real* c = coeff + mu * rb;
+    const Real w = A[mu] * env;
+    Real t0 = 1, t1 = xi, d0 = 0, d1 = 1;
+    atomicAdd(D + mu * rb, static_cast<double>(w));
+    Real dsum = 0;
+    if (rb > 1) {
+      atomicAdd(D + mu * rb + 1, static_cast<double>(w * xi));
+      dsum = c[1];
+    }
+    for (int n = 2; n < rb; ++n) {
+      const Real t2 = Real(2) * xi * t1 - t0;
+      const Real d2 = Real(2) * (t1 + xi * d1) - d0;
+      atomicAdd(D + mu * rb + n, static_cast<double>(w * t2));
+      dsum += c[n] * d2;
+      t0 = t1;
+      t1 = t2;
+      d0 = d1;
+      d1 = d2;
+    }
+    atomicAdd(Da + mu, static_cast<double>(w * dsum * Real(0.5) * (r - s) * dxdz));
+    atomicAdd(Ds + mu, static_cast<double>(w * dsum * Real(-0.5) * a * dxdz));
+  }
+}
+
+template <typename Real, int L, int K, bool PG>
 __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
-                                                double* __restrict__ v_s)
+                                                double* __restrict__ v_s,
+                                                double* __restrict__ pg_rad)
 {
   constexpr int MR = K * (L + 1);
   constexpr int NC = (L + 1) * (L + 1);
   constexpr int FG = K * NC;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int S = a.rt.S;
+  const int rb = a.rt.rb;
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
-  Real* s_scal = s_coeff + a.rt.S * a.rt.S * MR * a.rt.rb;
+  Real* s_scal = s_coeff + S * S * MR * rb;
+  // PG: per-pair adjoint accumulators [S*S][MR*rb + 2*MR] (double), 16-byte aligned
+  const size_t pg_off = ((static_cast<size_t>(S) * S * MR * (rb + 2) * sizeof(Real)) + 15) & ~size_t(15);
+  double* s_pg = reinterpret_cast<double*>(smem_raw + pg_off);
+  const int pg_stride = MR * rb + 2 * MR;
+  if (PG) {
+    for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) s_pg[i] = 0.0;
+  }
   stage_tables(a.rt, MR, s_coeff, s_scal);
 
   const int N = a.N;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
+  const bool valid = p < N;
+  if (!PG && !valid) return;
   Real G[FG];
 #pragma unroll
   for (int f = 0; f < FG; ++f) {
     const int b = a.fmap[f];
-    G[f] = b >= 0 ? grads[static_cast<size_t>(b) * N + p] : Real(0);
+    G[f] = (b >= 0 && valid) ? grads[static_cast<size_t>(b) * N + p] : Real(0);
   }
-  const int ti = a.type_s[p];
-  const int cnt = a.count[p];
-  const int S = a.rt.S;
-  const int rb = a.rt.rb;
+  const int ti = valid ? a.type_s[p] : 0;
+  const int cnt = valid ? a.count[p] : 0;
   const Real rc = a.rt.rc;
   Real fx = 0, fy = 0, fz = 0;
   Real w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -139,6 +198,23 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
         gz += un * Gz[y];
       }
     }
+    if (PG) {
+      Real A[MR];
+#pragma unroll
+      for (int l = 0; l <= L; ++l) {
+        const Real nl = Real(norm_of_l(l));
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+          Real acc = 0;
+#pragma unroll
+          for (int m = -l; m <= l; ++m) acc += G[full_index<L, K>(l, kk, m)] * Sv[sh_index(l, m)];
+          A[kk * (L + 1) + l] = acc * nl;
+        }
+      }
+      double* Dp = s_pg + pair * pg_stride;
+      radial_param_adjoint<Real, MR>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+                                     a.rt.kind, r, A, Dp, Dp + MR * rb, Dp + MR * rb + MR);
+    }
     const Real g0 = ux * sv + gx * inv;
     const Real g1 = uy * sv + gy * inv;
     const Real g2 = uz * sv + gz * inv;
@@ -158,11 +234,20 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     w[7] -= e.dz * g0;
     w[8] -= e.dz * g1;
   }
-  unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
-  unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
-  unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
+  if (valid) {
+    unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
+    unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
+    unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
 #pragma unroll
-  for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+    for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+  }
+  if (PG) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) {
+      const double v = s_pg[i];
+      if (v != 0.0) unsafeAtomicAdd(pg_rad + i, v);
+    }
+  }
 }
 
 template <typename Real>
@@ -201,14 +286,21 @@ template <typename Real>
 bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
                    double* pg_rad, hipStream_t s)
 {
-  (void)pg_rad;
   if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
-  const size_t lds = table_bytes(a);
-#define SUS2SH_FORCES_CASE(LL, KK)                                              \
-  if (a.L == LL && a.K == KK) {                                                 \
-    k_forces<Real, LL, KK><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s);         \
-    return true;                                                                \
+  size_t lds = table_bytes(a);
+  if (pg_rad) {
+    const int Mr = a.K * (a.L + 1);
+    lds = ((lds + 15) & ~size_t(15)) +
+          sizeof(double) * a.rt.S * a.rt.S * (Mr * a.rt.rb + 2 * Mr);
+  }
+#define SUS2SH_FORCES_CASE(LL, KK)                                                         \
+  if (a.L == LL && a.K == KK) {                                                            \
+    if (pg_rad)                                                                            \
+      k_forces<Real, LL, KK, true><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);   \
+    else                                                                                   \
+      k_forces<Real, LL, KK, false><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);  \
+    return true;                                                                           \
   }
   SUS2SH_FOR_EACH_LK(SUS2SH_FORCES_CASE)
 #undef SUS2SH_FORCES_CASE

@@ -99,6 +99,8 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_),
             "hipDeviceGetAttribute");
   max_lds_ = static_cast<size_t>(lds);
+  hip_check(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, device_),
+            "hipDeviceGetAttribute");
 
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
@@ -220,8 +222,10 @@ void Engine::upload_tables()
   seed_.upload(seed.data(), seed.size() * sizeof(Real));
   shift_.upload(m.shift_coeffs.data(), m.shift_coeffs.size() * sizeof(double));
   sigma_.upload(m.species_coeffs.data(), m.species_coeffs.size() * sizeof(double));
-  scalar_slot_.upload(g.scalar_slot.data(), g.scalar_slot.size() * sizeof(int));
-  scalar_row_.upload(g.scalar_row.data(), g.scalar_row.size() * sizeof(int));
+  row_sbegin_.upload(g.row_scalar_begin.data(), g.row_scalar_begin.size() * sizeof(int));
+  row_slist_.upload(g.row_scalar_list.data(), g.row_scalar_list.size() * sizeof(int));
+  stored_index_.upload(g.stored_scalar_index.data(), g.stored_scalar_index.size() * sizeof(int));
+  stored_slot_.upload(g.stored_scalar_slot.data(), g.stored_scalar_slot.size() * sizeof(int));
 }
 
 void Engine::ensure_atoms(int N, size_t edge_bytes)
@@ -306,8 +310,6 @@ template <typename Real>
 void Engine::run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
                  double* force, double* virial, hipStream_t s, double* param_grad)
 {
-  if (param_grad)
-    throw UnsupportedError("SUS2_SH: parameter gradients are not available in this build.");
   int ncell = 0;
   const CellGrid grid = make_grid(box, N, &ncell);
   if (cap_ == 0) cap_ = 64;
@@ -400,19 +402,41 @@ void Engine::run(const sus2sh_box& box, int N, const int* type, const double* po
   gt.slots = prog_.slot_count;
   gt.basics = model_.basic_count;
   gt.layers = prog_.layer_count;
+  ParamGradOut pgo{};
+  const int S = model_.species_count, Q = model_.scalar_count, Mr = model_.radial_funcs_count;
+  const size_t n_rad = static_cast<size_t>(S) * S * (Mr * model_.rb_size + 2 * Mr);
+  if (param_grad) {
+    pg_misc_.resize((2 * S + Q) * sizeof(double));
+    pg_rad_.resize(n_rad * sizeof(double));
+    launch_fill_double(pg_misc_.as<double>(), 0.0, 2 * S + Q, s);
+    launch_fill_double(pg_rad_.as<double>(), 0.0, n_rad, s);
+    pgo.d_shift = pg_misc_.as<double>();
+    pgo.d_species = pg_misc_.as<double>() + S;
+    pgo.d_moment = pg_misc_.as<double>() + 2 * S;
+    pgo.row_sbegin = row_sbegin_.as<int>();
+    pgo.row_slist = row_slist_.as<int>();
+    pgo.stored_index = stored_index_.as<int>();
+    pgo.stored_slot = stored_slot_.as<int>();
+    pgo.n_stored = static_cast<int>(prog_.stored_scalar_index.size());
+    pgo.Q = Q;
+    pgo.S = S;
+  }
   if (!launch_graph<Real>(tile_, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
-                          grads_.as<Real>(), nullptr, s, max_lds_))
+                          grads_.as<Real>(), param_grad ? &pgo : nullptr, s, max_lds_,
+                          4 * cu_count_))
     throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
   if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
   launch_fill_double(f_s_.as<double>(), 0.0, 3 * static_cast<size_t>(N), s);
-  if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(), nullptr, s))
+  if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(),
+                           param_grad ? pg_rad_.as<double>() : nullptr, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
   if (opt_.profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
   launch_scatter(N, satom_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
                  pot, force, virial, s);
   hip_check(hipGetLastError(), "kernel launch");
+  if (param_grad) finish_param_grad(param_grad, s);
   if (opt_.profile) {
     hip_check(hipEventRecord(events_[5], s), "hipEventRecord");
     hip_check(hipEventSynchronize(events_[5]), "hipEventSynchronize");
@@ -421,6 +445,92 @@ void Engine::run(const sus2sh_box& box, int N, const int* type, const double* po
       hip_check(hipEventElapsedTime(&stage_ms_[k], events_[k], events_[k + 1]),
                 "hipEventElapsedTime");
   }
+}
+
+void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double* masses,
+                    double* pos, double* vel, double dt_fs, int nsteps, double* pot, double* force,
+                    double* virial, double* energy_trace, hipStream_t s)
+{
+  if (N <= 0 || nsteps < 0) throw ArgumentError("md_nve: N must be > 0 and nsteps >= 0");
+  if (!masses || !pos || !vel) throw ArgumentError("md_nve: null pointer");
+  const int S = model_.species_count;
+  for (int t = 0; t < S; ++t)
+    if (!(masses[t] > 0.0)) throw ArgumentError("md_nve: masses must be positive");
+  md_mass_.upload(masses, S * sizeof(double));
+  md_energy_.resize(2 * static_cast<size_t>(nsteps + 1) * sizeof(double));
+  double* en = md_energy_.as<double>();
+  launch_fill_double(en, 0.0, 2 * static_cast<size_t>(nsteps + 1), s);
+  const size_t n = static_cast<size_t>(N);
+  auto evaluate = [&](int k) {
+    launch_fill_double(pot, 0.0, n, s);
+    launch_fill_double(force, 0.0, 3 * n, s);
+    launch_fill_double(virial, 0.0, 9 * n, s);
+    compute(box, N, type, pos, pot, force, virial, s, nullptr);
+    (void)k;
+  };
+  evaluate(0);
+  launch_energies(N, type, md_mass_.as<double>(), pot, vel, en, s);
+  for (int k = 1; k <= nsteps; ++k) {
+    launch_half_kick(N, 0.5 * dt_fs, type, md_mass_.as<double>(), force, vel, s);
+    launch_drift(N, dt_fs, vel, pos, s);
+    evaluate(k);
+    launch_half_kick(N, 0.5 * dt_fs, type, md_mass_.as<double>(), force, vel, s);
+    launch_energies(N, type, md_mass_.as<double>(), pot, vel, en + 2 * k, s);
+  }
+  hip_check(hipGetLastError(), "md kernel launch");
+  if (energy_trace) {
+    hip_check(hipMemcpyAsync(energy_trace, en, 2 * static_cast<size_t>(nsteps + 1) * sizeof(double),
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+}
+
+// dE/dtheta from the per-type / per-pair adjoint sums (host, double):
+//   shift, species, moment_coeffs directly; radial[mu][n] and type coefficients through
+//   c^p_{mu n} = scaling w_ti w_tj radial[mu][n]; scal_coeffs per (mu, pair).
+void Engine::finish_param_grad(double* d_param_grad, hipStream_t s)
+{
+  const Model& m = model_;
+  const int S = m.species_count, Q = m.scalar_count, Mr = m.radial_funcs_count, rb = m.rb_size;
+  const int stride = Mr * rb + 2 * Mr;
+  std::vector<double> misc(2 * S + Q), rad(static_cast<size_t>(S) * S * stride);
+  const int P = param_count();
+  std::vector<double> grad(P);
+  hip_check(hipMemcpyAsync(misc.data(), pg_misc_.as<double>(), misc.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(rad.data(), pg_rad_.as<double>(), rad.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(grad.data(), d_param_grad, P * sizeof(double), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const int o_shift = 0, o_species = S, o_w = 2 * S, o_rad = 2 * S + Q, o_type = o_rad + Mr * rb,
+            o_scal = o_type + S;
+  for (int t = 0; t < S; ++t) {
+    grad[o_shift + t] += misc[t];
+    grad[o_species + t] += misc[S + t];
+  }
+  for (int q = 0; q < Q; ++q) grad[o_w + q] += misc[2 * S + q];
+  for (int ti = 0; ti < S; ++ti)
+    for (int tj = 0; tj < S; ++tj) {
+      const double* D = rad.data() + static_cast<size_t>(ti * S + tj) * stride;
+      const double wi = m.radial_type_coeffs[ti], wj = m.radial_type_coeffs[tj];
+      for (int mu = 0; mu < Mr; ++mu) {
+        for (int n = 0; n < rb; ++n) {
+          const double d = D[mu * rb + n];
+          grad[o_rad + mu * rb + n] += d * m.scaling * wi * wj;
+          const double dv = d * m.scaling * m.radial_coeffs[mu * rb + n];
+          grad[o_type + ti] += dv * wj;
+          grad[o_type + tj] += dv * wi;
+        }
+        const size_t so = static_cast<size_t>(2) * mu * S * S + static_cast<size_t>(S) * ti + tj;
+        grad[o_scal + so] += D[Mr * rb + mu];
+        grad[o_scal + so + static_cast<size_t>(S) * S] += D[Mr * rb + Mr + mu];
+      }
+    }
+  hip_check(hipMemcpyAsync(d_param_grad, grad.data(), P * sizeof(double), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 template void Engine::upload_tables<float>();

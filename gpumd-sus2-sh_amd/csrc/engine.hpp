@@ -67,6 +67,12 @@ class Engine {
   void compute(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
                double* force, double* virial, hipStream_t stream, double* param_grad);
 
+  // Velocity-Verlet NVE: nsteps steps of dt_fs; energy_trace (host, may be null) receives
+  // (sum potential, sum kinetic) in eV for the initial state and after every step.
+  void md_nve(const sus2sh_box& box, int N, const int* type, const double* masses, double* pos,
+              double* vel, double dt_fs, int nsteps, double* pot, double* force, double* virial,
+              double* energy_trace, hipStream_t stream);
+
   const Model& model() const { return model_; }
   const GraphProgram& program() const { return prog_; }
   const EngineOptions& options() const { return opt_; }
@@ -83,6 +89,7 @@ class Engine {
   void run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
            double* force, double* virial, hipStream_t stream, double* param_grad);
   void ensure_atoms(int N, size_t edge_bytes);
+  void finish_param_grad(double* d_param_grad, hipStream_t s);
   CellGrid make_grid(const sus2sh_box& box, int N, int* ncell) const;
 
   Model model_;
@@ -98,7 +105,8 @@ class Engine {
   DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
   DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;
   DeviceBuffer e_slot_, e_w_, seed_, shift_, sigma_;
-  DeviceBuffer scalar_slot_, scalar_row_;
+  DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
+  int cu_count_ = 256;
   bool radial_uniform_ = false;
 
   // per-atom work buffers
@@ -106,7 +114,7 @@ class Engine {
   DeviceBuffer key_, val_, skey_, satom_, wrapped_, pos_s_, type_s_, count_;
   DeviceBuffer cell_begin_, cell_end_, sort_temp_, max_count_;
   DeviceBuffer nl_, basics_, grads_, e_s_, f_s_, v_s_;
-  DeviceBuffer pg_rad_, pg_misc_;
+  DeviceBuffer pg_rad_, pg_misc_, md_mass_, md_energy_;
   int* host_max_ = nullptr;  // pinned
 
   std::vector<float> stage_ms_;

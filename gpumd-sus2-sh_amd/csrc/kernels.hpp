@@ -32,14 +32,18 @@ struct GraphTable {
   int layers;
 };
 
-// Optional outputs for the parameter gradient.
+// Parameter-gradient outputs of the graph kernel (all double, accumulated with atomics).
 struct ParamGradOut {
-  double* d_species;   // [S]  += per-type sum of (1 + E') (nullptr: off)
-  double* d_shift;     // [S]  += per-type atom count
-  double* d_moment;    // [Q]  += sum_i sigma_i M_i[map[s]]
-  const int* scalar_slot;  // [Q]
-  const int* scalar_row;   // [Q]
+  double* d_shift;          // [S]  += number of atoms of each type
+  double* d_species;        // [S]  += sum over atoms of that type of (1 + E')
+  double* d_moment;         // [Q]  += sum_i sigma_i M_i[map[s]]
+  const int* row_sbegin;    // [rows + 1] scalars of each terminal row
+  const int* row_slist;
+  const int* stored_index;  // scalars held in a slot
+  const int* stored_slot;
+  int n_stored;
   int Q;
+  int S;
 };
 
 struct NeighborArgs {
@@ -91,13 +95,22 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
                    double* pg_rad, hipStream_t s);
 
 // --- graph kernel (graph_kernel.hip) ---
+// pg != nullptr selects the persistent parameter-gradient variant with at most
+// persistent_blocks workgroups.
 template <typename Real>
 bool launch_graph(int tile, int N, const Real* basics, const int* type_s,
                   const GraphTable<Real>& gt, double* e_s, Real* grads, const ParamGradOut* pg,
-                  hipStream_t s, size_t max_lds);
+                  hipStream_t s, size_t max_lds, int persistent_blocks);
 
 template <typename Real>
 size_t graph_lds_bytes(int tile, int slots);
+
+// --- velocity-Verlet NVE driver (md.hip) ---
+void launch_half_kick(int N, double half_dt, const int* type, const double* mass, const double* f,
+                      double* v, hipStream_t s);
+void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s);
+void launch_energies(int N, const int* type, const double* mass, const double* pot,
+                     const double* v, double* out, hipStream_t s);
 
 // --- scatter of per-sorted-atom results to the caller's arrays ---
 void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,

@@ -93,6 +93,23 @@ GraphProgram compile_graph(const Model& m, int chunk_terms)
     g.scalar_slot[s] = g.node_slot[n];
     if (g.node_slot[n] < 0) g.scalar_row[s] = node_row[n];
   }
+  {
+    const int nrows = static_cast<int>(g.fwd_row_target.size());
+    std::vector<std::vector<int>> per_row(nrows);
+    for (int s = 0; s < m.scalar_count; ++s) {
+      if (g.scalar_row[s] >= 0) per_row[g.scalar_row[s]].push_back(s);
+      if (g.scalar_slot[s] >= 0) {
+        g.stored_scalar_index.push_back(s);
+        g.stored_scalar_slot.push_back(g.scalar_slot[s]);
+      }
+    }
+    g.row_scalar_begin.assign(nrows + 1, 0);
+    for (int r = 0; r < nrows; ++r) {
+      g.row_scalar_begin[r] = static_cast<int>(g.row_scalar_list.size());
+      g.row_scalar_list.insert(g.row_scalar_list.end(), per_row[r].begin(), per_row[r].end());
+    }
+    g.row_scalar_begin[nrows] = static_cast<int>(g.row_scalar_list.size());
+  }
 
   // --- reverse: per layer, per source; merge duplicate (target, other) terms ---
   g.bwd_layer_begin.assign(m.layer_count + 1, 0);

@@ -53,9 +53,14 @@ struct GraphProgram {
   std::vector<uint32_t> bwd_term_idx;  // other_slot | target_slot << 16 (0xFFFF: constant)
   std::vector<double> bwd_term_coeff;  // c, or c * w_t for a terminal target
 
-  // parameter-gradient support: scalar s -> (slot or -1, terminal row or -1)
+  // parameter-gradient support: scalar s -> (slot or -1, terminal row or -1), and for each
+  // forward row the scalars mapped to its (terminal) node, CSR [rows + 1] / [entries]
   std::vector<int> scalar_slot;
   std::vector<int> scalar_row;
+  std::vector<int> row_scalar_begin;
+  std::vector<int> row_scalar_list;
+  std::vector<int> stored_scalar_index;  // scalars whose node has a slot
+  std::vector<int> stored_scalar_slot;
 
   int forward_terms() const { return static_cast<int>(fwd_term_idx.size()); }
   int backward_terms() const { return static_cast<int>(bwd_term_idx.size()); }

@@ -22,6 +22,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_destroy",
     "sus2sh_compute",
     "sus2sh_compute_param_grad",
+    "sus2sh_md_nve",
     "sus2sh_last_error",
     "sus2sh_cutoff",
     "sus2sh_get_model_info",
@@ -104,6 +105,10 @@ def lib():
     L.sus2sh_compute_param_grad.argtypes = [c_void_p, P(Box), c_int, c_void_p, c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]
     L.sus2sh_compute_param_grad.restype = c_int
+    L.sus2sh_md_nve.argtypes = [c_void_p, P(Box), c_int, c_void_p, P(c_double), c_void_p,
+                                c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p,
+                                P(c_double), c_void_p]
+    L.sus2sh_md_nve.restype = c_int
     L.sus2sh_last_error.argtypes = [c_void_p]
     L.sus2sh_last_error.restype = c_char_p
     L.sus2sh_cutoff.argtypes = [c_void_p]

@@ -98,6 +98,38 @@ class SUS2_SH:
         check(code, self._h)
 
 
+def md_nve(pot, box, type, position, velocity, masses, dt_fs, nsteps, stream=None):
+    """Velocity-Verlet NVE on the GPU (sus2sh_md_nve).  position/velocity: float64 [3, N]
+    cuda tensors advanced in place; masses: per-species amu.  Returns (energy trace
+    [nsteps+1, 2] of (potential, kinetic) in eV, potential[N], force[3,N], virial[9,N])."""
+    import torch
+    n = int(type.numel())
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    e = torch.zeros(n, dtype=torch.float64, device=position.device)
+    f = torch.zeros(3, n, dtype=torch.float64, device=position.device)
+    v = torch.zeros(9, n, dtype=torch.float64, device=position.device)
+    trace = np.zeros((nsteps + 1, 2))
+    s = ctypes.c_void_p(stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+    code = lib().sus2sh_md_nve(pot._h, ctypes.byref(box), n, _ptr(type),
+                               m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(position),
+                               _ptr(velocity), float(dt_fs), int(nsteps), _ptr(e), _ptr(f), _ptr(v),
+                               trace.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s)
+    check(code, pot._h)
+    return trace, e, f, v
+
+
+def maxwell_velocities(masses_per_atom, temperature, seed):
+    """Maxwell-Boltzmann velocities (A/fs) at `temperature` K with zero total momentum."""
+    kB = 8.617333262e-5  # eV/K
+    m = np.asarray(masses_per_atom, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    sd = np.sqrt(kB * temperature / m * 9.648533212331e-3)  # sqrt(kT/m) in A/fs
+    v = rng.normal(size=(3, len(m))) * sd[None, :]
+    p = (v * m[None, :]).sum(axis=1) / m.sum()
+    v -= p[:, None]
+    return v
+
+
 def evaluate(model_path, cell, types, positions, options=(), pbc=(1, 1, 1), device="cuda"):
     """One evaluation from host arrays; returns numpy (energy[N], force[3,N], virial[9,N])."""
     import torch

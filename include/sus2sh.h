@@ -100,6 +100,18 @@ int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int 
                               double* d_potential, double* d_force, double* d_virial,
                               double* d_param_grad, void* stream);
 
+/* Velocity-Verlet NVE driver around sus2sh_compute: nsteps steps of dt_fs femtoseconds.
+ * masses: host double[species_count] in amu; d_position / d_velocity (A, A/fs, SoA [3N]) are
+ * advanced in place; d_potential / d_force / d_virial hold the last evaluation (overwritten,
+ * not accumulated); energy_trace (host, may be NULL) receives double[nsteps+1][2] = (sum of
+ * potential, kinetic) in eV for the initial state and after each step.  No counterpart in
+ * the reference overlay (GPUMD's integrators are upstream code); used for the NVE
+ * energy-drift configurations. */
+int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                  const int* d_type, const double* masses, double* d_position,
+                  double* d_velocity, double dt_fs, int nsteps, double* d_potential,
+                  double* d_force, double* d_virial, double* energy_trace, void* stream);
+
 /* Last error message of a handle, or of the last failed sus2sh_create on this thread
  * when handle is NULL.  Never NULL. */
 const char* sus2sh_last_error(const sus2sh_handle* handle);

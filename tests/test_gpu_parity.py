@@ -164,3 +164,35 @@ def test_invalid_model_reports_error(tmp_path):
         sus2sh.SUS2_SH(str(bad), 10, ["Cu"])
     assert err.value.code == 1
     assert "Missing token" in err.value.message
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_parameter_gradient_matches_oracle(make_model, fp32):
+    """dE/dtheta over every model coefficient (no reference counterpart) against the FP64
+    oracle's adjoint; the oracle's adjoint is itself pinned by finite differences on CPU."""
+    import torch
+    path = make_model("l3k2_pg", l_max=3, k_max=2, body_order=5, rb_size=6, max_dist=4.8, seed=41,
+                      scal_mode="random", species_count=3, scaling=0.5)
+    pos, types, cell = st.fcc_alloy(3, 3, 3, 3.7, [0.4, 0.4, 0.2], 0.08, 5, 6)
+    ref = Oracle(path).compute(cell, types, pos, param_grad=True)
+    pg_ref = ref[3]
+    pot = sus2sh.SUS2_SH(path, len(types), ["A", "B", "C", "sus2_float=%d" % fp32])
+    P = pot.info()["param_count"]
+    assert P == len(pg_ref)
+    N = len(types)
+    print(f"[case] param grad N={N} P={P} fp32={fp32}", flush=True)
+    t = torch.as_tensor(types, device="cuda")
+    x = torch.as_tensor(pos, device="cuda")
+    e = torch.zeros(N, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3, N, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9, N, dtype=torch.float64, device="cuda")
+    g = torch.zeros(P, dtype=torch.float64, device="cuda")
+    pot.compute(sus2sh.box_struct(cell), t, x, e, f, v, param_grad=g)
+    torch.cuda.synchronize()
+    got = g.cpu().numpy()
+    err = np.abs(got - pg_ref).max()
+    scale = np.abs(pg_ref).max()
+    print(f"  max|d(dE/dtheta)|={err:.3e} (max |dE/dtheta|={scale:.3e})", flush=True)
+    assert err <= (1e-4 if fp32 else 1e-9) * max(1.0, scale)
+    assert np.abs(e.cpu().numpy() - ref[0]).max() < (1e-4 if fp32 else 1e-9)
+    pot.close()

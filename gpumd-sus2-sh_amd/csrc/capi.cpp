@@ -116,6 +116,19 @@ int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int 
   });
 }
 
+int sus2sh_compute_batch(sus2sh_handle* handle, int num_structures, const sus2sh_box* boxes,
+                         const int* atom_offsets, const int* d_type, const double* d_position,
+                         double* d_potential, double* d_force, double* d_virial,
+                         double* d_param_grad, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&handle->error, [&] {
+    handle->engine->compute_batch(num_structures, boxes, atom_offsets, d_type, d_position,
+                                  d_potential, d_force, d_virial,
+                                  static_cast<hipStream_t>(stream), d_param_grad);
+  });
+}
+
 int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                   const int* d_type, const double* masses, double* d_position,
                   double* d_velocity, double dt_fs, int nsteps, double* d_potential,

@@ -45,6 +45,8 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.use_float = parse_bool(val, key);
     } else if (key == "sus2_sh_profile") {
       o.profile = parse_bool(val, key);
+    } else if (key == "sus2_sh_autotune") {
+      o.autotune = parse_bool(val, key);
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -114,6 +116,7 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
                           : graph_lds_bytes<double>(t, prog_.slot_count);
   };
   if (opt_.graph_tile > 0) {
+    tuned_ = true;  // a fixed tile disables the autotuner
     tile_ = opt_.graph_tile;
     if (lds_of(tile_) > max_lds_)
       throw InputError("sus2_sh_graph_tile too large for this model's on-chip graph state.");
@@ -300,18 +303,55 @@ void Engine::compute(const sus2sh_box& box, int N, const int* type, const double
   if (N == 0) return;
   if (!type || !pos || !pot || !force || !virial)
     throw ArgumentError("null device pointer passed to compute");
+  const int offsets[2] = {0, N};
   if (opt_.use_float)
-    run<float>(box, N, type, pos, pot, force, virial, stream, param_grad);
+    run<float>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad);
   else
-    run<double>(box, N, type, pos, pot, force, virial, stream, param_grad);
+    run<double>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad);
+}
+
+void Engine::compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom_offsets,
+                           const int* type, const double* pos, double* pot, double* force,
+                           double* virial, hipStream_t stream, double* param_grad)
+{
+  if (nstruct <= 0 || !boxes || !atom_offsets) throw ArgumentError("compute_batch: empty batch");
+  if (atom_offsets[0] != 0) throw ArgumentError("compute_batch: atom_offsets[0] must be 0");
+  for (int s = 0; s < nstruct; ++s)
+    if (atom_offsets[s + 1] < atom_offsets[s])
+      throw ArgumentError("compute_batch: atom_offsets must be non-decreasing");
+  const int N = atom_offsets[nstruct];
+  if (N == 0) return;
+  if (!type || !pos || !pot || !force || !virial)
+    throw ArgumentError("null device pointer passed to compute_batch");
+  if (opt_.use_float)
+    run<float>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
+  else
+    run<double>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
 }
 
 template <typename Real>
-void Engine::run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
-                 double* force, double* virial, hipStream_t s, double* param_grad)
+void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, int N,
+                 const int* type, const double* pos, double* pot, double* force, double* virial,
+                 hipStream_t s, double* param_grad)
 {
-  int ncell = 0;
-  const CellGrid grid = make_grid(box, N, &ncell);
+  const bool batch = nstruct > 1;
+  std::vector<CellGrid> grids(nstruct);
+  std::vector<int> cell_off(nstruct + 1, 0);
+  for (int st = 0; st < nstruct; ++st) {
+    int nc = 0;
+    grids[st] = make_grid(boxes[st], atom_offsets[st + 1] - atom_offsets[st], &nc);
+    if (static_cast<long>(cell_off[st]) + nc > (1L << 30)) throw ArgumentError("too many cells");
+    cell_off[st + 1] = cell_off[st] + nc;
+  }
+  const int ncell = cell_off[nstruct];
+  const CellGrid& grid = grids[0];
+  if (batch) {
+    batch_grids_.upload(grids.data(), grids.size() * sizeof(CellGrid));
+    batch_cell_off_.upload(cell_off.data(), cell_off.size() * sizeof(int));
+    batch_atom_off_.upload(atom_offsets, (nstruct + 1) * sizeof(int));
+    sid_.resize(static_cast<size_t>(N) * sizeof(int));
+    sid_s_.resize(static_cast<size_t>(N) * sizeof(int));
+  }
   if (cap_ == 0) cap_ = 64;
   ensure_atoms(N, sizeof(Edge<Real>));
   cell_begin_.resize(static_cast<size_t>(ncell) * sizeof(int));
@@ -327,7 +367,10 @@ void Engine::run(const sus2sh_box& box, int N, const int* type, const double* po
   }
   if (opt_.profile) hip_check(hipEventRecord(events_[0], s), "hipEventRecord");
 
-  launch_bin(N, pos, grid, key_.as<int>(), val_.as<int>(), wrapped_.as<double>(), s);
+  launch_bin(N, pos, grid, batch ? batch_grids_.as<CellGrid>() : nullptr,
+             batch ? batch_cell_off_.as<int>() : nullptr,
+             batch ? batch_atom_off_.as<int>() : nullptr, nstruct, key_.as<int>(), val_.as<int>(),
+             wrapped_.as<double>(), batch ? sid_.as<int>() : nullptr, s);
   hip_check(radix_sort_pairs(sort_temp_.as<void>(), temp, key_.as<int>(), skey_.as<int>(),
                              val_.as<int>(), satom_.as<int>(), N, bits, s),
             "radix sort");
@@ -336,10 +379,14 @@ void Engine::run(const sus2sh_box& box, int N, const int* type, const double* po
   launch_fill_int(max_count_.as<int>(), 0, 4, s);
   launch_gather(N, model_.species_count, skey_.as<int>(), satom_.as<int>(), wrapped_.as<double>(),
                 type, pos_s_.as<double>(), type_s_.as<int>(), cell_begin_.as<int>(),
-                cell_end_.as<int>(), max_count_.as<int>() + 1, s);
+                cell_end_.as<int>(), max_count_.as<int>() + 1, batch ? sid_.as<int>() : nullptr,
+                batch ? sid_s_.as<int>() : nullptr, s);
   NeighborArgs na{};
   na.N = N;
   na.grid = grid;
+  na.grids = batch ? batch_grids_.as<CellGrid>() : nullptr;
+  na.cell_off = batch ? batch_cell_off_.as<int>() : nullptr;
+  na.sid_s = batch ? sid_s_.as<int>() : nullptr;
   na.rc2 = model_.max_dist * model_.max_dist;
   for (int attempt = 0; attempt < 2; ++attempt) {
     na.cap = cap_;
@@ -420,6 +467,43 @@ void Engine::run(const sus2sh_box& box, int N, const int* type, const double* po
     pgo.n_stored = static_cast<int>(prog_.stored_scalar_index.size());
     pgo.Q = Q;
     pgo.S = S;
+  }
+  if (opt_.autotune && !tuned_ && !param_grad && N >= 4096) {
+    // Autotuner: time every tile whose LDS footprint fits, keep the fastest (HIP events on
+    // the engine's stream; the graph kernel is idempotent, so the trial launches only
+    // rewrite e_s and grads).
+    hipEvent_t t0, t1;
+    hip_check(hipEventCreate(&t0), "hipEventCreate");
+    hip_check(hipEventCreate(&t1), "hipEventCreate");
+    float best = 1e30f;
+    int best_tile = tile_;
+    tune_ms_.assign(5, -1.0f);
+    int slot = 0;
+    for (int t : {1, 2, 4, 8, 16}) {
+      const bool fits = graph_lds_bytes<Real>(t, prog_.slot_count) <= max_lds_;
+      if (fits) {
+        float ms = 0.0f;
+        launch_graph<Real>(t, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
+                           grads_.as<Real>(), nullptr, s, max_lds_, 0);  // warm-up
+        hip_check(hipEventRecord(t0, s), "hipEventRecord");
+        for (int rep = 0; rep < 2; ++rep)
+          launch_graph<Real>(t, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
+                             grads_.as<Real>(), nullptr, s, max_lds_, 0);
+        hip_check(hipEventRecord(t1, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+        tune_ms_[slot] = ms / 2;
+        if (ms < best) {
+          best = ms;
+          best_tile = t;
+        }
+      }
+      ++slot;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    tile_ = best_tile;
+    tuned_ = true;
   }
   if (!launch_graph<Real>(tile_, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
                           grads_.as<Real>(), param_grad ? &pgo : nullptr, s, max_lds_,

@@ -57,6 +57,7 @@ struct EngineOptions {
   bool use_float = true;
   int graph_tile = 0;  // 0: automatic
   bool profile = false;
+  bool autotune = true;  // time every feasible graph tile on the first call
 };
 
 class Engine {
@@ -66,6 +67,11 @@ class Engine {
 
   void compute(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
                double* force, double* virial, hipStream_t stream, double* param_grad);
+  // Batch of independent periodic structures: structure s owns atoms
+  // [atom_offsets[s], atom_offsets[s+1]) and has box boxes[s] (host arrays).
+  void compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom_offsets,
+                     const int* type, const double* pos, double* pot, double* force,
+                     double* virial, hipStream_t stream, double* param_grad);
 
   // Velocity-Verlet NVE: nsteps steps of dt_fs; energy_trace (host, may be null) receives
   // (sum potential, sum kinetic) in eV for the initial state and after every step.
@@ -77,6 +83,7 @@ class Engine {
   const GraphProgram& program() const { return prog_; }
   const EngineOptions& options() const { return opt_; }
   int graph_tile() const { return tile_; }
+  const std::vector<float>& tune_ms() const { return tune_ms_; }
   int neighbor_capacity() const { return cap_; }
   int param_count() const;
   const std::vector<float>& stage_ms() const { return stage_ms_; }
@@ -86,8 +93,9 @@ class Engine {
   template <typename Real>
   void upload_tables();
   template <typename Real>
-  void run(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
-           double* force, double* virial, hipStream_t stream, double* param_grad);
+  void run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, int N, const int* type,
+           const double* pos, double* pot, double* force, double* virial, hipStream_t stream,
+           double* param_grad);
   void ensure_atoms(int N, size_t edge_bytes);
   void finish_param_grad(double* d_param_grad, hipStream_t s);
   CellGrid make_grid(const sus2sh_box& box, int N, int* ncell) const;
@@ -96,6 +104,8 @@ class Engine {
   GraphProgram prog_;
   EngineOptions opt_;
   int tile_ = 1;
+  bool tuned_ = false;
+  std::vector<float> tune_ms_;  // measured graph-kernel time per candidate tile (1,2,4,8,16)
   int cap_ = 0;
   size_t max_lds_ = 65536;
   int device_ = 0;
@@ -115,6 +125,7 @@ class Engine {
   DeviceBuffer cell_begin_, cell_end_, sort_temp_, max_count_;
   DeviceBuffer nl_, basics_, grads_, e_s_, f_s_, v_s_;
   DeviceBuffer pg_rad_, pg_misc_, md_mass_, md_energy_;
+  DeviceBuffer batch_grids_, batch_cell_off_, batch_atom_off_, sid_, sid_s_;
   int* host_max_ = nullptr;  // pinned
 
   std::vector<float> stage_ms_;

@@ -48,20 +48,27 @@ struct ParamGradOut {
 
 struct NeighborArgs {
   int N;
-  CellGrid grid;
+  CellGrid grid;              // single structure
+  const CellGrid* grids;      // batch: per-structure grids (nullptr for a single structure)
+  const int* cell_off;        // batch: first cell key of each structure
+  const int* sid_s;           // batch: structure of each sorted atom
   double rc2;
   int cap;
 };
 
 // --- neighbour build (neighbor.hip) ---
-void launch_bin(int N, const double* pos, const CellGrid& g, int* cell_key, int* atom_val,
-                double* wrapped, hipStream_t s);
+// grids == nullptr: one structure with grid g; otherwise atom_off [nstruct+1] (device) maps
+// atoms to structures, and sid receives each atom's structure.
+void launch_bin(int N, const double* pos, const CellGrid& g, const CellGrid* grids,
+                const int* cell_off, const int* atom_off, int nstruct, int* cell_key,
+                int* atom_val, double* wrapped, int* sid, hipStream_t s);
 void launch_fill_int(int* p, int value, size_t n, hipStream_t s);
 void launch_fill_double(double* p, double value, size_t n, hipStream_t s);
 // bad_type: set to 1 when a type is outside [0, S) (clamped to 0 for the computation)
 void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
                    const double* wrapped, const int* type, double* pos_s, int* type_s,
-                   int* cell_begin, int* cell_end, int* bad_type, hipStream_t s);
+                   int* cell_begin, int* cell_end, int* bad_type, const int* sid, int* sid_s,
+                   hipStream_t s);
 template <typename Real>
 void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sorted_key,
                       const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,

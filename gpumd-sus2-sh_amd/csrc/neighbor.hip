@@ -16,11 +16,36 @@ namespace sus2sh {
 
 namespace {
 
-__global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g, int* __restrict__ key,
-                      int* __restrict__ val, double* __restrict__ wrapped)
+// Structure of atom i in a batch (atom_off: [nstruct + 1] prefix offsets).
+__device__ __forceinline__ int structure_of(int i, const int* __restrict__ atom_off, int nstruct)
+{
+  int lo = 0, hi = nstruct - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (atom_off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Wrap into the box (fractional coordinates), cell key.  BATCH: every structure has its own
+// box and cell grid (grids[s]) and its cells occupy keys [cell_off[s], cell_off[s+1]).
+template <bool BATCH>
+__global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g1,
+                      const CellGrid* __restrict__ grids, const int* __restrict__ cell_off,
+                      const int* __restrict__ atom_off, int nstruct, int* __restrict__ key,
+                      int* __restrict__ val, double* __restrict__ wrapped, int* __restrict__ sid)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  int st = 0;
+  if (BATCH) {
+    st = structure_of(i, atom_off, nstruct);
+    sid[i] = st;
+  }
+  const CellGrid& g = BATCH ? grids[st] : g1;
   const double x = pos[i], y = pos[N + i], z = pos[2 * N + i];
   double s[3] = {g.hi[0] * x + g.hi[1] * y + g.hi[2] * z, g.hi[3] * x + g.hi[4] * y + g.hi[5] * z,
                  g.hi[6] * x + g.hi[7] * y + g.hi[8] * z};
@@ -38,7 +63,7 @@ __global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g, int* __
   wrapped[i] = g.h[0] * s[0] + g.h[1] * s[1] + g.h[2] * s[2];
   wrapped[N + i] = g.h[3] * s[0] + g.h[4] * s[1] + g.h[5] * s[2];
   wrapped[2 * N + i] = g.h[6] * s[0] + g.h[7] * s[1] + g.h[8] * s[2];
-  key[i] = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+  key[i] = (BATCH ? cell_off[st] : 0) + (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
   val[i] = i;
 }
 
@@ -57,7 +82,8 @@ __global__ void k_fill_double(double* __restrict__ p, double v, size_t n)
 __global__ void k_gather(int N, int S, const int* __restrict__ skey, const int* __restrict__ satom,
                          const double* __restrict__ wrapped, const int* __restrict__ type,
                          double* __restrict__ pos_s, int* __restrict__ type_s,
-                         int* __restrict__ cbeg, int* __restrict__ cend, int* __restrict__ bad_type)
+                         int* __restrict__ cbeg, int* __restrict__ cend, int* __restrict__ bad_type,
+                         const int* __restrict__ sid, int* __restrict__ sid_s)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
@@ -71,6 +97,7 @@ __global__ void k_gather(int N, int S, const int* __restrict__ skey, const int* 
     t = 0;
   }
   type_s[p] = t;
+  if (sid_s) sid_s[p] = sid[i];
   const int k = skey[p];
   if (p == 0 || skey[p - 1] != k) cbeg[k] = p;
   if (p == N - 1 || skey[p + 1] != k) cend[k] = p + 1;
@@ -83,7 +110,7 @@ __device__ __forceinline__ int wrap_cell(int c, int n, int& shift)
   return c - shift * n;
 }
 
-template <typename Real>
+template <typename Real, bool BATCH>
 __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double* __restrict__ pos,
                                                    const int* __restrict__ skey,
                                                    const int* __restrict__ cbeg,
@@ -96,8 +123,10 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
   const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p0 < N;
   const int p = valid ? p0 : N - 1;  // inactive lanes repeat the last atom, store nothing
-  const CellGrid& g = a.grid;
-  const int key = skey[p];
+  const int st = BATCH ? a.sid_s[p] : 0;
+  const CellGrid& g = BATCH ? a.grids[st] : a.grid;
+  const int base = BATCH ? a.cell_off[st] : 0;
+  const int key = skey[p] - base;
   const int cx = key % g.n[0];
   const int cy = (key / g.n[0]) % g.n[1];
   const int cz = key / (g.n[0] * g.n[1]);
@@ -124,7 +153,7 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
         } else if (x < 0 || x >= g.n[0]) {
           continue;
         }
-        const int cc = (z * g.n[1] + y) * g.n[0] + x;
+        const int cc = base + (z * g.n[1] + y) * g.n[0] + x;
         const double shx = sx * g.h[0] + sy * g.h[1] + sz * g.h[2];
         const double shy = sx * g.h[3] + sy * g.h[4] + sz * g.h[5];
         const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
@@ -184,10 +213,16 @@ inline int blocks_for(size_t n, int bs) { return static_cast<int>((n + bs - 1) /
 
 }  // namespace
 
-void launch_bin(int N, const double* pos, const CellGrid& g, int* cell_key, int* atom_val,
-                double* wrapped, hipStream_t s)
+void launch_bin(int N, const double* pos, const CellGrid& g, const CellGrid* grids,
+                const int* cell_off, const int* atom_off, int nstruct, int* cell_key,
+                int* atom_val, double* wrapped, int* sid, hipStream_t s)
 {
-  k_bin<<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, cell_key, atom_val, wrapped);
+  if (grids)
+    k_bin<true><<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, grids, cell_off, atom_off, nstruct,
+                                                   cell_key, atom_val, wrapped, sid);
+  else
+    k_bin<false><<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, grids, cell_off, atom_off, nstruct,
+                                                    cell_key, atom_val, wrapped, sid);
 }
 
 void launch_fill_int(int* p, int value, size_t n, hipStream_t s)
@@ -204,10 +239,11 @@ void launch_fill_double(double* p, double value, size_t n, hipStream_t s)
 
 void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
                    const double* wrapped, const int* type, double* pos_s, int* type_s,
-                   int* cell_begin, int* cell_end, int* bad_type, hipStream_t s)
+                   int* cell_begin, int* cell_end, int* bad_type, const int* sid, int* sid_s,
+                   hipStream_t s)
 {
   k_gather<<<blocks_for(N, 256), 256, 0, s>>>(N, S, sorted_key, sorted_atom, wrapped, type, pos_s,
-                                               type_s, cell_begin, cell_end, bad_type);
+                                               type_s, cell_begin, cell_end, bad_type, sid, sid_s);
 }
 
 template <typename Real>
@@ -215,8 +251,12 @@ void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sor
                       const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,
                       int* max_count, hipStream_t s)
 {
-  k_neighbors<Real><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
-                                                         cell_end, count, nl, max_count);
+  if (a.grids)
+    k_neighbors<Real, true><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
+                                                                 cell_end, count, nl, max_count);
+  else
+    k_neighbors<Real, false><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
+                                                                  cell_end, count, nl, max_count);
 }
 
 template void launch_neighbors<float>(const NeighborArgs&, const double*, const int*, const int*,

@@ -22,6 +22,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_destroy",
     "sus2sh_compute",
     "sus2sh_compute_param_grad",
+    "sus2sh_compute_batch",
     "sus2sh_md_nve",
     "sus2sh_last_error",
     "sus2sh_cutoff",
@@ -81,11 +82,22 @@ class ModelInfo(ctypes.Structure):
 _lib = None
 
 
+def _load_torch_runtime_first():
+    """torch bundles its own HIP runtime with the same soname (libamdhip64.so.7) as the system
+    runtime libsus2sh.so links; whichever is loaded first serves both.  Importing torch first
+    keeps torch working in processes that use both."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     """Load libsus2sh.so once; raise if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    _load_torch_runtime_first()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C gpumd-sus2-sh_amd` "
@@ -105,6 +117,9 @@ def lib():
     L.sus2sh_compute_param_grad.argtypes = [c_void_p, P(Box), c_int, c_void_p, c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]
     L.sus2sh_compute_param_grad.restype = c_int
+    L.sus2sh_compute_batch.argtypes = [c_void_p, c_int, P(Box), P(c_int), c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.sus2sh_compute_batch.restype = c_int
     L.sus2sh_md_nve.argtypes = [c_void_p, P(Box), c_int, c_void_p, P(c_double), c_void_p,
                                 c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p,
                                 P(c_double), c_void_p]

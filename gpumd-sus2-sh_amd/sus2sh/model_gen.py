@@ -101,7 +101,8 @@ class ModelSpec:
 
 def synthetic_model(l_max, k_max, body_order=5, body_l_max=None, species_count=2, rb_size=8,
                     max_dist=5.0, radial_basis_type="RBChebyshev_sss", seed=0, coeff_scale=0.1,
-                    scaling=1.0, type_coeff_mode="one_plus", scal_mode="uniform"):
+                    scaling=1.0, type_coeff_mode="one_plus", scal_mode="uniform",
+                    pair_core=None):
     """Seeded synthetic model on the standard graph.
 
     Coefficients: radial_coeffs, shift_coeffs, species_coeffs and moment_coeffs are drawn
@@ -110,6 +111,10 @@ def synthetic_model(l_max, k_max, body_order=5, body_l_max=None, species_count=2
     pair scale ~1e-2) or N(0, coeff_scale) with "normal".  scal_coeffs: with
     scal_mode="uniform" every (mu, pair) uses a = 1, s = max_dist / 2; with "random"
     a ~ U[0.7, 1.3] and s ~ max_dist/2 + N(0, 0.3), independently per (mu, pair).
+    pair_core (eV, optional): adds a repulsive pair core for molecular dynamics -- the
+    l = 0 radial rows get c_0 += 1 (so R ~ (r - rc)^2, decreasing up to rc), the species
+    coefficients get +1 and every body-2 scalar weight is set to +pair_core.  Purely random
+    coefficients have no repulsion and heat up without bound under NVE.
     """
     g = standard_graph(l_max, k_max, body_order, body_l_max)
     rng = np.random.default_rng(seed)
@@ -130,6 +135,12 @@ def synthetic_model(l_max, k_max, body_order=5, body_l_max=None, species_count=2
     else:
         scal[:, 0, :] = rng.uniform(0.7, 1.3, size=(Mr, S * S))
         scal[:, 1, :] = max_dist / 2.0 + rng.normal(0.0, 0.3, size=(Mr, S * S))
+    if pair_core is not None:
+        for mu in range(Mr):
+            if mu % (l_max + 1) == 0:  # mu = k (L+1) + l with l = 0
+                radial[mu, 0] += 1.0
+        species = species + 1.0
+        moment = np.where(g.scalar_body == 2, float(pair_core), moment)
     return ModelSpec(g, S, radial_basis_type, rb_size, max_dist, scaling, shift, species,
                      scal.reshape(-1), moment, radial, tcoef,
                      name=f"l{l_max}k{k_max}_b{body_order}_seed{seed}")

@@ -3,8 +3,8 @@
 `SUS2_SH(file, num_atoms, options)` and `SUS2_SH.compute(box, type, position, potential,
 force, virial)` follow src/force/sus2_sh.cuh:12-25: outputs are accumulated (+=) into
 caller-owned device arrays in GPUMD's SoA layout (position/force [3, N], virial [9, N] rows
-xx yy zz xy xz yz yx zx zy).  Device memory is provided by torch tensors (plumbing only);
-all arithmetic happens in the HIP kernels of libsus2sh.so.
+xx yy zz xy xz yz yx zx zy).  Device memory comes from torch tensors or hipmem.DeviceArray
+(plumbing only); all arithmetic happens in the HIP kernels of libsus2sh.so.
 """
 import ctypes
 
@@ -16,6 +16,29 @@ from .structures import make_box
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _is_torch(t):
+    return type(t).__module__.startswith("torch")
+
+
+def _check_arrays(pairs):
+    """Device arrays are torch cuda tensors or hipmem.DeviceArray objects."""
+    for t, dt in pairs:
+        if _is_torch(t):
+            if str(t.dtype) != "torch." + dt or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("arguments must be contiguous device tensors of dtype " + dt)
+        elif str(getattr(t, "dtype", "")) != dt:
+            raise ValueError("arguments must be device arrays of dtype " + dt)
+
+
+def _stream(t, stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    if _is_torch(t):
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(0)
 
 
 def box_struct(cell, pbc=(1, 1, 1)):
@@ -76,18 +99,13 @@ class SUS2_SH:
         [3, N] (contiguous SoA), potential: float64 [N], force: float64 [3, N], virial:
         float64 [9, N]; param_grad (optional): float64 [param_count] += dE/dtheta.
         """
-        import torch
         n = int(type.numel())
-        for t, dt in ((type, torch.int32), (position, torch.float64), (potential, torch.float64),
-                      (force, torch.float64), (virial, torch.float64)):
-            if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
-                raise ValueError("compute: arguments must be contiguous cuda tensors of the "
-                                 "documented dtypes")
+        _check_arrays(((type, "int32"), (position, "float64"), (potential, "float64"),
+                       (force, "float64"), (virial, "float64")))
         if position.numel() != 3 * n or potential.numel() != n or force.numel() != 3 * n \
                 or virial.numel() != 9 * n:
             raise ValueError("compute: inconsistent array sizes")
-        s = ctypes.c_void_p(stream if stream is not None else
-                            torch.cuda.current_stream().cuda_stream)
+        s = _stream(type, stream)
         if param_grad is None:
             code = lib().sus2sh_compute(self._h, ctypes.byref(box), n, _ptr(type), _ptr(position),
                                         _ptr(potential), _ptr(force), _ptr(virial), s)
@@ -98,18 +116,43 @@ class SUS2_SH:
         check(code, self._h)
 
 
+def compute_batch(pot, cells, atom_offsets, type, position, potential, force, virial,
+                  pbc=(1, 1, 1), param_grad=None, stream=None):
+    """Batched evaluation (sus2sh_compute_batch) of independent periodic structures whose atoms
+    are concatenated in the SoA arrays; cells: list of 3x3 (rows = cell vectors)."""
+    ns = len(cells)
+    boxes = (Box * ns)()
+    for k, c in enumerate(cells):
+        boxes[k] = box_struct(c, pbc)
+    offs = np.ascontiguousarray(atom_offsets, dtype=np.int32)
+    assert len(offs) == ns + 1
+    s = _stream(type, stream)
+    code = lib().sus2sh_compute_batch(pot._h, ns, boxes,
+                                      offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      _ptr(type), _ptr(position), _ptr(potential), _ptr(force),
+                                      _ptr(virial),
+                                      _ptr(param_grad) if param_grad is not None else None, s)
+    check(code, pot._h)
+
+
 def md_nve(pot, box, type, position, velocity, masses, dt_fs, nsteps, stream=None):
     """Velocity-Verlet NVE on the GPU (sus2sh_md_nve).  position/velocity: float64 [3, N]
     cuda tensors advanced in place; masses: per-species amu.  Returns (energy trace
     [nsteps+1, 2] of (potential, kinetic) in eV, potential[N], force[3,N], virial[9,N])."""
-    import torch
     n = int(type.numel())
     m = np.ascontiguousarray(masses, dtype=np.float64)
-    e = torch.zeros(n, dtype=torch.float64, device=position.device)
-    f = torch.zeros(3, n, dtype=torch.float64, device=position.device)
-    v = torch.zeros(9, n, dtype=torch.float64, device=position.device)
+    if _is_torch(position):
+        import torch
+        e = torch.zeros(n, dtype=torch.float64, device=position.device)
+        f = torch.zeros(3, n, dtype=torch.float64, device=position.device)
+        v = torch.zeros(9, n, dtype=torch.float64, device=position.device)
+    else:
+        from .hipmem import DeviceArray
+        e = DeviceArray.zeros((n,), np.float64)
+        f = DeviceArray.zeros((3, n), np.float64)
+        v = DeviceArray.zeros((9, n), np.float64)
     trace = np.zeros((nsteps + 1, 2))
-    s = ctypes.c_void_p(stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+    s = _stream(type, stream)
     code = lib().sus2sh_md_nve(pot._h, ctypes.byref(box), n, _ptr(type),
                                m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(position),
                                _ptr(velocity), float(dt_fs), int(nsteps), _ptr(e), _ptr(f), _ptr(v),
@@ -130,18 +173,19 @@ def maxwell_velocities(masses_per_atom, temperature, seed):
     return v
 
 
-def evaluate(model_path, cell, types, positions, options=(), pbc=(1, 1, 1), device="cuda"):
-    """One evaluation from host arrays; returns numpy (energy[N], force[3,N], virial[9,N])."""
-    import torch
+def evaluate(model_path, cell, types, positions, options=(), pbc=(1, 1, 1)):
+    """One evaluation from host arrays; returns numpy (energy[N], force[3,N], virial[9,N]).
+    Device buffers come from hipmem (no torch needed)."""
+    from .hipmem import DeviceArray, synchronize
     n = len(types)
     pot = SUS2_SH(model_path, n, options)
-    t = torch.as_tensor(np.asarray(types, dtype=np.int32), device=device)
-    x = torch.as_tensor(np.ascontiguousarray(positions, dtype=np.float64), device=device)
-    e = torch.zeros(n, dtype=torch.float64, device=device)
-    f = torch.zeros(3, n, dtype=torch.float64, device=device)
-    v = torch.zeros(9, n, dtype=torch.float64, device=device)
+    t = DeviceArray.from_numpy(np.asarray(types, dtype=np.int32))
+    x = DeviceArray.from_numpy(np.ascontiguousarray(positions, dtype=np.float64))
+    e = DeviceArray.zeros((n,), np.float64)
+    f = DeviceArray.zeros((3, n), np.float64)
+    v = DeviceArray.zeros((9, n), np.float64)
     pot.compute(box_struct(cell, pbc), t, x, e, f, v)
-    torch.cuda.synchronize()
-    out = (e.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy())
+    synchronize()
+    out = (e.numpy(), f.numpy(), v.numpy())
     pot.close()
     return out

@@ -100,6 +100,17 @@ int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int 
                               double* d_potential, double* d_force, double* d_virial,
                               double* d_param_grad, void* stream);
 
+/* Batched evaluation of independent periodic structures in one pass of every kernel (one
+ * launch per stage for the whole batch).  Structure s owns atoms [atom_offsets[s],
+ * atom_offsets[s+1]) of the concatenated SoA arrays (N = atom_offsets[num_structures]) and
+ * has box boxes[s]; boxes and atom_offsets are host arrays.  Outputs are per atom and
+ * accumulated as in sus2sh_compute; d_param_grad (may be NULL) += dE/dtheta summed over all
+ * structures.  No counterpart in the reference (GPUMD evaluates one box per call). */
+int sus2sh_compute_batch(sus2sh_handle* handle, int num_structures, const sus2sh_box* boxes,
+                         const int* atom_offsets, const int* d_type, const double* d_position,
+                         double* d_potential, double* d_force, double* d_virial,
+                         double* d_param_grad, void* stream);
+
 /* Velocity-Verlet NVE driver around sus2sh_compute: nsteps steps of dt_fs femtoseconds.
  * masses: host double[species_count] in amu; d_position / d_velocity (A, A/fs, SoA [3N]) are
  * advanced in place; d_potential / d_force / d_virial hold the last evaluation (overwritten,

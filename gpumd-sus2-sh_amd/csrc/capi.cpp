@@ -146,6 +146,20 @@ int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
   });
 }
 
+int sus2sh_set_params(sus2sh_handle* handle, const double* theta, int n)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&handle->error, [&] { handle->engine->set_params(theta, n); });
+}
+
+int sus2sh_get_params(const sus2sh_handle* handle, double* theta, int n)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  std::string err;
+  const int rc = guarded(&err, [&] { handle->engine->get_params(theta, n); });
+  return rc;
+}
+
 const char* sus2sh_last_error(const sus2sh_handle* handle)
 {
   if (!handle) return g_create_error.c_str();

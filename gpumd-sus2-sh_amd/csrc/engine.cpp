@@ -531,6 +531,41 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
 }
 
+void Engine::get_params(double* theta, int n) const
+{
+  if (!theta || n != param_count()) throw ArgumentError("get_params: wrong parameter count");
+  const Model& m = model_;
+  double* o = theta;
+  o = std::copy(m.shift_coeffs.begin(), m.shift_coeffs.end(), o);
+  o = std::copy(m.species_coeffs.begin(), m.species_coeffs.end(), o);
+  o = std::copy(m.moment_coeffs.begin(), m.moment_coeffs.end(), o);
+  o = std::copy(m.radial_coeffs.begin(), m.radial_coeffs.end(), o);
+  o = std::copy(m.radial_type_coeffs.begin(), m.radial_type_coeffs.end(), o);
+  std::copy(m.scal_coeffs.begin(), m.scal_coeffs.end(), o);
+}
+
+void Engine::set_params(const double* theta, int n)
+{
+  if (!theta || n != param_count()) throw ArgumentError("set_params: wrong parameter count");
+  Model& m = model_;
+  const double* p = theta;
+  auto take = [&](std::vector<double>& v) {
+    std::copy(p, p + v.size(), v.begin());
+    p += v.size();
+  };
+  take(m.shift_coeffs);
+  take(m.species_coeffs);
+  take(m.moment_coeffs);
+  take(m.radial_coeffs);
+  take(m.radial_type_coeffs);
+  take(m.scal_coeffs);
+  prog_ = compile_graph(m);
+  if (opt_.use_float)
+    upload_tables<float>();
+  else
+    upload_tables<double>();
+}
+
 void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double* masses,
                     double* pos, double* vel, double dt_fs, int nsteps, double* pot, double* force,
                     double* virial, double* energy_trace, hipStream_t s)

@@ -79,6 +79,10 @@ class Engine {
               double* vel, double dt_fs, int nsteps, double* pot, double* force, double* virial,
               double* energy_trace, hipStream_t stream);
 
+  // Replace the model coefficients (dE/dtheta order) and re-upload the device tables.
+  void set_params(const double* theta, int n);
+  void get_params(double* theta, int n) const;
+
   const Model& model() const { return model_; }
   const GraphProgram& program() const { return prog_; }
   const EngineOptions& options() const { return opt_; }

@@ -24,6 +24,8 @@ EXPORTED_SYMBOLS = (
     "sus2sh_compute_param_grad",
     "sus2sh_compute_batch",
     "sus2sh_md_nve",
+    "sus2sh_set_params",
+    "sus2sh_get_params",
     "sus2sh_last_error",
     "sus2sh_cutoff",
     "sus2sh_get_model_info",
@@ -124,6 +126,10 @@ def lib():
                                 c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p,
                                 P(c_double), c_void_p]
     L.sus2sh_md_nve.restype = c_int
+    L.sus2sh_set_params.argtypes = [c_void_p, P(c_double), c_int]
+    L.sus2sh_set_params.restype = c_int
+    L.sus2sh_get_params.argtypes = [c_void_p, P(c_double), c_int]
+    L.sus2sh_get_params.restype = c_int
     L.sus2sh_last_error.argtypes = [c_void_p]
     L.sus2sh_last_error.restype = c_char_p
     L.sus2sh_cutoff.argtypes = [c_void_p]

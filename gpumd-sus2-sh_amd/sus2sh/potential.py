@@ -91,6 +91,18 @@ class SUS2_SH:
     def mean_neighbors(self):
         return lib().sus2sh_mean_neighbors(self._h)
 
+    def get_params(self):
+        n = self.info()["param_count"]
+        out = np.zeros(n)
+        check(lib().sus2sh_get_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      n), self._h)
+        return out
+
+    def set_params(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        check(lib().sus2sh_set_params(self._h, theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      len(theta)), self._h)
+
     def compute(self, box, type, position, potential, force, virial, stream=None,
                 param_grad=None):
         """Accumulate per-atom energy, forces and virials.

@@ -123,6 +123,12 @@ int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                   double* d_velocity, double dt_fs, int nsteps, double* d_potential,
                   double* d_force, double* d_virial, double* energy_trace, void* stream);
 
+/* Read / replace the model coefficients in the dE/dtheta order (host double[param_count]).
+ * set_params re-uploads the device tables; the graph topology is unchanged.  For training
+ * and gradient checks; no reference counterpart (the reference reads coefficients once). */
+int sus2sh_set_params(sus2sh_handle* handle, const double* theta, int n);
+int sus2sh_get_params(const sus2sh_handle* handle, double* theta, int n);
+
 /* Last error message of a handle, or of the last failed sus2sh_create on this thread
  * when handle is NULL.  Never NULL. */
 const char* sus2sh_last_error(const sus2sh_handle* handle);

@@ -93,10 +93,11 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
             stage_sum = st if stage_sum is None else [a + b for a, b in zip(stage_sum, st)]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    info = pot.info()
     nbar = pot.mean_neighbors()
+    info = pot.info()
+    tuning = {f"v{v}_w{w}": round(ms, 4) for (v, w), ms in pot.tuning().items()}
     out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
-               info=info, nbar=nbar, energy=float(e.sum().item()),
+               info=info, nbar=nbar, energy=float(e.sum().item()), tuning=tuning,
                force=f.cpu().numpy(), stream=stream.cuda_stream)
     if stage_sum is not None:
         out["stage_ms"] = dict(zip(["neighbor", "basic", "graph", "force", "scatter"],
@@ -197,6 +198,12 @@ def main():
         "rms_force_err_fp32_vs_fp64": frms,
         "stage_ms_fp32": stages,
         "stage_ms_fp64": r64["stage_ms"],
+        "graph_tile_fp32": r32["info"]["graph_tile"],
+        "graph_variant_fp32": r32["info"]["graph_variant"],
+        "graph_tile_fp64": r64["info"]["graph_tile"],
+        "graph_variant_fp64": r64["info"]["graph_variant"],
+        "graph_tuning_ms_fp32": r32["tuning"],
+        "graph_tuning_ms_fp64": r64["tuning"],
         "step_tflops_fp32": total_flops / (r32["ms_per_step"] * 1e-3) / 1e12,
         "roofline": {"bound": "valu", "kernel": dominant, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

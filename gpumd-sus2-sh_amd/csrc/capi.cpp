@@ -196,6 +196,7 @@ int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out)
   out->param_count = handle->engine->param_count();
   out->stored_slots = g.slot_count;
   out->graph_tile = handle->engine->graph_tile();
+  out->graph_variant = handle->engine->graph_variant();
   out->neighbor_capacity = handle->engine->neighbor_capacity();
   out->max_dist = m.max_dist;
   return SUS2SH_OK;
@@ -207,6 +208,15 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n)
   const std::vector<float>& st = handle->engine->stage_ms();
   const int k = std::min<int>(n, static_cast<int>(st.size()));
   for (int i = 0; i < k; ++i) ms[i] = st[i];
+  return k;
+}
+
+int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n)
+{
+  if (!handle || !ms || n < 0) return 0;
+  const std::vector<float>& t = handle->engine->tune_ms();
+  const int k = std::min<int>(n, static_cast<int>(t.size()));
+  for (int i = 0; i < k; ++i) ms[i] = t[i];
   return k;
 }
 

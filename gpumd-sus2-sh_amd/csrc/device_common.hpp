@@ -90,10 +90,42 @@ __device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
   const Real env = dr * dr;
   const Real denv = Real(2) * dr;
   Real xi = 0, dxi = 0;
-  if (uniform) radial_map<Real>(kind, scal[0], scal[1], r, xi, dxi);
+  if (uniform) {
+    // every mu shares xi: the Chebyshev values T_n(xi), T'_n(xi) are computed once per edge
+    radial_map<Real>(kind, scal[0], scal[1], r, xi, dxi);
+    constexpr int kMaxN = 16;
+    Real T[kMaxN], dT[kMaxN];
+    T[0] = Real(1);
+    dT[0] = Real(0);
+    T[1] = xi;
+    dT[1] = Real(1);
+#pragma unroll
+    for (int n = 2; n < kMaxN; ++n) {
+      if (n < rb) {
+        T[n] = Real(2) * xi * T[n - 1] - T[n - 2];
+        if (DERIV) dT[n] = Real(2) * (T[n - 1] + xi * dT[n - 1]) - dT[n - 2];
+      }
+    }
+#pragma unroll
+    for (int mu = 0; mu < MR; ++mu) {
+      const Real* c = coeff + mu * rb;
+      Real acc = 0, accd = 0;
+#pragma unroll
+      for (int n = 0; n < kMaxN; ++n) {
+        if (n < rb) {
+          const Real cn = c[n];
+          acc += cn * T[n];
+          if (DERIV) accd += cn * dT[n];
+        }
+      }
+      R[mu] = env * acc;
+      if (DERIV) dR[mu] = denv * acc + env * dxi * accd;
+    }
+    return;
+  }
 #pragma unroll
   for (int mu = 0; mu < MR; ++mu) {
-    if (!uniform) radial_map<Real>(kind, scal[2 * mu], scal[2 * mu + 1], r, xi, dxi);
+    radial_map<Real>(kind, scal[2 * mu], scal[2 * mu + 1], r, xi, dxi);
     const Real* c = coeff + mu * rb;
     Real t0 = Real(1), t1 = xi, d0 = Real(0), d1 = Real(1);
     Real acc = c[0];

@@ -47,6 +47,10 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.profile = parse_bool(val, key);
     } else if (key == "sus2_sh_autotune") {
       o.autotune = parse_bool(val, key);
+    } else if (key == "sus2_sh_graph_variant") {
+      const int v = std::atoi(val.c_str());
+      if (v < 0 || v > 2) throw InputError("sus2_sh_graph_variant must be 0, 1 or 2: " + val);
+      o.graph_variant = v;
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -115,6 +119,7 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     return opt_.use_float ? graph_lds_bytes<float>(t, prog_.slot_count)
                           : graph_lds_bytes<double>(t, prog_.slot_count);
   };
+  if (opt_.graph_variant >= 0) variant_ = opt_.graph_variant;
   if (opt_.graph_tile > 0) {
     tuned_ = true;  // a fixed tile disables the autotuner
     tile_ = opt_.graph_tile;
@@ -218,6 +223,25 @@ void Engine::upload_tables()
   bwd_terms_.upload(bt.data(), bt.size() * sizeof(Term<Real>));
   bwd_layer_.upload(g.bwd_layer_begin.data(), g.bwd_layer_begin.size() * sizeof(int));
 
+  auto upload_stream = [&](const std::vector<uint32_t>& ab, const std::vector<double>& c,
+                           DeviceBuffer& dst) {
+    std::vector<Term<Real>> v(ab.size());
+    for (size_t k = 0; k < v.size(); ++k) {
+      v[k] = Term<Real>{};
+      v[k].idx = ab[k];
+      v[k].c = static_cast<Real>(c[k]);
+    }
+    dst.upload(v.data(), v.size() * sizeof(Term<Real>));
+  };
+  for (int v = 0; v < 2; ++v) {
+    const GraphProgram::Streams& st = g.streams[v];
+    upload_stream(st.fwd_ab, st.fwd_c, fwd_ins_[v]);
+    upload_stream(st.bwd_ab, st.bwd_c, bwd_ins_[v]);
+    fwd_sbegin_[v].upload(st.fwd_begin.data(), st.fwd_begin.size() * sizeof(int));
+    fwd_slen_[v].upload(st.fwd_len.data(), st.fwd_len.size() * sizeof(int));
+    bwd_sbegin_[v].upload(st.bwd_begin.data(), st.bwd_begin.size() * sizeof(int));
+    bwd_slen_[v].upload(st.bwd_len.data(), st.bwd_len.size() * sizeof(int));
+  }
   e_slot_.upload(g.energy_slot.data(), g.energy_slot.size() * sizeof(int));
   std::vector<Real> ew(g.energy_weight.begin(), g.energy_weight.end());
   e_w_.upload(ew.data(), ew.size() * sizeof(Real));
@@ -445,6 +469,16 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   gt.seed = seed_.as<Real>();
   gt.shift = shift_.as<double>();
   gt.sigma = sigma_.as<double>();
+  auto use_streams = [&](GraphTable<Real>& t, int v) {
+    t.fwd_ins = fwd_ins_[v].as<Term<Real>>();
+    t.fwd_sbegin = fwd_sbegin_[v].as<int>();
+    t.fwd_slen = fwd_slen_[v].as<int>();
+    t.bwd_ins = bwd_ins_[v].as<Term<Real>>();
+    t.bwd_sbegin = bwd_sbegin_[v].as<int>();
+    t.bwd_slen = bwd_slen_[v].as<int>();
+  };
+  use_streams(gt, variant_ > 0 ? variant_ - 1 : 0);
+  gt.ones_slot = prog_.ones_slot;
   gt.n_e = static_cast<int>(prog_.energy_slot.size());
   gt.slots = prog_.slot_count;
   gt.basics = model_.basic_count;
@@ -469,45 +503,50 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     pgo.S = S;
   }
   if (opt_.autotune && !tuned_ && !param_grad && N >= 4096) {
-    // Autotuner: time every tile whose LDS footprint fits, keep the fastest (HIP events on
-    // the engine's stream; the graph kernel is idempotent, so the trial launches only
-    // rewrite e_s and grads).
+    // Autotuner: time every (variant, tile) whose LDS footprint fits and keep the fastest
+    // (HIP events on the engine's stream; the graph kernel is idempotent, so the trial
+    // launches only rewrite e_s and grads).
     hipEvent_t t0, t1;
     hip_check(hipEventCreate(&t0), "hipEventCreate");
     hip_check(hipEventCreate(&t1), "hipEventCreate");
     float best = 1e30f;
-    int best_tile = tile_;
-    tune_ms_.assign(5, -1.0f);
-    int slot = 0;
-    for (int t : {1, 2, 4, 8, 16}) {
-      const bool fits = graph_lds_bytes<Real>(t, prog_.slot_count) <= max_lds_;
-      if (fits) {
+    int best_tile = tile_, best_variant = variant_;
+    tune_ms_.assign(15, -1.0f);
+    for (int v = 0; v < 3; ++v) {
+      GraphTable<Real> tt = gt;
+      use_streams(tt, v > 0 ? v - 1 : 0);
+      for (int li = 0; li < 5; ++li) {
+        const int t = 1 << li;
+        if (graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
+        const int mode = v == 0 ? 0 : 1;
         float ms = 0.0f;
-        launch_graph<Real>(t, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
-                           grads_.as<Real>(), nullptr, s, max_lds_, 0);  // warm-up
+        launch_graph<Real>(t, mode, N, basics_.as<Real>(), type_s_.as<int>(), tt,
+                           e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
         hip_check(hipEventRecord(t0, s), "hipEventRecord");
         for (int rep = 0; rep < 2; ++rep)
-          launch_graph<Real>(t, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
-                             grads_.as<Real>(), nullptr, s, max_lds_, 0);
+          launch_graph<Real>(t, mode, N, basics_.as<Real>(), type_s_.as<int>(), tt,
+                             e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
         hip_check(hipEventRecord(t1, s), "hipEventRecord");
         hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
         hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
-        tune_ms_[slot] = ms / 2;
+        tune_ms_[v * 5 + li] = ms / 2;
         if (ms < best) {
           best = ms;
           best_tile = t;
+          best_variant = v;
         }
       }
-      ++slot;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     tile_ = best_tile;
+    variant_ = best_variant;
+    use_streams(gt, variant_ > 0 ? variant_ - 1 : 0);
     tuned_ = true;
   }
-  if (!launch_graph<Real>(tile_, N, basics_.as<Real>(), type_s_.as<int>(), gt, e_s_.as<double>(),
-                          grads_.as<Real>(), param_grad ? &pgo : nullptr, s, max_lds_,
-                          4 * cu_count_))
+  if (!launch_graph<Real>(tile_, variant_ == 0 ? 0 : 1, N, basics_.as<Real>(), type_s_.as<int>(),
+                          gt, e_s_.as<double>(), grads_.as<Real>(), param_grad ? &pgo : nullptr, s,
+                          max_lds_, 4 * cu_count_))
     throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
   if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 

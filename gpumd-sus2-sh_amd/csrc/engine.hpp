@@ -56,6 +56,7 @@ class DeviceBuffer {
 struct EngineOptions {
   bool use_float = true;
   int graph_tile = 0;  // 0: automatic
+  int graph_variant = -1;  // -1: automatic; 0 rows, 1 balanced streams, 2 round-robin
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
 };
@@ -87,6 +88,7 @@ class Engine {
   const GraphProgram& program() const { return prog_; }
   const EngineOptions& options() const { return opt_; }
   int graph_tile() const { return tile_; }
+  int graph_variant() const { return variant_; }
   const std::vector<float>& tune_ms() const { return tune_ms_; }
   int neighbor_capacity() const { return cap_; }
   int param_count() const;
@@ -108,8 +110,9 @@ class Engine {
   GraphProgram prog_;
   EngineOptions opt_;
   int tile_ = 1;
+  int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams
   bool tuned_ = false;
-  std::vector<float> tune_ms_;  // measured graph-kernel time per candidate tile (1,2,4,8,16)
+  std::vector<float> tune_ms_;  // graph-kernel ms per candidate, [variant * 5 + log2(tile)]
   int cap_ = 0;
   size_t max_lds_ = 65536;
   int device_ = 0;
@@ -119,6 +122,8 @@ class Engine {
   DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
   DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;
   DeviceBuffer e_slot_, e_w_, seed_, shift_, sigma_;
+  DeviceBuffer fwd_ins_[2], fwd_sbegin_[2], fwd_slen_[2], bwd_ins_[2], bwd_sbegin_[2],
+    bwd_slen_[2];
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
   int cu_count_ = 256;
   bool radial_uniform_ = false;

@@ -38,7 +38,7 @@ __device__ __forceinline__ Real dot_w(const vec_t<Real, W>& a, const vec_t<Real,
   return s;
 }
 
-template <typename Real, int W, int NT, bool PG>
+template <typename Real, int W, int NT, bool PG, bool VM>
 __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ basics,
                                               const int* __restrict__ type_s, GraphTable<Real> gt,
                                               double* __restrict__ e_out,
@@ -46,9 +46,9 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
 {
   using V = vec_t<Real, W>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* vals = reinterpret_cast<V*>(smem_raw);
-  V* grads = vals + gt.slots;
-  V* red = grads + gt.slots;  // [NT / 64] wave partial energies
+  V* vals = reinterpret_cast<V*>(smem_raw);  // [slots + 1]: the last slot holds 0
+  V* grads = vals + gt.slots + 1;             // [slots + 1]: the last slot holds 1
+  V* red = grads + gt.slots + 1;              // [NT / 64] wave partial energies
   V* sig = red + NT / 64;     // sigma of the tile's atoms (0 for padding lanes), PG only
   // PG: [Q] dE/dw, [S] dshift, [S] dsigma, 16-byte aligned after the tile state
   const size_t pw_off = (static_cast<size_t>(reinterpret_cast<unsigned char*>(sig + 1) - smem_raw) + 15) & ~size_t(15);
@@ -57,6 +57,8 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
   Real* grads_s = reinterpret_cast<Real*>(grads);
   const int tid = threadIdx.x;
   const int tiles = (N + W - 1) / W;
+  const uint32_t zero_slot = static_cast<uint32_t>(gt.slots);
+  const uint32_t ones_slot = static_cast<uint32_t>(gt.slots);
 
   if (PG) {
     for (int k = tid; k < pg.Q + 2 * pg.S; k += NT) pw[k] = 0.0;
@@ -71,6 +73,10 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
       vals_s[idx] = atom < N ? basics[static_cast<size_t>(b) * N + atom] : Real(0);
     }
     for (int s = tid; s < gt.slots; s += NT) grads[s] = splat<Real, W>(gt.seed[s]);
+    if (tid == 0) {
+      grads[gt.slots] = splat<Real, W>(Real(1));
+      vals[gt.slots] = splat<Real, W>(Real(0));
+    }
     if (PG && tid < W) {
       const int atom = a0 + tid;
       reinterpret_cast<Real*>(sig)[tid] =
@@ -80,7 +86,51 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
 
     // forward, layer by layer
     V e = splat<Real, W>(Real(0));
-    for (int layer = 0; layer < gt.layers; ++layer) {
+    if (VM) {
+      for (int layer = 0; layer < gt.layers; ++layer) {
+        const int L = gt.fwd_slen[layer];
+        const Term<Real>* ins = gt.fwd_ins + gt.fwd_sbegin[layer] + tid;
+        V acc = splat<Real, W>(Real(0));
+        Term<Real> buf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) buf[j] = L > j ? ins[j * NT] : Term<Real>{};
+        for (int k = 0; k < L; k += 4) {
+          Term<Real> cur[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            cur[j] = buf[j];
+            if (k + 4 + j < L) buf[j] = ins[(k + 4 + j) * NT];
+          }
+          // decode, then issue all LDS loads of the group before any FMA / store: END and
+          // NOP instructions read the zero slot so the FMA path has no branch
+          V x[4], y[4];
+          uint32_t op[4], tg[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t a = cur[j].idx & 0xFFFFu, b = cur[j].idx >> 16;
+            op[j] = a;
+            tg[j] = b;
+            const uint32_t la = a < 0xFFFEu ? a : zero_slot;
+            const uint32_t lb = a < 0xFFFEu ? b : zero_slot;
+            x[j] = vals[la];
+            y[j] = vals[lb];
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            acc += (op[j] < 0xFFFEu ? cur[j].c : Real(0)) * x[j] * y[j];
+            if (op[j] == 0xFFFFu) {
+              if (tg[j] != 0xFFFFu)
+                vals[tg[j]] = acc;
+              else
+                e += cur[j].c * acc;
+              acc = splat<Real, W>(Real(0));
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int layer = 0; !VM && layer < gt.layers; ++layer) {
       const int re = gt.fwd_layer[layer + 1];
       for (int r = gt.fwd_layer[layer] + tid; r < re; r += NT) {
         const int4 row = gt.fwd_rows[r];
@@ -136,7 +186,49 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
     }
 
     // reverse, layer by layer (descending)
-    for (int layer = gt.layers - 1; layer >= 0; --layer) {
+    if (VM) {
+      for (int layer = gt.layers - 1; layer >= 0; --layer) {
+        const int L = gt.bwd_slen[layer];
+        const Term<Real>* ins = gt.bwd_ins + gt.bwd_sbegin[layer] + tid;
+        V acc = splat<Real, W>(Real(0));
+        Term<Real> buf[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) buf[j] = L > j ? ins[j * NT] : Term<Real>{};
+        for (int k = 0; k < L; k += 4) {
+          Term<Real> cur[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            cur[j] = buf[j];
+            if (k + 4 + j < L) buf[j] = ins[(k + 4 + j) * NT];
+          }
+          V x[4], y[4];
+          uint32_t op[4], tg[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t a = cur[j].idx & 0xFFFFu, b = cur[j].idx >> 16;
+            op[j] = a;
+            tg[j] = b;
+            x[j] = grads[a < 0xFFFEu ? b : ones_slot];
+            y[j] = vals[a < 0xFFFEu ? a : zero_slot];
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            acc += (op[j] < 0xFFFEu ? cur[j].c : Real(0)) * x[j] * y[j];
+            if (op[j] == 0xFFFFu) {
+              if (cur[j].c != Real(0)) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) atomicAdd(grads_s + tg[j] * W + w, acc[w]);
+              } else {
+                grads[tg[j]] += acc;
+              }
+              acc = splat<Real, W>(Real(0));
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int layer = gt.layers - 1; !VM && layer >= 0; --layer) {
       const int ce = gt.bwd_layer[layer + 1];
       for (int c = gt.bwd_layer[layer] + tid; c < ce; c += NT) {
         const int4 ch = gt.bwd_chunks[c];
@@ -186,8 +278,20 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
   }
 }
 
+template <typename Real, int W, int NT, bool PG, bool VM>
+void launch_variant(int N, const Real* basics, const int* type_s, const GraphTable<Real>& gt,
+                    double* e_s, Real* grads, const ParamGradOut& pg, hipStream_t s, size_t lds,
+                    size_t max_lds, int blocks)
+{
+  auto kern = &k_graph<Real, W, NT, PG, VM>;
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds));
+  kern<<<blocks, NT, lds, s>>>(N, basics, type_s, gt, e_s, grads, pg);
+}
+
 template <typename Real, int W, int NT>
-bool launch_tile(int N, const Real* basics, const int* type_s, const GraphTable<Real>& gt,
+bool launch_tile(int mode, int N, const Real* basics, const int* type_s, const GraphTable<Real>& gt,
                  double* e_s, Real* grads, const ParamGradOut* pg, hipStream_t s, size_t max_lds,
                  int persistent_blocks)
 {
@@ -196,19 +300,15 @@ bool launch_tile(int N, const Real* basics, const int* type_s, const GraphTable<
   if (lds > max_lds) return false;
   const int tiles = (N + W - 1) / W;
   if (pg) {
-    auto kern = &k_graph<Real, W, NT, true>;
-    if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds));
     const int blocks = std::max(1, std::min(tiles, persistent_blocks));
-    kern<<<blocks, NT, lds, s>>>(N, basics, type_s, gt, e_s, grads, *pg);
+    launch_variant<Real, W, NT, true, false>(N, basics, type_s, gt, e_s, grads, *pg, s, lds,
+                                             max_lds, blocks);
+  } else if (mode == 0) {
+    launch_variant<Real, W, NT, false, false>(N, basics, type_s, gt, e_s, grads, ParamGradOut{}, s,
+                                              lds, max_lds, tiles);
   } else {
-    auto kern = &k_graph<Real, W, NT, false>;
-    if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds));
-    ParamGradOut none{};
-    kern<<<tiles, NT, lds, s>>>(N, basics, type_s, gt, e_s, grads, none);
+    launch_variant<Real, W, NT, false, true>(N, basics, type_s, gt, e_s, grads, ParamGradOut{}, s,
+                                             lds, max_lds, tiles);
   }
   return true;
 }
@@ -218,30 +318,32 @@ bool launch_tile(int N, const Real* basics, const int* type_s, const GraphTable<
 template <typename Real>
 size_t graph_lds_bytes(int tile, int slots)
 {
-  return (2 * static_cast<size_t>(slots) + 4 + 1) * tile * sizeof(Real);
+  return (2 * static_cast<size_t>(slots) + 2 + 4 + 1) * tile * sizeof(Real);
 }
 
 template <typename Real>
-bool launch_graph(int tile, int N, const Real* basics, const int* type_s,
+bool launch_graph(int tile, int mode, int N, const Real* basics, const int* type_s,
                   const GraphTable<Real>& gt, double* e_s, Real* grads, const ParamGradOut* pg,
                   hipStream_t s, size_t max_lds, int persistent_blocks)
 {
   if (N <= 0) return true;
   switch (tile) {
-    case 1: return launch_tile<Real, 1, 256>(N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
-    case 2: return launch_tile<Real, 2, 256>(N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
-    case 4: return launch_tile<Real, 4, 256>(N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
-    case 8: return launch_tile<Real, 8, 256>(N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
-    case 16: return launch_tile<Real, 16, 256>(N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
+    case 1: return launch_tile<Real, 1, 256>(mode, N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
+    case 2: return launch_tile<Real, 2, 256>(mode, N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
+    case 4: return launch_tile<Real, 4, 256>(mode, N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
+    case 8: return launch_tile<Real, 8, 256>(mode, N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
+    case 16: return launch_tile<Real, 16, 256>(mode, N, basics, type_s, gt, e_s, grads, pg, s, max_lds, persistent_blocks);
     default: return false;
   }
 }
 
 template size_t graph_lds_bytes<float>(int, int);
 template size_t graph_lds_bytes<double>(int, int);
-template bool launch_graph<float>(int, int, const float*, const int*, const GraphTable<float>&,
-                                  double*, float*, const ParamGradOut*, hipStream_t, size_t, int);
-template bool launch_graph<double>(int, int, const double*, const int*, const GraphTable<double>&,
-                                   double*, double*, const ParamGradOut*, hipStream_t, size_t, int);
+template bool launch_graph<float>(int, int, int, const float*, const int*,
+                                  const GraphTable<float>&, double*, float*, const ParamGradOut*,
+                                  hipStream_t, size_t, int);
+template bool launch_graph<double>(int, int, int, const double*, const int*,
+                                   const GraphTable<double>&, double*, double*,
+                                   const ParamGradOut*, hipStream_t, size_t, int);
 
 }  // namespace sus2sh

@@ -24,6 +24,14 @@ struct GraphTable {
   const int* e_slot;
   const Real* e_w;
   const Real* seed;       // [slots]
+  // per-thread instruction streams (see GraphProgram)
+  const Term<Real>* fwd_ins;
+  const int* fwd_sbegin;  // [layers + 1]
+  const int* fwd_slen;    // [layers]
+  const Term<Real>* bwd_ins;
+  const int* bwd_sbegin;
+  const int* bwd_slen;
+  int ones_slot;
   const double* shift;    // [S]
   const double* sigma;    // [S]
   int n_e;
@@ -102,10 +110,11 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
                    double* pg_rad, hipStream_t s);
 
 // --- graph kernel (graph_kernel.hip) ---
-// pg != nullptr selects the persistent parameter-gradient variant with at most
-// persistent_blocks workgroups.
+// mode 0: rows / back chunks read per thread; mode 1: interleaved per-thread instruction
+// streams (gt.fwd_ins / gt.bwd_ins).  pg != nullptr selects the persistent
+// parameter-gradient variant (row mode) with at most persistent_blocks workgroups.
 template <typename Real>
-bool launch_graph(int tile, int N, const Real* basics, const int* type_s,
+bool launch_graph(int tile, int mode, int N, const Real* basics, const int* type_s,
                   const GraphTable<Real>& gt, double* e_s, Real* grads, const ParamGradOut* pg,
                   hipStream_t s, size_t max_lds, int persistent_blocks);
 

@@ -7,7 +7,56 @@
 
 namespace sus2sh {
 
-GraphProgram compile_graph(const Model& m, int chunk_terms)
+namespace {
+
+constexpr uint32_t kEnd = 0xFFFFu;
+constexpr uint32_t kNop = 0xFFFEu;
+
+struct Ins {
+  uint32_t a, b;
+  double c;
+};
+
+// Longest-processing-time assignment of work items (each a list of instructions) to
+// `threads` streams, then interleaving with no-op padding.
+void emit_streams(const std::vector<std::vector<Ins>>& items, int threads, bool balanced,
+                  std::vector<uint32_t>& ab, std::vector<double>& cs, int* len)
+{
+  std::vector<std::vector<Ins>> per(threads);
+  if (balanced) {
+    std::vector<int> order(items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return items[x].size() > items[y].size(); });
+    std::vector<size_t> load(threads, 0);
+    for (int it : order) {
+      int best = 0;
+      for (int t = 1; t < threads; ++t)
+        if (load[t] < load[best]) best = t;
+      per[best].insert(per[best].end(), items[it].begin(), items[it].end());
+      load[best] += items[it].size();
+    }
+  } else {
+    for (size_t it = 0; it < items.size(); ++it) {
+      std::vector<Ins>& dst = per[it % threads];
+      dst.insert(dst.end(), items[it].begin(), items[it].end());
+    }
+  }
+  size_t L = 0;
+  for (int t = 0; t < threads; ++t) L = std::max(L, per[t].size());
+  L = (L + 3) / 4 * 4;  // the kernel consumes instructions in groups of four
+  for (size_t k = 0; k < L; ++k)
+    for (int t = 0; t < threads; ++t) {
+      const Ins in = k < per[t].size() ? per[t][k] : Ins{kNop, 0, 0.0};
+      ab.push_back((in.a & 0xFFFFu) | (in.b << 16));
+      cs.push_back(in.c);
+    }
+  *len = static_cast<int>(L);
+}
+
+}  // namespace
+
+GraphProgram compile_graph(const Model& m, int chunk_terms, int threads)
 {
   const int A = m.moments_count;
   const int B = m.basic_count;
@@ -38,7 +87,8 @@ GraphProgram compile_graph(const Model& m, int chunk_terms)
   for (int b = 0; b < B; ++b) g.node_slot[b] = slots++;
   for (int n = B; n < A; ++n)
     if (live[n] && consumed[n]) g.node_slot[n] = slots++;
-  if (slots >= kMaxSlots) throw InputError("SUS2_SH graph too large for the on-chip program.");
+  if (slots + 1 >= static_cast<int>(kNoTarget) - 1)
+    throw InputError("SUS2_SH graph too large for the on-chip program.");
   g.slot_count = slots;
 
   // terminal weights w_t (summed over scalars mapped to the same node)
@@ -157,6 +207,64 @@ GraphProgram compile_graph(const Model& m, int chunk_terms)
     }
   }
   g.bwd_layer_begin[m.layer_count] = static_cast<int>(g.bwd_chunk_src.size());
+
+  // --- per-thread instruction streams ---
+  g.threads = threads;
+  g.ones_slot = g.slot_count;
+  // forward items in layer order; within a layer by ascending target (round-robin keeps the
+  // rows of one tensor on neighbouring lanes)
+  std::vector<std::vector<std::vector<Ins>>> fwd_items(m.layer_count), bwd_items(m.layer_count);
+  for (int layer = 0; layer < m.layer_count; ++layer) {
+    std::vector<int> rows;
+    for (int r = g.fwd_layer_begin[layer]; r < g.fwd_layer_begin[layer + 1]; ++r) rows.push_back(r);
+    std::sort(rows.begin(), rows.end(),
+              [&](int a, int b) { return g.fwd_row_node[a] < g.fwd_row_node[b]; });
+    for (int r : rows) {
+      std::vector<Ins> it;
+      for (int t = 0; t < g.fwd_row_count[r]; ++t) {
+        const uint32_t idx = g.fwd_term_idx[g.fwd_row_begin[r] + t];
+        it.push_back({idx & 0xFFFFu, idx >> 16, g.fwd_term_coeff[g.fwd_row_begin[r] + t]});
+      }
+      const int tgt = g.fwd_row_target[r];
+      it.push_back({kEnd, tgt >= 0 ? static_cast<uint32_t>(tgt) : kEnd,
+                    tgt >= 0 ? 0.0 : g.fwd_row_weight[r]});
+      fwd_items[layer].push_back(std::move(it));
+    }
+    std::vector<int> chunks;
+    for (int c = g.bwd_layer_begin[layer]; c < g.bwd_layer_begin[layer + 1]; ++c) chunks.push_back(c);
+    std::stable_sort(chunks.begin(), chunks.end(), [&](int a, int b) {
+      return g.bwd_chunk_src[a] != g.bwd_chunk_src[b] ? g.bwd_chunk_src[a] < g.bwd_chunk_src[b]
+                                                      : g.bwd_chunk_begin[a] < g.bwd_chunk_begin[b];
+    });
+    for (int c : chunks) {
+      std::vector<Ins> it;
+      for (int t = 0; t < g.bwd_chunk_count[c]; ++t) {
+        const uint32_t idx = g.bwd_term_idx[g.bwd_chunk_begin[c] + t];
+        const uint32_t other = idx & 0xFFFFu, tgt = idx >> 16;
+        it.push_back({other, tgt == kNoTarget ? static_cast<uint32_t>(g.ones_slot) : tgt,
+                      g.bwd_term_coeff[g.bwd_chunk_begin[c] + t]});
+      }
+      it.push_back({kEnd, static_cast<uint32_t>(g.bwd_chunk_src[c]),
+                    g.bwd_chunk_atomic[c] ? 1.0 : 0.0});
+      bwd_items[layer].push_back(std::move(it));
+    }
+  }
+  for (int v = 0; v < 2; ++v) {
+    GraphProgram::Streams& st = g.streams[v];
+    const bool balanced = v == 0;
+    st.fwd_begin.assign(m.layer_count + 1, 0);
+    st.fwd_len.assign(m.layer_count, 0);
+    st.bwd_begin.assign(m.layer_count + 1, 0);
+    st.bwd_len.assign(m.layer_count, 0);
+    for (int layer = 0; layer < m.layer_count; ++layer) {
+      st.fwd_begin[layer] = static_cast<int>(st.fwd_ab.size());
+      emit_streams(fwd_items[layer], threads, balanced, st.fwd_ab, st.fwd_c, &st.fwd_len[layer]);
+      st.bwd_begin[layer] = static_cast<int>(st.bwd_ab.size());
+      emit_streams(bwd_items[layer], threads, balanced, st.bwd_ab, st.bwd_c, &st.bwd_len[layer]);
+    }
+    st.fwd_begin[m.layer_count] = static_cast<int>(st.fwd_ab.size());
+    st.bwd_begin[m.layer_count] = static_cast<int>(st.bwd_ab.size());
+  }
   return g;
 }
 

@@ -62,6 +62,29 @@ struct GraphProgram {
   std::vector<int> stored_scalar_index;  // scalars whose node has a slot
   std::vector<int> stored_scalar_slot;
 
+  // Per-thread instruction streams (threads_per_block lanes), interleaved so that
+  // instruction k of thread t sits at stream_begin[layer] + k * threads + t.
+  //   forward:  a < 0xFFFE: acc += c V[a] V[b];  a == 0xFFFF: end of row -> V[b] = acc, or
+  //             (b == 0xFFFF) e += c acc;  a == 0xFFFE: no-op
+  //   backward: a < 0xFFFE: acc += c G[b] V[a]  (b = ones slot for a terminal target);
+  //             a == 0xFFFF: flush acc into G[b] (c != 0: atomically);  a == 0xFFFE: no-op
+  // Two assignments of rows/chunks to threads are compiled: [0] longest-first onto the
+  // least-loaded thread (balanced), [1] round-robin in target order (neighbouring lanes
+  // work on rows of the same tensor, so their LDS operands coincide).
+  struct Streams {
+    std::vector<int> fwd_begin;  // [layer_count + 1]
+    std::vector<int> fwd_len;    // [layer_count] instructions per thread
+    std::vector<uint32_t> fwd_ab;
+    std::vector<double> fwd_c;
+    std::vector<int> bwd_begin;
+    std::vector<int> bwd_len;
+    std::vector<uint32_t> bwd_ab;
+    std::vector<double> bwd_c;
+  };
+  int threads = 0;
+  Streams streams[2];
+  int ones_slot = 0;  // adjoint slot that always holds 1 (== slot_count)
+
   int forward_terms() const { return static_cast<int>(fwd_term_idx.size()); }
   int backward_terms() const { return static_cast<int>(bwd_term_idx.size()); }
 };
@@ -69,6 +92,6 @@ struct GraphProgram {
 constexpr uint32_t kNoTarget = 0xFFFFu;
 constexpr int kMaxSlots = 0xFFFF;
 
-GraphProgram compile_graph(const Model& model, int chunk_terms = 16);
+GraphProgram compile_graph(const Model& model, int chunk_terms = 8, int threads = 256);
 
 }  // namespace sus2sh

@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_cutoff",
     "sus2sh_get_model_info",
     "sus2sh_get_stage_times",
+    "sus2sh_get_tuning",
     "sus2sh_mean_neighbors",
     "sus2sh_clebsch_gordan",
     "sus2sh_real_cg",
@@ -73,6 +74,7 @@ class ModelInfo(ctypes.Structure):
         ("param_count", ctypes.c_int),
         ("stored_slots", ctypes.c_int),
         ("graph_tile", ctypes.c_int),
+        ("graph_variant", ctypes.c_int),
         ("neighbor_capacity", ctypes.c_int),
         ("max_dist", ctypes.c_double),
     ]
@@ -138,6 +140,8 @@ def lib():
     L.sus2sh_get_model_info.restype = c_int
     L.sus2sh_get_stage_times.argtypes = [c_void_p, P(ctypes.c_float), c_int]
     L.sus2sh_get_stage_times.restype = c_int
+    L.sus2sh_get_tuning.argtypes = [c_void_p, P(ctypes.c_float), c_int]
+    L.sus2sh_get_tuning.restype = c_int
     L.sus2sh_mean_neighbors.argtypes = [c_void_p]
     L.sus2sh_mean_neighbors.restype = c_double
     L.sus2sh_clebsch_gordan.argtypes = [c_int] * 6

@@ -88,6 +88,13 @@ class SUS2_SH:
         n = lib().sus2sh_get_stage_times(self._h, buf, 8)
         return list(buf)[:n]
 
+    def tuning(self):
+        """Autotuner graph-kernel times {(variant, tile): ms} (variant 0 rows, 1 balanced
+        streams, 2 round-robin streams)."""
+        buf = (ctypes.c_float * 15)()
+        n = lib().sus2sh_get_tuning(self._h, buf, 15)
+        return {(k // 5, 1 << (k % 5)): buf[k] for k in range(n) if buf[k] >= 0}
+
     def mean_neighbors(self):
         return lib().sus2sh_mean_neighbors(self._h)
 

@@ -52,6 +52,7 @@ typedef struct sus2sh_model_info {
   int param_count;        /* length of the dE/dtheta vector */
   int stored_slots;       /* per-atom on-chip graph state (values and adjoints each) */
   int graph_tile;         /* atoms per workgroup in the graph kernel */
+  int graph_variant;      /* graph kernel variant: 0 rows, 1 balanced streams, 2 round-robin */
   int neighbor_capacity;  /* current per-atom neighbour capacity */
   double max_dist;
 } sus2sh_model_info;
@@ -67,8 +68,8 @@ int sus2sh_probe(const char* path);
  * src/force/force.cu:114).  `opts` are the run.in tokens after the model file: the
  * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
  * Honoured options: sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
- * sus2_sh_graph_tile=<n>, sus2_sh_profile=<0|1>; other sus2_* tokens are accepted and
- * ignored.  Uses the current HIP device. */
+ * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2>, sus2_sh_autotune=<0|1>,
+ * sus2_sh_profile=<0|1>; other sus2_* tokens are accepted and ignored.  Uses the current HIP device. */
 int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
                   sus2sh_handle** out);
 
@@ -142,6 +143,10 @@ int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out);
  * ms[0..n) = neighbour, basic, graph, force, scatter.  Returns the number written.
  * Replaces the reference's host-clock stage profile (src/force/sus2_sh.cu:9631-9645). */
 int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
+
+/* Autotuner measurements of the graph kernel (ms per launch, -1 when not tried):
+ * ms[v * 5 + log2(tile)] for variant v = 0..2 and tile 1..16.  Returns the count written. */
+int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
 /* Mean number of neighbours per atom found by the last compute. */
 double sus2sh_mean_neighbors(const sus2sh_handle* handle);

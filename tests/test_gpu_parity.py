@@ -196,3 +196,23 @@ def test_parameter_gradient_matches_oracle(make_model, fp32):
     assert err <= (1e-4 if fp32 else 1e-9) * max(1.0, scale)
     assert np.abs(e.cpu().numpy() - ref[0]).max() < (1e-4 if fp32 else 1e-9)
     pot.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("tile", [1, 4, 16])
+def test_graph_kernel_variants(make_model, variant, tile):
+    """Every graph-kernel variant and tile the autotuner may pick gives the oracle's result."""
+    path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])
+    pos, types, cell = _structure(CASES["l3k3_fcc"]["struct"], 2)
+    ref = Oracle(path).compute(cell, types, pos)
+    for fp32 in (0, 1):
+        opts = ["Cu", "Zr", f"sus2_float={fp32}", f"sus2_sh_graph_variant={variant}",
+                f"sus2_sh_graph_tile={tile}"]
+        if tile == 16 and not fp32:
+            # 2 x 756 slots x 16 atoms x 8 B exceeds the LDS of a workgroup: rejected cleanly
+            with pytest.raises(sus2sh.Sus2shError) as err:
+                _run(path, cell, types, pos, options=opts)
+            assert "too large" in err.value.message
+            continue
+        got = _run(path, cell, types, pos, options=opts)
+        _compare(ref, got, bool(fp32))

@@ -5,6 +5,7 @@
 
 #include "../../include/sus2sh.h"
 #include "engine.hpp"
+#include "lane_program.hpp"
 #include "model.hpp"
 
 struct sus2sh_handle {
@@ -199,6 +200,10 @@ int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out)
   out->graph_variant = handle->engine->graph_variant();
   out->neighbor_capacity = handle->engine->neighbor_capacity();
   out->max_dist = m.max_dist;
+  const sus2sh::LaneProgram& lp = handle->engine->lane_program();
+  out->lane_program = lp.ok ? 1 : 0;
+  out->lane_slots = lp.ok ? lp.slots : 0;
+  out->lane_instructions = lp.ok ? static_cast<int>(lp.ins.size()) : 0;
   return SUS2SH_OK;
 }
 
@@ -299,6 +304,29 @@ int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_
 }
 
 void sus2sh_graph_free(sus2sh_graph* graph) { delete graph; }
+
+int sus2sh_lane_check(const char* mtp_path, int max_slots, int n, const double* basics,
+                      double* e_prime, double* grad, int* info)
+{
+  if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad))) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&g_create_error, [&] {
+    const sus2sh::Model m = sus2sh::load_model(mtp_path);
+    const sus2sh::LaneProgram p = sus2sh::compile_lane_program(m, max_slots);
+    if (info) {
+      info[0] = p.ok ? 1 : 0;
+      info[1] = p.slots;
+      info[2] = static_cast<int>(p.ins.size());
+      info[3] = p.components;
+      info[4] = p.batches;
+      info[5] = p.largest_component;
+    }
+    if (!p.ok) return;
+    const int B = m.basic_count;
+    for (int i = 0; i < n; ++i)
+      e_prime[i] = sus2sh::run_lane_program_host(p, basics + static_cast<size_t>(i) * B,
+                                                 grad + static_cast<size_t>(i) * B);
+  });
+}
 
 int sus2sh_model_check(const char* mtp_path, int* dims)
 {

@@ -49,7 +49,7 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.autotune = parse_bool(val, key);
     } else if (key == "sus2_sh_graph_variant") {
       const int v = std::atoi(val.c_str());
-      if (v < 0 || v > 2) throw InputError("sus2_sh_graph_variant must be 0, 1 or 2: " + val);
+      if (v < 0 || v > 3) throw InputError("sus2_sh_graph_variant must be 0, 1, 2 or 3: " + val);
       o.graph_variant = v;
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
@@ -137,6 +137,13 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
       throw UnsupportedError("SUS2_SH: graph state of one atom exceeds the LDS of a workgroup.");
   }
 
+  compile_lane();
+  if (opt_.graph_variant == 3) {
+    if (!lane_.ok)
+      throw InputError("sus2_sh_graph_variant=3: no lane program for this model (" + lane_.reason +
+                       ").");
+    tuned_ = true;  // a fixed variant disables the autotuner
+  }
   if (opt_.use_float)
     upload_tables<float>();
   else
@@ -157,6 +164,25 @@ int Engine::param_count() const
   const int S = model_.species_count;
   return 2 * S + model_.scalar_count + model_.radial_funcs_count * model_.rb_size + S +
          2 * S * S * model_.radial_funcs_count;
+}
+
+// Lane program with the smallest per-lane slot budget that holds every graph component:
+// 160 slots are 40 KiB of LDS per 64-atom workgroup in FP32 (four workgroups per CU).
+void Engine::compile_lane()
+{
+  const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  lane_ = LaneProgram{};
+  lane_.reason = "no slot budget fits the LDS of a workgroup";
+  for (int max_slots : {160, 256, 384, 512, 1024}) {
+    if (static_cast<size_t>(max_slots) * 64 * rsz > max_lds_) break;
+    LaneProgram p = compile_lane_program(model_, max_slots);
+    if (p.ok) {
+      lane_ = std::move(p);
+      return;
+    }
+    lane_.reason = p.reason;
+    if (p.reason.find("slot budget") == std::string::npos) return;  // not a budget limit
+  }
 }
 
 template <typename Real>
@@ -253,6 +279,16 @@ void Engine::upload_tables()
   row_slist_.upload(g.row_scalar_list.data(), g.row_scalar_list.size() * sizeof(int));
   stored_index_.upload(g.stored_scalar_index.data(), g.stored_scalar_index.size() * sizeof(int));
   stored_slot_.upload(g.stored_scalar_slot.data(), g.stored_scalar_slot.size() * sizeof(int));
+  if (lane_.ok) {
+    const size_t nseg = lane_.seg_kind.size();
+    std::vector<int4> segs(nseg);
+    for (size_t k = 0; k < nseg; ++k)
+      segs[k] = make_int4(lane_.seg_kind[k], lane_.seg_begin[k], lane_.seg_groups[k], 0);
+    lane_segs_.upload(segs.data(), segs.size() * sizeof(int4));
+    lane_ins_.upload(lane_.ins.data(), lane_.ins.size() * sizeof(uint32_t));
+    std::vector<Real> lc(lane_.coeff.begin(), lane_.coeff.end());
+    lane_coeff_.upload(lc.data(), lc.size() * sizeof(Real));
+  }
 }
 
 void Engine::ensure_atoms(int N, size_t edge_bytes)
@@ -351,6 +387,37 @@ void Engine::compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom
     run<float>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
   else
     run<double>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
+}
+
+template <typename Real>
+bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable<Real>& gt,
+                                  hipStream_t s)
+{
+  if (variant == 3) {
+    if (!lane_.ok) return false;
+    LaneTable<Real> lt{};
+    lt.segs = lane_segs_.as<int4>();
+    lt.ins = lane_ins_.as<uint32_t>();
+    lt.coeff = lane_coeff_.as<Real>();
+    lt.shift = shift_.as<double>();
+    lt.sigma = sigma_.as<double>();
+    lt.nseg = static_cast<int>(lane_.seg_kind.size());
+    lt.basics = model_.basic_count;
+    lt.ones_slot = lane_.ones_slot;
+    lt.slots = lane_.slots;
+    return launch_lane_graph<Real>(N, basics_.as<Real>(), type_s_.as<int>(), lt, e_s_.as<double>(),
+                                   grads_.as<Real>(), s, max_lds_);
+  }
+  GraphTable<Real> t = gt;
+  const int k = variant == 2 ? 1 : 0;
+  t.fwd_ins = fwd_ins_[k].as<Term<Real>>();
+  t.fwd_sbegin = fwd_sbegin_[k].as<int>();
+  t.fwd_slen = fwd_slen_[k].as<int>();
+  t.bwd_ins = bwd_ins_[k].as<Term<Real>>();
+  t.bwd_sbegin = bwd_sbegin_[k].as<int>();
+  t.bwd_slen = bwd_slen_[k].as<int>();
+  return launch_graph<Real>(tile, variant == 0 ? 0 : 1, N, basics_.as<Real>(), type_s_.as<int>(),
+                            t, e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
 }
 
 template <typename Real>
@@ -469,15 +536,6 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   gt.seed = seed_.as<Real>();
   gt.shift = shift_.as<double>();
   gt.sigma = sigma_.as<double>();
-  auto use_streams = [&](GraphTable<Real>& t, int v) {
-    t.fwd_ins = fwd_ins_[v].as<Term<Real>>();
-    t.fwd_sbegin = fwd_sbegin_[v].as<int>();
-    t.fwd_slen = fwd_slen_[v].as<int>();
-    t.bwd_ins = bwd_ins_[v].as<Term<Real>>();
-    t.bwd_sbegin = bwd_sbegin_[v].as<int>();
-    t.bwd_slen = bwd_slen_[v].as<int>();
-  };
-  use_streams(gt, variant_ > 0 ? variant_ - 1 : 0);
   gt.ones_slot = prog_.ones_slot;
   gt.n_e = static_cast<int>(prog_.energy_slot.size());
   gt.slots = prog_.slot_count;
@@ -503,33 +561,29 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     pgo.S = S;
   }
   if (opt_.autotune && !tuned_ && !param_grad && N >= 4096) {
-    // Autotuner: time every (variant, tile) whose LDS footprint fits and keep the fastest
-    // (HIP events on the engine's stream; the graph kernel is idempotent, so the trial
-    // launches only rewrite e_s and grads).
+    // Autotuner: time every feasible candidate -- (variant, tile) of the workgroup kernel
+    // and the lane program -- and keep the fastest (HIP events on the engine's stream; the
+    // graph kernels are idempotent, so the trial launches only rewrite e_s and grads).
     hipEvent_t t0, t1;
     hip_check(hipEventCreate(&t0), "hipEventCreate");
     hip_check(hipEventCreate(&t1), "hipEventCreate");
     float best = 1e30f;
     int best_tile = tile_, best_variant = variant_;
-    tune_ms_.assign(15, -1.0f);
-    for (int v = 0; v < 3; ++v) {
-      GraphTable<Real> tt = gt;
-      use_streams(tt, v > 0 ? v - 1 : 0);
-      for (int li = 0; li < 5; ++li) {
+    tune_ms_.assign(16, -1.0f);
+    for (int v = 0; v < 4; ++v) {
+      if (opt_.graph_variant >= 0 && v != opt_.graph_variant) continue;  // fixed variant
+      for (int li = 0; li < (v == 3 ? 1 : 5); ++li) {
         const int t = 1 << li;
-        if (graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
-        const int mode = v == 0 ? 0 : 1;
+        if (v < 3 && graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
+        if (v == 3 && !lane_.ok) continue;
         float ms = 0.0f;
-        launch_graph<Real>(t, mode, N, basics_.as<Real>(), type_s_.as<int>(), tt,
-                           e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
+        if (!launch_graph_variant<Real>(v, t, N, gt, s)) continue;
         hip_check(hipEventRecord(t0, s), "hipEventRecord");
-        for (int rep = 0; rep < 2; ++rep)
-          launch_graph<Real>(t, mode, N, basics_.as<Real>(), type_s_.as<int>(), tt,
-                             e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
+        for (int rep = 0; rep < 2; ++rep) launch_graph_variant<Real>(v, t, N, gt, s);
         hip_check(hipEventRecord(t1, s), "hipEventRecord");
         hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
         hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
-        tune_ms_[v * 5 + li] = ms / 2;
+        tune_ms_[v == 3 ? 15 : v * 5 + li] = ms / 2;
         if (ms < best) {
           best = ms;
           best_tile = t;
@@ -539,15 +593,19 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    tile_ = best_tile;
+    if (best_variant < 3) tile_ = best_tile;
     variant_ = best_variant;
-    use_streams(gt, variant_ > 0 ? variant_ - 1 : 0);
     tuned_ = true;
   }
-  if (!launch_graph<Real>(tile_, variant_ == 0 ? 0 : 1, N, basics_.as<Real>(), type_s_.as<int>(),
-                          gt, e_s_.as<double>(), grads_.as<Real>(), param_grad ? &pgo : nullptr, s,
-                          max_lds_, 4 * cu_count_))
-    throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
+  if (param_grad) {
+    // the parameter gradient needs per-scalar sums: persistent row-mode kernel
+    if (!launch_graph<Real>(tile_, 0, N, basics_.as<Real>(), type_s_.as<int>(), gt,
+                            e_s_.as<double>(), grads_.as<Real>(), &pgo, s, max_lds_,
+                            4 * cu_count_))
+      throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
+  } else if (!launch_graph_variant<Real>(variant_, tile_, N, gt, s)) {
+    throw UnsupportedError("SUS2_SH: graph kernel variant does not fit in LDS.");
+  }
   if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
   launch_fill_double(f_s_.as<double>(), 0.0, 3 * static_cast<size_t>(N), s);
@@ -599,6 +657,7 @@ void Engine::set_params(const double* theta, int n)
   take(m.radial_type_coeffs);
   take(m.scal_coeffs);
   prog_ = compile_graph(m);
+  compile_lane();
   if (opt_.use_float)
     upload_tables<float>();
   else

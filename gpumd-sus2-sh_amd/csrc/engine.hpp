@@ -16,6 +16,7 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "lane_program.hpp"
 #include "program.hpp"
 
 namespace sus2sh {
@@ -56,7 +57,8 @@ class DeviceBuffer {
 struct EngineOptions {
   bool use_float = true;
   int graph_tile = 0;  // 0: automatic
-  int graph_variant = -1;  // -1: automatic; 0 rows, 1 balanced streams, 2 round-robin
+  int graph_variant = -1;  // -1: automatic; 0 rows, 1 balanced streams, 2 round-robin,
+                          // 3 lane program
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
 };
@@ -86,6 +88,7 @@ class Engine {
 
   const Model& model() const { return model_; }
   const GraphProgram& program() const { return prog_; }
+  const LaneProgram& lane_program() const { return lane_; }
   const EngineOptions& options() const { return opt_; }
   int graph_tile() const { return tile_; }
   int graph_variant() const { return variant_; }
@@ -98,6 +101,10 @@ class Engine {
  private:
   template <typename Real>
   void upload_tables();
+  void compile_lane();
+  template <typename Real>
+  bool launch_graph_variant(int variant, int tile, int N, const GraphTable<Real>& gt,
+                            hipStream_t s);
   template <typename Real>
   void run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, int N, const int* type,
            const double* pos, double* pot, double* force, double* virial, hipStream_t stream,
@@ -110,9 +117,11 @@ class Engine {
   GraphProgram prog_;
   EngineOptions opt_;
   int tile_ = 1;
-  int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams
+  int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane
   bool tuned_ = false;
-  std::vector<float> tune_ms_;  // graph-kernel ms per candidate, [variant * 5 + log2(tile)]
+  // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2, [15] lane
+  std::vector<float> tune_ms_;
+  LaneProgram lane_;
   int cap_ = 0;
   size_t max_lds_ = 65536;
   int device_ = 0;
@@ -125,6 +134,7 @@ class Engine {
   DeviceBuffer fwd_ins_[2], fwd_sbegin_[2], fwd_slen_[2], bwd_ins_[2], bwd_sbegin_[2],
     bwd_slen_[2];
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
+  DeviceBuffer lane_segs_, lane_ins_, lane_coeff_;
   int cu_count_ = 256;
   bool radial_uniform_ = false;
 

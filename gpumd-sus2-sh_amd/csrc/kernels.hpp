@@ -1,6 +1,7 @@
 // Kernel launch interface of the SUS2-SH engine.  Implementations live in
 // neighbor.hip (cell list), edge_kernels.hip (basic moments, forces/virial),
-// graph_kernel.hip (CG product graph forward / energy / reverse).
+// graph_kernel.hip (CG product graph forward / energy / reverse), lane_kernel.hip (the
+// same graph as a wave-uniform program, one atom per lane).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -120,6 +121,26 @@ bool launch_graph(int tile, int mode, int N, const Real* basics, const int* type
 
 template <typename Real>
 size_t graph_lds_bytes(int tile, int slots);
+
+// Lane program tables (see lane_program.hpp): one atom per lane, wave-uniform program.
+template <typename Real>
+struct LaneTable {
+  const int4* segs;        // (kind, first instruction, groups, 0)
+  const uint32_t* ins;     // x | y << 10 | t << 20 | end << 30
+  const Real* coeff;       // one per instruction
+  const double* shift;     // [S]
+  const double* sigma;     // [S]
+  int nseg;
+  int basics;
+  int ones_slot;
+  int slots;               // per-lane slots: LDS of a 64-lane workgroup = slots * 64 * sizeof(Real)
+};
+
+// Lane graph kernel (lane_kernel.hip): workgroups of 64 atoms; returns false when the
+// per-workgroup LDS would exceed max_lds.
+template <typename Real>
+bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
+                       double* e_s, Real* grads, hipStream_t s, size_t max_lds);
 
 // --- velocity-Verlet NVE driver (md.hip) ---
 void launch_half_kick(int N, double half_dt, const int* type, const double* mass, const double* f,

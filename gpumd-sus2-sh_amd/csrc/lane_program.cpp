@@ -1,0 +1,400 @@
+#include "lane_program.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace sus2sh {
+
+namespace {
+
+struct Term {
+  double c;
+  int x, y;
+};
+
+struct Row {
+  std::vector<Term> terms;
+  int target;
+  uint32_t end;
+};
+
+int find_root(std::vector<int>& parent, int x)
+{
+  while (parent[x] != x) {
+    parent[x] = parent[parent[x]];
+    x = parent[x];
+  }
+  return x;
+}
+
+// Longest-first assignment of independent rows to nslot accumulators, interleaved as groups
+// of nslot instructions (padding: c = 0 on the constant slot).
+void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
+{
+  const int ns = p.nslot;
+  rows.erase(std::remove_if(rows.begin(), rows.end(), [](const Row& r) { return r.terms.empty(); }),
+             rows.end());
+  if (rows.empty()) return;
+  std::vector<int> order(rows.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return rows[a].terms.size() > rows[b].terms.size(); });
+  std::vector<std::vector<int>> per(ns);
+  std::vector<size_t> load(ns, 0);
+  for (int r : order) {
+    const int j = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    per[j].push_back(r);
+    load[j] += rows[r].terms.size();
+  }
+  const size_t groups = *std::max_element(load.begin(), load.end());
+  std::vector<std::vector<uint32_t>> words(ns);
+  std::vector<std::vector<double>> cs(ns);
+  for (int j = 0; j < ns; ++j) {
+    for (int r : per[j]) {
+      const Row& row = rows[r];
+      for (size_t k = 0; k < row.terms.size(); ++k) {
+        const Term& t = row.terms[k];
+        const bool last = k + 1 == row.terms.size();
+        const uint32_t end = last ? row.end : kLaneEndNone;
+        const uint32_t tgt = last && row.end != kLaneEndEnergy ? static_cast<uint32_t>(row.target) : 0u;
+        words[j].push_back(static_cast<uint32_t>(t.x) | (static_cast<uint32_t>(t.y) << 10) |
+                           (tgt << 20) | (end << 30));
+        cs[j].push_back(t.c);
+        ++p.useful_terms;
+      }
+    }
+    const uint32_t one = static_cast<uint32_t>(p.ones_slot);
+    while (words[j].size() < groups) {
+      words[j].push_back(one | (one << 10));
+      cs[j].push_back(0.0);
+    }
+  }
+  p.seg_kind.push_back(kind);
+  p.seg_begin.push_back(static_cast<int>(p.ins.size()));
+  p.seg_groups.push_back(static_cast<int>(groups));
+  for (size_t g = 0; g < groups; ++g)
+    for (int j = 0; j < ns; ++j) {
+      p.ins.push_back(words[j][g]);
+      p.coeff.push_back(cs[j][g]);
+    }
+}
+
+uint64_t lcg(uint64_t& s)
+{
+  s = s * 6364136223846793005ULL + 1442695040888963407ULL;
+  return s >> 11;
+}
+
+}  // namespace
+
+LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot)
+{
+  LaneProgram p;
+  p.nslot = nslot;
+  const int A = m.moments_count, B = m.basic_count;
+  p.basic_count = B;
+  if (m.layer_count > 2) {
+    p.reason = "product graph deeper than two layers";
+    return p;
+  }
+  max_slots = std::min(max_slots, kLaneMaxSlot + 1);
+  if (max_slots - 2 * B - 1 < 2) {
+    p.reason = "too many basic moments for the per-lane slot budget";
+    return p;
+  }
+  p.ones_slot = 2 * B;
+  p.v_base = 2 * B + 1;
+
+  // direct weights d_n and the structural "mapped" flag
+  std::vector<double> d(A, 0.0);
+  std::vector<char> mapped(A, 0);
+  for (int s = 0; s < m.scalar_count; ++s) {
+    d[m.moment_mapping[s]] += m.moment_coeffs[s];
+    mapped[m.moment_mapping[s]] = 1;
+  }
+  const int T = static_cast<int>(m.products.size());
+  std::vector<std::vector<int>> layer1_terms(A);  // products of each layer-1 target
+  std::vector<int> l2;                             // live layer-2 products
+  for (int q = 0; q < T; ++q) {
+    const Product& pr = m.products[q];
+    const int lt = m.node_layer[pr.target];
+    if (lt == 1) {
+      layer1_terms[pr.target].push_back(q);
+    } else if (mapped[pr.target]) {
+      l2.push_back(q);
+    }
+  }
+  // component members: layer <= 1 nodes that are mapped or feed a live layer-2 product
+  std::vector<char> member(A, 0);
+  for (int n = 0; n < A; ++n)
+    if (mapped[n] && m.node_layer[n] <= 1) member[n] = 1;
+  for (int q : l2) {
+    member[m.products[q].left] = 1;
+    member[m.products[q].right] = 1;
+  }
+  std::vector<int> parent(A);
+  std::iota(parent.begin(), parent.end(), 0);
+  for (int q : l2) {
+    const int a = find_root(parent, m.products[q].left), b = find_root(parent, m.products[q].right);
+    if (a != b) parent[a] = b;
+  }
+  std::vector<std::vector<int>> comps;
+  {
+    std::vector<int> comp_of(A, -1);
+    for (int n = 0; n < A; ++n) {
+      if (!member[n]) continue;
+      const int r = find_root(parent, n);
+      if (comp_of[r] < 0) {
+        comp_of[r] = static_cast<int>(comps.size());
+        comps.emplace_back();
+      }
+      comps[comp_of[r]].push_back(n);
+    }
+  }
+  p.components = static_cast<int>(comps.size());
+  // layer-2 products by member node
+  std::vector<std::vector<int>> l2_of(A);
+  for (int q : l2) {
+    l2_of[m.products[q].left].push_back(q);
+    if (m.products[q].right != m.products[q].left) l2_of[m.products[q].right].push_back(q);
+  }
+
+  // region sizes: the free slots are split between values and adjoints in proportion to
+  // the needs of the largest component
+  auto v_need = [&](const std::vector<int>& c) {
+    int k = 0;
+    for (int n : c) k += n >= B ? 1 : 0;
+    return k;
+  };
+  int nv_max = 0, ng_max = 0;
+  for (const std::vector<int>& c : comps) {
+    nv_max = std::max(nv_max, v_need(c));
+    ng_max = std::max(ng_max, static_cast<int>(c.size()));
+  }
+  p.largest_component = ng_max;
+  const int free_slots = max_slots - p.v_base;
+  if (nv_max + ng_max > free_slots) {
+    p.reason = "a graph component exceeds the per-lane slot budget";
+    return p;
+  }
+  p.cap_v = ng_max > 0 ? std::max(nv_max, free_slots * nv_max / (nv_max + ng_max)) : 0;
+  p.cap_g = free_slots - p.cap_v;
+  p.g_base = p.v_base + p.cap_v;
+  p.slots = p.g_base + p.cap_g;
+
+  // batches: first fit, largest components first
+  std::vector<int> order(comps.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return comps[a].size() > comps[b].size(); });
+  std::vector<std::vector<int>> batches;
+  std::vector<int> bv, bg;
+  for (int c : order) {
+    const int nv = v_need(comps[c]), ng = static_cast<int>(comps[c].size());
+    size_t b = 0;
+    while (b < batches.size() && (bv[b] + nv > p.cap_v || bg[b] + ng > p.cap_g)) ++b;
+    if (b == batches.size()) {
+      batches.emplace_back();
+      bv.push_back(0);
+      bg.push_back(0);
+    }
+    batches[b].push_back(c);
+    bv[b] += nv;
+    bg[b] += ng;
+  }
+  p.batches = static_cast<int>(batches.size());
+
+  std::vector<int> vslot(A, -1), gslot(A, -1);
+  for (const std::vector<int>& batch : batches) {
+    std::vector<int> nodes;
+    for (int c : batch) nodes.insert(nodes.end(), comps[c].begin(), comps[c].end());
+    int nv = 0, ng = 0;
+    for (int n : nodes) {
+      vslot[n] = n < B ? n : p.v_base + nv++;
+      gslot[n] = p.g_base + ng++;
+    }
+    // FWD: layer-1 values
+    std::vector<Row> fwd;
+    for (int n : nodes) {
+      if (n < B) continue;
+      Row r;
+      r.target = vslot[n];
+      r.end = kLaneEndStore;
+      for (int q : layer1_terms[n])
+        r.terms.push_back({m.products[q].coeff, m.products[q].left, m.products[q].right});
+      if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, p.ones_slot});
+      fwd.push_back(std::move(r));
+    }
+    emit_segment(p, 0, fwd);
+    // ADJ: g_n = d_n + sum k_p v_other
+    std::vector<Row> adj;
+    for (int n : nodes) {
+      Row r;
+      r.target = gslot[n];
+      r.end = kLaneEndStore;
+      if (mapped[n]) r.terms.push_back({d[n], p.ones_slot, 0});
+      for (int q : l2_of[n]) {
+        const Product& pr = m.products[q];
+        const double k = d[pr.target] * pr.coeff;
+        if (pr.left == pr.right) {
+          r.terms.push_back({2.0 * k, vslot[n], 0});
+        } else {
+          const int other = pr.left == n ? pr.right : pr.left;
+          r.terms.push_back({k, vslot[other], 0});
+        }
+      }
+      if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, 0});
+      adj.push_back(std::move(r));
+    }
+    emit_segment(p, 1, adj);
+    // ENG + BWD
+    std::vector<Row> eb;
+    {
+      std::vector<Term> et;
+      for (int n : nodes) {
+        et.push_back({0.5, vslot[n], gslot[n]});
+        if (mapped[n]) et.push_back({0.5 * d[n], vslot[n], p.ones_slot});
+      }
+      const int parts = std::max(1, std::min<int>(nslot, static_cast<int>(et.size() + 3) / 4));
+      for (int k = 0; k < parts; ++k) {
+        Row r;
+        r.target = 0;
+        r.end = kLaneEndEnergy;
+        for (size_t i = k; i < et.size(); i += parts) r.terms.push_back(et[i]);
+        eb.push_back(std::move(r));
+      }
+    }
+    std::vector<std::vector<Term>> gb(B);
+    for (int n : nodes) {
+      if (n < B) {
+        gb[n].push_back({1.0, gslot[n], p.ones_slot});
+        continue;
+      }
+      for (int q : layer1_terms[n]) {
+        const Product& pr = m.products[q];
+        if (pr.left == pr.right) {
+          gb[pr.left].push_back({2.0 * pr.coeff, gslot[n], pr.left});
+        } else {
+          gb[pr.left].push_back({pr.coeff, gslot[n], pr.right});
+          gb[pr.right].push_back({pr.coeff, gslot[n], pr.left});
+        }
+      }
+    }
+    for (int q = 0; q < B; ++q) {
+      if (gb[q].empty()) continue;
+      Row r;
+      r.target = B + q;
+      r.end = kLaneEndAdd;
+      r.terms = std::move(gb[q]);
+      eb.push_back(std::move(r));
+    }
+    emit_segment(p, 0, eb);
+  }
+  if (p.slots > kLaneMaxSlot + 1) {
+    p.reason = "per-lane slot count exceeds the instruction encoding";
+    return p;
+  }
+  // every slot the kernel touches lies inside the lane's column, and every segment covers
+  // whole groups inside the instruction array
+  const uint32_t nslots = static_cast<uint32_t>(p.slots);
+  for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+    const size_t b = static_cast<size_t>(p.seg_begin[s]);
+    const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot;
+    if (b % nslot != 0 || e > p.ins.size() || p.seg_groups[s] <= 0) {
+      p.reason = "lane program segment table is inconsistent";
+      return p;
+    }
+    for (size_t k = b; k < e; ++k) {
+      const uint32_t w = p.ins[k];
+      const uint32_t end = w >> 30;
+      if ((w & 1023u) >= nslots || (p.seg_kind[s] == 0 && ((w >> 10) & 1023u) >= nslots) ||
+          ((end == kLaneEndStore || end == kLaneEndAdd) && ((w >> 20) & 1023u) >= nslots)) {
+        p.reason = "lane program addresses a slot outside the lane's column";
+        return p;
+      }
+    }
+  }
+  if (p.ins.size() != p.coeff.size() || p.ins.size() > (1u << 30)) {
+    p.reason = "lane program instruction table is inconsistent";
+    return p;
+  }
+
+  // self-check against the flat graph on two seeded basic vectors
+  uint64_t seed = 0x5eed5eedULL;
+  std::vector<double> b(B), g1(B), g2(B);
+  for (int trial = 0; trial < 2; ++trial) {
+    for (int i = 0; i < B; ++i) b[i] = (static_cast<double>(lcg(seed) % 2000001) / 1e6 - 1.0) * 0.5;
+    p.ok = true;
+    const double e1 = run_lane_program_host(p, b.data(), g1.data());
+    const double e2 = run_flat_graph_host(m, b.data(), g2.data());
+    double scale = std::abs(e2), err = std::abs(e1 - e2);
+    for (int i = 0; i < B; ++i) {
+      scale = std::max(scale, std::abs(g2[i]));
+      err = std::max(err, std::abs(g1[i] - g2[i]));
+    }
+    if (!(err <= 1e-9 * std::max(1.0, scale))) {
+      p.ok = false;
+      p.reason = "lane program self-check failed";
+      return p;
+    }
+  }
+  p.ok = true;
+  return p;
+}
+
+double run_lane_program_host(const LaneProgram& p, const double* basics, double* grad_b)
+{
+  const int B = p.basic_count, ns = p.nslot;
+  std::vector<double> X(p.slots, 0.0);
+  for (int i = 0; i < B; ++i) X[i] = basics[i];
+  X[p.ones_slot] = 1.0;
+  double e = 0.0;
+  std::vector<double> acc(ns), xv(ns), yv(ns);
+  for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    const bool bin = p.seg_kind[s] == 0;
+    for (int g = 0; g < p.seg_groups[s]; ++g) {
+      const size_t base = static_cast<size_t>(p.seg_begin[s]) + static_cast<size_t>(g) * ns;
+      for (int j = 0; j < ns; ++j) {
+        const uint32_t w = p.ins[base + j];
+        xv[j] = X[w & 1023u];
+        yv[j] = bin ? X[(w >> 10) & 1023u] : 1.0;
+      }
+      for (int j = 0; j < ns; ++j) acc[j] += p.coeff[base + j] * xv[j] * yv[j];
+      for (int j = 0; j < ns; ++j) {
+        const uint32_t w = p.ins[base + j];
+        const uint32_t end = w >> 30, t = (w >> 20) & 1023u;
+        if (end == kLaneEndStore) X[t] = acc[j];
+        if (end == kLaneEndAdd) X[t] += acc[j];
+        if (end == kLaneEndEnergy) e += acc[j];
+        if (end != kLaneEndNone) acc[j] = 0.0;
+      }
+    }
+  }
+  for (int i = 0; i < B; ++i) grad_b[i] = X[B + i];
+  return e;
+}
+
+double run_flat_graph_host(const Model& m, const double* basics, double* grad_b)
+{
+  const int A = m.moments_count, B = m.basic_count;
+  std::vector<double> M(A, 0.0), G(A, 0.0);
+  for (int i = 0; i < B; ++i) M[i] = basics[i];
+  for (const Product& q : m.products) M[q.target] += q.coeff * M[q.left] * M[q.right];
+  double e = 0.0;
+  for (int s = 0; s < m.scalar_count; ++s) {
+    e += m.moment_coeffs[s] * M[m.moment_mapping[s]];
+    G[m.moment_mapping[s]] += m.moment_coeffs[s];
+  }
+  for (size_t k = m.products.size(); k-- > 0;) {
+    const Product& q = m.products[k];
+    const double gt = q.coeff * G[q.target];
+    G[q.left] += gt * M[q.right];
+    G[q.right] += gt * M[q.left];
+  }
+  for (int i = 0; i < B; ++i) grad_b[i] = G[i];
+  return e;
+}
+
+}  // namespace sus2sh

@@ -1,0 +1,70 @@
+// Lane program: the product graph of a depth <= 2 model as one wave-uniform instruction
+// stream executed by one atom per lane (kernel in lane_kernel.hip).
+//
+// Semantics are those of the flat loop of src/force/sus2_sh.cu:4799-4846: forward
+// M[t] += c M[l] M[r], E = sum_s w_s M[map[s]], reverse through the same products.
+//
+// For a graph whose products are at most two layers deep (every standard graph up to body
+// order 5: layer 1 couples basic tensors, layer 2 contracts them into scalars) the
+// per-atom energy is a quadratic form in the layer-<=1 nodes:
+//   E' = sum_{layer-2 p} k_p v_l v_r + sum_n d_n v_n,   k_p = w_{t_p} c_p,  d_n = sum w_s [map_s = n]
+// Its layer-2 products couple only nodes of one (L, M) class, so the layer-<=1 nodes split
+// into small connected components (union-find over the layer-2 products).  A component is
+// evaluated on its own:
+//   FWD  v_n = sum c b_l b_r                   (layer-1 nodes; basic nodes alias b)
+//   ADJ  g_n = d_n + sum_{p contains n} k_p v_other   (the dE'/dv_n of the quadratic form)
+//   ENG  E' += 1/2 sum_n v_n (g_n + d_n)       (exact for a quadratic form)
+//   BWD  G_b[q] += sum c g_n b_other  (+ g_q for a basic node q)
+// Components are packed into batches whose nodes fit the per-lane value region; each batch
+// runs as three segments (FWD, ADJ, ENG+BWD).  Inside a segment rows are independent and
+// are spread longest-first over `nslot` accumulators per lane, interleaved as groups of
+// nslot instructions so that one group issues nslot independent LDS reads per operand.
+//
+// Per-lane slot map: [0, B) basics, [B, 2B) basic adjoints, 2B the constant 1,
+// [v_base, v_base + cap_v) component values, [g_base, g_base + cap_g) component adjoints.
+// Instruction word: x | y << 10 | t << 20 | end << 30 with end 0 (none), 1 (slot t = acc),
+// 2 (slot t += acc), 3 (site energy += acc).  Binary segments add c X[x] X[y], unary
+// segments add c X[x].
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "model.hpp"
+
+namespace sus2sh {
+
+struct LaneProgram {
+  bool ok = false;
+  std::string reason;  // why the model cannot use the lane program (when !ok)
+  int basic_count = 0;
+  int nslot = 8;
+  int cap_v = 0, cap_g = 0;  // value / adjoint region sizes
+  int slots = 0;  // per-lane slots (LDS = slots * 64 * sizeof(Real) per wave)
+  int ones_slot = 0, v_base = 0, g_base = 0;
+  int components = 0, largest_component = 0, batches = 0;
+  long long useful_terms = 0;  // instructions that are not padding
+  // segments: {kind (0 binary, 1 unary), first instruction, groups}
+  std::vector<int> seg_kind, seg_begin, seg_groups;
+  std::vector<uint32_t> ins;
+  std::vector<double> coeff;
+};
+
+constexpr uint32_t kLaneEndNone = 0, kLaneEndStore = 1, kLaneEndAdd = 2, kLaneEndEnergy = 3;
+constexpr int kLaneMaxSlot = 1023;
+
+// Build the lane program with at most `max_slots` per-lane slots (the value / adjoint
+// region sizes follow from it).  Never throws for an unsuitable graph: returns
+// ok = false with the reason instead.
+LaneProgram compile_lane_program(const Model& model, int max_slots, int nslot = 8);
+
+// Host execution of a lane program for one atom (test and self-check support):
+// returns E' and writes dE'/db into grad_b (B values).
+double run_lane_program_host(const LaneProgram& p, const double* basics, double* grad_b);
+
+// Host execution of the flat graph for one atom (same outputs), the reference the lane
+// program is checked against when it is compiled.
+double run_flat_graph_host(const Model& model, const double* basics, double* grad_b);
+
+}  // namespace sus2sh

@@ -34,3 +34,30 @@ def model_check(path):
             "product_count", "layer_count", "row_count", "row_term_count", "back_row_count",
             "back_term_count", "standard_graph")
     return dict(zip(keys, list(dims)))
+
+
+def lane_check(path, basics=None, max_slots=160):
+    """Host-only run of the lane program (the one-atom-per-lane graph kernel's instruction
+    stream) for basic vectors basics [n, B].  Returns (info dict, e_prime [n], grad [n, B]);
+    e_prime / grad are None when the model has no lane program."""
+    import ctypes
+    import numpy as np
+    from ._capi import check
+    P = ctypes.POINTER(ctypes.c_double)
+    info = (ctypes.c_int * 6)()
+    if basics is None:
+        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), 0, None, None, None,
+                                      info))
+        e = g = None
+    else:
+        b = np.ascontiguousarray(np.atleast_2d(basics), dtype=np.float64)
+        e = np.zeros(b.shape[0])
+        g = np.zeros_like(b)
+        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), b.shape[0],
+                                      b.ctypes.data_as(P), e.ctypes.data_as(P),
+                                      g.ctypes.data_as(P), info))
+    keys = ("ok", "slots", "instructions", "components", "batches", "largest_component")
+    d = dict(zip(keys, list(info)))
+    if not d["ok"]:
+        e = g = None
+    return d, e, g

@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_graph_export",
     "sus2sh_graph_free",
     "sus2sh_model_check",
+    "sus2sh_lane_check",
 )
 
 
@@ -77,6 +78,9 @@ class ModelInfo(ctypes.Structure):
         ("graph_variant", ctypes.c_int),
         ("neighbor_capacity", ctypes.c_int),
         ("max_dist", ctypes.c_double),
+        ("lane_program", ctypes.c_int),
+        ("lane_slots", ctypes.c_int),
+        ("lane_instructions", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -158,6 +162,9 @@ def lib():
     L.sus2sh_graph_free.restype = None
     L.sus2sh_model_check.argtypes = [c_char_p, P(c_int)]
     L.sus2sh_model_check.restype = c_int
+    L.sus2sh_lane_check.argtypes = [c_char_p, c_int, c_int, P(c_double), P(c_double),
+                                    P(c_double), P(c_int)]
+    L.sus2sh_lane_check.restype = c_int
     _lib = L
     return L
 

@@ -40,6 +40,8 @@ def lib():
     L.oracle_compute.argtypes = [c_void_p, c_int, P(c_int), P(c_double), P(c_double), P(c_int),
                                  c_int, P(c_double), P(c_double), P(c_double), P(c_double)]
     L.oracle_compute.restype = c_int
+    L.oracle_graph.argtypes = [c_void_p, c_int, P(c_double), P(c_double), P(c_double)]
+    L.oracle_graph.restype = c_int
     L.oracle_ylm.argtypes = [c_int, c_double, c_double, c_double, P(c_double), P(c_double)]
     L.oracle_ylm.restype = None
     L.oracle_radial.argtypes = [c_void_p, c_int, c_int, c_double, P(c_double), P(c_double)]
@@ -113,6 +115,16 @@ class Oracle:
         dR = np.zeros(Mr)
         lib().oracle_radial(self._m, ti, tj, float(r), _dp(R), _dp(dR))
         return R, dR
+
+    def graph(self, basics):
+        """Graph part alone: basics [n, B] -> (E' [n], dE'/dbasic [n, B])."""
+        b = np.ascontiguousarray(np.atleast_2d(basics), dtype=np.float64)
+        n = b.shape[0]
+        e = np.zeros(n)
+        g = np.zeros_like(b)
+        if lib().oracle_graph(self._m, n, _dp(b), _dp(e), _dp(g)) != 0:
+            raise MemoryError("oracle_graph")
+        return e, g
 
     def compute(self, cell, types, positions, pbc=(1, 1, 1), brute=False, param_grad=False):
         """Returns (energy[N], force[3,N], virial[9,N]) and optionally dE/dtheta."""

@@ -719,6 +719,52 @@ int oracle_param_count(const oracle_model* m)
 
 /* mode bit 0: use brute-force neighbours (else cell list).
  * dparam (optional, length oracle_param_count): accumulates dE_total/dtheta.        */
+/* Flat product graph on the moments M[0..A) whose basics M[0..B) are set
+ * (src/force/sus2_sh.cu:4799-4846): M[t] += c M[l] M[r] in file order, returns
+ * E' = sum_s w_s M[map[s]]. */
+static double or_graph_forward(const oracle_model* m, double* M)
+{
+  for (int p = 0; p < m->T; ++p)
+    M[m->p_target[p]] += m->p_coeff[p] * M[m->p_left[p]] * M[m->p_right[p]];
+  double sum = 0.0;
+  for (int s = 0; s < m->Q; ++s) sum += m->moment_coeffs[s] * M[m->mapping[s]];
+  return sum;
+}
+
+/* Reverse of or_graph_forward scaled by `seed`: G = seed * dE'/dM for every node. */
+static void or_graph_reverse(const oracle_model* m, double seed, const double* M, double* G)
+{
+  for (int a = 0; a < m->A; ++a) G[a] = 0.0;
+  for (int s = 0; s < m->Q; ++s) G[m->mapping[s]] += seed * m->moment_coeffs[s];
+  for (int p = m->T - 1; p >= 0; --p) {
+    const double g = G[m->p_target[p]] * m->p_coeff[p];
+    G[m->p_left[p]] += g * M[m->p_right[p]];
+    G[m->p_right[p]] += g * M[m->p_left[p]];
+  }
+}
+
+/* Graph part alone for `n` atoms given their basic moments basics[n][B]:
+ * e_prime[n] = E', grad[n][B] = dE'/d(basic). */
+int oracle_graph(const oracle_model* m, int n, const double* basics, double* e_prime, double* grad)
+{
+  double* M = (double*)malloc((size_t)m->A * sizeof(double));
+  double* G = (double*)malloc((size_t)m->A * sizeof(double));
+  if (!M || !G) {
+    free(M);
+    free(G);
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < m->A; ++a) M[a] = a < m->B ? basics[(size_t)i * m->B + a] : 0.0;
+    e_prime[i] = or_graph_forward(m, M);
+    or_graph_reverse(m, 1.0, M, G);
+    for (int b = 0; b < m->B; ++b) grad[(size_t)i * m->B + b] = G[b];
+  }
+  free(M);
+  free(G);
+  return 0;
+}
+
 int oracle_compute(const oracle_model* m, int N, const int* type, const double* pos,
                    const double* h, const int* pbc, int mode, double* energy, double* force,
                    double* virial, double* dparam)
@@ -770,11 +816,8 @@ int oracle_compute(const oracle_model* m, int N, const int* type, const double* 
         }
       }
       /* flat product graph */
-      for (int p = 0; p < m->T; ++p)
-        M[m->p_target[p]] += m->p_coeff[p] * M[m->p_left[p]] * M[m->p_right[p]];
       const double sigma = m->species[ti];
-      double sum = 0.0;
-      for (int s = 0; s < m->Q; ++s) sum += m->moment_coeffs[s] * M[m->mapping[s]];
+      const double sum = or_graph_forward(m, M);
       energy[i] = m->shift[ti] + sigma + sigma * sum;
       if (pg) {
         pg[off_shift + ti] += 1.0;
@@ -782,13 +825,7 @@ int oracle_compute(const oracle_model* m, int N, const int* type, const double* 
         for (int s = 0; s < m->Q; ++s) pg[off_w + s] += sigma * M[m->mapping[s]];
       }
       /* reverse */
-      for (int a = 0; a < A; ++a) G[a] = 0.0;
-      for (int s = 0; s < m->Q; ++s) G[m->mapping[s]] += sigma * m->moment_coeffs[s];
-      for (int p = m->T - 1; p >= 0; --p) {
-        const double g = G[m->p_target[p]] * m->p_coeff[p];
-        G[m->p_left[p]] += g * M[m->p_right[p]];
-        G[m->p_right[p]] += g * M[m->p_left[p]];
-      }
+      or_graph_reverse(m, sigma, M, G);
       /* forces and virial */
       double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       double fs[3] = {0, 0, 0};

@@ -198,6 +198,27 @@ def test_parameter_gradient_matches_oracle(make_model, fp32):
     pot.close()
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("fp32", [False, True])
+def test_lane_program_matches_oracle(make_model, name, fp32):
+    """Graph variant 3 (one atom per lane, wave-uniform program) on every case; the
+    body-order-6 graph is three layers deep and has no lane program: rejected cleanly."""
+    case = CASES[name]
+    path = make_model(name, **case["model"])
+    S = case["model"].get("species_count", 2)
+    pos, types, cell = _structure(case["struct"], S)
+    opts = [f"T{s}" for s in range(S)] + ["sus2_float=%d" % (1 if fp32 else 0),
+                                          "sus2_sh_graph_variant=3"]
+    if case["model"]["body_order"] > 5:
+        with pytest.raises(sus2sh.Sus2shError) as err:
+            _run(path, cell, types, pos, options=opts)
+        assert "no lane program" in err.value.message
+        return
+    ref = Oracle(path).compute(cell, types, pos)
+    got = _run(path, cell, types, pos, options=opts)
+    _compare(ref, got, fp32)
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("tile", [1, 4, 16])
 def test_graph_kernel_variants(make_model, variant, tile):

@@ -95,7 +95,8 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
     dt = time.perf_counter() - t0
     nbar = pot.mean_neighbors()
     info = pot.info()
-    tuning = {f"v{v}_w{w}": round(ms, 4) for (v, w), ms in pot.tuning().items()}
+    tuning = {(f"v{v}_w{w}" if v < 3 else f"lane{w}"): round(ms, 4)
+              for (v, w), ms in pot.tuning().items()}
     out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
                info=info, nbar=nbar, energy=float(e.sum().item()), tuning=tuning,
                force=f.cpu().numpy(), stream=stream.cuda_stream)

@@ -305,13 +305,15 @@ int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_
 
 void sus2sh_graph_free(sus2sh_graph* graph) { delete graph; }
 
-int sus2sh_lane_check(const char* mtp_path, int max_slots, int n, const double* basics,
-                      double* e_prime, double* grad, int* info)
+int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
+                      const double* basics, double* e_prime, double* grad, int* info)
 {
-  if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad))) return SUS2SH_ERR_ARGUMENT;
+  if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
+      (waves != 1 && waves != 2 && waves != 4))
+    return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);
-    const sus2sh::LaneProgram p = sus2sh::compile_lane_program(m, max_slots);
+    const sus2sh::LaneProgram p = sus2sh::compile_lane_program(m, max_slots, 8, waves);
     if (info) {
       info[0] = p.ok ? 1 : 0;
       info[1] = p.slots;

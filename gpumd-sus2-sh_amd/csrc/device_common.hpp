@@ -8,11 +8,17 @@
 namespace sus2sh {
 
 // One neighbour edge, stored k-major (edge (k, p) at k * N + p) so that a wavefront whose
-// lanes are 64 consecutive atoms reads 64 consecutive edges.
+// lanes are 64 consecutive atoms reads 64 consecutive edges.  j packs the neighbour's
+// cell-sorted index (low 27 bits) and its species (bits 27-30), so one 16-byte load gives
+// everything an edge kernel needs.
+constexpr int kEdgeTypeShift = 27;
+constexpr int kEdgeIndexMask = (1 << kEdgeTypeShift) - 1;
+constexpr int kMaxEdgeAtoms = 1 << kEdgeTypeShift;
+
 template <typename Real>
 struct alignas(16) Edge {
   Real dx, dy, dz;  // r_q(image) - r_p
-  int j;            // neighbour in cell-sorted order
+  int j;            // neighbour index | species << kEdgeTypeShift
 };
 template <>
 struct alignas(16) Edge<double> {

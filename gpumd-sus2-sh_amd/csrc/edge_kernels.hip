@@ -47,12 +47,15 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   const int S = a.rt.S;
   const int rb = a.rt.rb;
   const Real rc = a.rt.rc;
+  // the next edge is loaded while the current one is evaluated
+  Edge<Real> next = cnt > 0 ? a.nl[p] : Edge<Real>{};
   for (int k = 0; k < cnt; ++k) {
-    const Edge<Real> e = a.nl[static_cast<size_t>(k) * N + p];
+    const Edge<Real> e = next;
+    if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
     const Real r = sqrt(r2);
     const Real inv = Real(1) / r;
-    const int pair = ti * S + a.type_s[e.j];
+    const int pair = ti * S + (e.j >> kEdgeTypeShift);
     Real R[MR], dR[MR];
     eval_radial<Real, MR, false>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
                                  a.rt.kind, a.rt.uniform, r, R, dR);
@@ -164,13 +167,16 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   const Real rc = a.rt.rc;
   Real fx = 0, fy = 0, fz = 0;
   Real w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  Edge<Real> next = cnt > 0 ? a.nl[p] : Edge<Real>{};
   for (int k = 0; k < cnt; ++k) {
-    const Edge<Real> e = a.nl[static_cast<size_t>(k) * N + p];
+    const Edge<Real> e = next;
+    if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
     const Real r = sqrt(r2);
     const Real inv = Real(1) / r;
     const Real ux = e.dx * inv, uy = e.dy * inv, uz = e.dz * inv;
-    const int pair = ti * S + a.type_s[e.j];
+    const int pair = ti * S + (e.j >> kEdgeTypeShift);
+    const int jn = e.j & kEdgeIndexMask;
     Real R[MR], dR[MR];
     eval_radial<Real, MR, true>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
                                 a.rt.kind, a.rt.uniform, r, R, dR);
@@ -221,9 +227,9 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     fx += g0;
     fy += g1;
     fz += g2;
-    unsafeAtomicAdd(f_s + e.j, -static_cast<double>(g0));
-    unsafeAtomicAdd(f_s + N + e.j, -static_cast<double>(g1));
-    unsafeAtomicAdd(f_s + 2 * N + e.j, -static_cast<double>(g2));
+    unsafeAtomicAdd(f_s + jn, -static_cast<double>(g0));
+    unsafeAtomicAdd(f_s + N + jn, -static_cast<double>(g1));
+    unsafeAtomicAdd(f_s + 2 * N + jn, -static_cast<double>(g2));
     w[0] -= e.dx * g0;
     w[1] -= e.dy * g1;
     w[2] -= e.dz * g2;

@@ -51,6 +51,10 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       const int v = std::atoi(val.c_str());
       if (v < 0 || v > 3) throw InputError("sus2_sh_graph_variant must be 0, 1, 2 or 3: " + val);
       o.graph_variant = v;
+    } else if (key == "sus2_sh_lane_waves") {
+      const int w = std::atoi(val.c_str());
+      if (w != 1 && w != 2 && w != 4) throw InputError("sus2_sh_lane_waves must be 1, 2 or 4: " + val);
+      o.lane_waves = w;  // fixes the lane program shape (4 accumulators per wave)
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -166,22 +170,59 @@ int Engine::param_count() const
          2 * S * S * model_.radial_funcs_count;
 }
 
-// Lane program with the smallest per-lane slot budget that holds every graph component:
-// 160 slots are 40 KiB of LDS per 64-atom workgroup in FP32 (four workgroups per CU).
-void Engine::compile_lane()
+// Lane program of one shape: the smallest per-lane slot budget >= min_slots that holds every
+// graph component (160 slots are 40 KiB of LDS per 64-atom FP32 tile).
+LaneProgram Engine::build_lane(const LaneShape& shape) const
 {
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
-  lane_ = LaneProgram{};
-  lane_.reason = "no slot budget fits the LDS of a workgroup";
-  for (int max_slots : {160, 256, 384, 512, 1024}) {
+  LaneProgram best;
+  best.reason = "no slot budget fits the LDS of a workgroup";
+  for (int max_slots : {160, 192, 256, 320, 384, 512, 640, 1024}) {
+    if (max_slots < shape.min_slots) continue;
     if (static_cast<size_t>(max_slots) * 64 * rsz > max_lds_) break;
-    LaneProgram p = compile_lane_program(model_, max_slots);
-    if (p.ok) {
-      lane_ = std::move(p);
-      return;
-    }
-    lane_.reason = p.reason;
-    if (p.reason.find("slot budget") == std::string::npos) return;  // not a budget limit
+    LaneProgram p = compile_lane_program(model_, max_slots, shape.nslot, shape.waves);
+    if (p.ok) return p;
+    best.reason = p.reason;
+    if (p.reason.find("slot budget") == std::string::npos) break;  // not a budget limit
+  }
+  return best;
+}
+
+// Shapes the autotuner tries (the first feasible one is the default): waves per 64-atom
+// tile x accumulators per wave x minimum slot budget.
+const std::vector<Engine::LaneShape>& Engine::lane_shapes()
+{
+  static const std::vector<LaneShape> shapes = {
+    {4, 4, 160}, {2, 4, 160}, {4, 4, 192}, {4, 4, 256}, {1, 8, 160}};
+  return shapes;
+}
+
+void Engine::compile_lane()
+{
+  if (opt_.lane_waves > 0) {
+    lane_ = build_lane(LaneShape{opt_.lane_waves, 4, 160});
+    return;
+  }
+  lane_ = LaneProgram{};
+  for (const LaneShape& sh : lane_shapes()) {
+    lane_ = build_lane(sh);
+    if (lane_.ok) return;
+  }
+}
+
+void Engine::upload_lane()
+{
+  const size_t nseg = lane_.seg_kind.size();
+  std::vector<int4> segs(nseg);
+  for (size_t k = 0; k < nseg; ++k)
+    segs[k] = make_int4(lane_.seg_kind[k], lane_.seg_begin[k], lane_.seg_groups[k], 0);
+  lane_segs_.upload(segs.data(), segs.size() * sizeof(int4));
+  lane_ins_.upload(lane_.ins.data(), lane_.ins.size() * sizeof(uint32_t));
+  if (opt_.use_float) {
+    std::vector<float> lc(lane_.coeff.begin(), lane_.coeff.end());
+    lane_coeff_.upload(lc.data(), lc.size() * sizeof(float));
+  } else {
+    lane_coeff_.upload(lane_.coeff.data(), lane_.coeff.size() * sizeof(double));
   }
 }
 
@@ -279,16 +320,7 @@ void Engine::upload_tables()
   row_slist_.upload(g.row_scalar_list.data(), g.row_scalar_list.size() * sizeof(int));
   stored_index_.upload(g.stored_scalar_index.data(), g.stored_scalar_index.size() * sizeof(int));
   stored_slot_.upload(g.stored_scalar_slot.data(), g.stored_scalar_slot.size() * sizeof(int));
-  if (lane_.ok) {
-    const size_t nseg = lane_.seg_kind.size();
-    std::vector<int4> segs(nseg);
-    for (size_t k = 0; k < nseg; ++k)
-      segs[k] = make_int4(lane_.seg_kind[k], lane_.seg_begin[k], lane_.seg_groups[k], 0);
-    lane_segs_.upload(segs.data(), segs.size() * sizeof(int4));
-    lane_ins_.upload(lane_.ins.data(), lane_.ins.size() * sizeof(uint32_t));
-    std::vector<Real> lc(lane_.coeff.begin(), lane_.coeff.end());
-    lane_coeff_.upload(lc.data(), lc.size() * sizeof(Real));
-  }
+  if (lane_.ok) upload_lane();
 }
 
 void Engine::ensure_atoms(int N, size_t edge_bytes)
@@ -404,7 +436,10 @@ bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable
     lt.nseg = static_cast<int>(lane_.seg_kind.size());
     lt.basics = model_.basic_count;
     lt.ones_slot = lane_.ones_slot;
+    lt.v_base = lane_.v_base;
     lt.slots = lane_.slots;
+    lt.nslot = lane_.nslot;
+    lt.waves = lane_.waves;
     return launch_lane_graph<Real>(N, basics_.as<Real>(), type_s_.as<int>(), lt, e_s_.as<double>(),
                                    grads_.as<Real>(), s, max_lds_);
   }
@@ -425,6 +460,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
                  const int* type, const double* pos, double* pot, double* force, double* virial,
                  hipStream_t s, double* param_grad)
 {
+  if (N >= kMaxEdgeAtoms)
+    throw ArgumentError("SUS2_SH: at most 134217727 atoms per call (neighbour index width)");
   const bool batch = nstruct > 1;
   std::vector<CellGrid> grids(nstruct);
   std::vector<int> cell_off(nstruct + 1, 0);
@@ -483,9 +520,9 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     na.cap = cap_;
     if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
     launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
-    launch_neighbors<Real>(na, pos_s_.as<double>(), skey_.as<int>(), cell_begin_.as<int>(),
-                           cell_end_.as<int>(), count_.as<int>(), nl_.as<Edge<Real>>(),
-                           max_count_.as<int>(), s);
+    launch_neighbors<Real>(na, pos_s_.as<double>(), type_s_.as<int>(), skey_.as<int>(),
+                           cell_begin_.as<int>(), cell_end_.as<int>(), count_.as<int>(),
+                           nl_.as<Edge<Real>>(), max_count_.as<int>(), s);
     hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync D2H");
@@ -569,25 +606,42 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     hip_check(hipEventCreate(&t1), "hipEventCreate");
     float best = 1e30f;
     int best_tile = tile_, best_variant = variant_;
-    tune_ms_.assign(16, -1.0f);
+    const std::vector<LaneShape>& shapes = lane_shapes();
+    const int n_lane = opt_.lane_waves > 0 ? 1 : static_cast<int>(shapes.size());
+    tune_ms_.assign(15 + shapes.size(), -1.0f);
+    LaneProgram best_lane = lane_;
+    auto time_it = [&](int v, int t) {
+      float ms = 0.0f;
+      if (!launch_graph_variant<Real>(v, t, N, gt, s)) return -1.0f;
+      hip_check(hipEventRecord(t0, s), "hipEventRecord");
+      for (int rep = 0; rep < 2; ++rep) launch_graph_variant<Real>(v, t, N, gt, s);
+      hip_check(hipEventRecord(t1, s), "hipEventRecord");
+      hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+      hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+      return ms / 2;
+    };
     for (int v = 0; v < 4; ++v) {
       if (opt_.graph_variant >= 0 && v != opt_.graph_variant) continue;  // fixed variant
-      for (int li = 0; li < (v == 3 ? 1 : 5); ++li) {
+      for (int li = 0; li < (v == 3 ? n_lane : 5); ++li) {
         const int t = 1 << li;
         if (v < 3 && graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
-        if (v == 3 && !lane_.ok) continue;
-        float ms = 0.0f;
-        if (!launch_graph_variant<Real>(v, t, N, gt, s)) continue;
-        hip_check(hipEventRecord(t0, s), "hipEventRecord");
-        for (int rep = 0; rep < 2; ++rep) launch_graph_variant<Real>(v, t, N, gt, s);
-        hip_check(hipEventRecord(t1, s), "hipEventRecord");
-        hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
-        hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
-        tune_ms_[v == 3 ? 15 : v * 5 + li] = ms / 2;
+        if (v == 3) {
+          if (opt_.lane_waves <= 0) {
+            lane_ = build_lane(shapes[li]);
+            if (!lane_.ok) continue;
+            upload_lane();
+          } else if (!lane_.ok) {
+            continue;
+          }
+        }
+        const float ms = time_it(v, t);
+        if (ms < 0.0f) continue;
+        tune_ms_[v == 3 ? 15 + li : v * 5 + li] = ms;
         if (ms < best) {
           best = ms;
           best_tile = t;
           best_variant = v;
+          if (v == 3) best_lane = lane_;
         }
       }
     }
@@ -595,6 +649,11 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     (void)hipEventDestroy(t1);
     if (best_variant < 3) tile_ = best_tile;
     variant_ = best_variant;
+    if (best_lane.ok && (best_lane.waves != lane_.waves || best_lane.nslot != lane_.nslot ||
+                         best_lane.slots != lane_.slots)) {
+      lane_ = std::move(best_lane);
+      upload_lane();
+    }
     tuned_ = true;
   }
   if (param_grad) {

@@ -61,6 +61,7 @@ struct EngineOptions {
                           // 3 lane program
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
+  int lane_waves = 0;    // lane kernel: waves per 64-atom tile (0: autotuned shape)
 };
 
 class Engine {
@@ -101,7 +102,15 @@ class Engine {
  private:
   template <typename Real>
   void upload_tables();
+  struct LaneShape {
+    int waves;      // waves per 64-atom tile
+    int nslot;      // accumulators per wave
+    int min_slots;  // smallest per-lane slot budget tried
+  };
+  static const std::vector<LaneShape>& lane_shapes();
+  LaneProgram build_lane(const LaneShape& shape) const;
   void compile_lane();
+  void upload_lane();
   template <typename Real>
   bool launch_graph_variant(int variant, int tile, int N, const GraphTable<Real>& gt,
                             hipStream_t s);
@@ -119,7 +128,8 @@ class Engine {
   int tile_ = 1;
   int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane
   bool tuned_ = false;
-  // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2, [15] lane
+  // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2 and
+  // [15 + c] for lane-program shape c
   std::vector<float> tune_ms_;
   LaneProgram lane_;
   int cap_ = 0;

@@ -79,9 +79,9 @@ void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
                    int* cell_begin, int* cell_end, int* bad_type, const int* sid, int* sid_s,
                    hipStream_t s);
 template <typename Real>
-void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sorted_key,
-                      const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,
-                      int* max_count, hipStream_t s);
+void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* type_s,
+                      const int* sorted_key, const int* cell_begin, const int* cell_end,
+                      int* count, Edge<Real>* nl, int* max_count, hipStream_t s);
 size_t radix_sort_temp_bytes(int N, int bits);
 hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
                             const int* vals_in, int* vals_out, int N, int bits, hipStream_t s);
@@ -133,11 +133,14 @@ struct LaneTable {
   int nseg;
   int basics;
   int ones_slot;
-  int slots;               // per-lane slots: LDS of a 64-lane workgroup = slots * 64 * sizeof(Real)
+  int v_base;              // first component-value slot (reused for the per-wave energies)
+  int slots;               // per-lane slots: LDS of a workgroup = slots * 64 * sizeof(Real)
+  int nslot;               // accumulators per wave (instructions per group)
+  int waves;               // waves sharing one 64-atom tile
 };
 
-// Lane graph kernel (lane_kernel.hip): workgroups of 64 atoms; returns false when the
-// per-workgroup LDS would exceed max_lds.
+// Lane graph kernel (lane_kernel.hip): one 64-atom tile per workgroup of lt.waves waves;
+// returns false when the tile's LDS would exceed max_lds or the program shape is unknown.
 template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);

@@ -28,11 +28,13 @@ int find_root(std::vector<int>& parent, int x)
   return x;
 }
 
-// Longest-first assignment of independent rows to nslot accumulators, interleaved as groups
-// of nslot instructions (padding: c = 0 on the constant slot).
+// Longest-first assignment of independent rows to waves * nslot accumulators.  The
+// instructions of wave w form groups of nslot (slot j feeds accumulator j); every wave of a
+// segment has the same number of groups (padding: c = 0 on the constant slot).
+// Layout: instruction j of group g of wave w at begin + (w * groups + g) * nslot + j.
 void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
 {
-  const int ns = p.nslot;
+  const int ns = p.nslot, bins = p.nslot * p.waves;
   rows.erase(std::remove_if(rows.begin(), rows.end(), [](const Row& r) { return r.terms.empty(); }),
              rows.end());
   if (rows.empty()) return;
@@ -40,17 +42,17 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return rows[a].terms.size() > rows[b].terms.size(); });
-  std::vector<std::vector<int>> per(ns);
-  std::vector<size_t> load(ns, 0);
+  std::vector<std::vector<int>> per(bins);
+  std::vector<size_t> load(bins, 0);
   for (int r : order) {
     const int j = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
     per[j].push_back(r);
     load[j] += rows[r].terms.size();
   }
   const size_t groups = *std::max_element(load.begin(), load.end());
-  std::vector<std::vector<uint32_t>> words(ns);
-  std::vector<std::vector<double>> cs(ns);
-  for (int j = 0; j < ns; ++j) {
+  std::vector<std::vector<uint32_t>> words(bins);
+  std::vector<std::vector<double>> cs(bins);
+  for (int j = 0; j < bins; ++j) {
     for (int r : per[j]) {
       const Row& row = rows[r];
       for (size_t k = 0; k < row.terms.size(); ++k) {
@@ -73,11 +75,12 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
   p.seg_kind.push_back(kind);
   p.seg_begin.push_back(static_cast<int>(p.ins.size()));
   p.seg_groups.push_back(static_cast<int>(groups));
-  for (size_t g = 0; g < groups; ++g)
-    for (int j = 0; j < ns; ++j) {
-      p.ins.push_back(words[j][g]);
-      p.coeff.push_back(cs[j][g]);
-    }
+  for (int w = 0; w < p.waves; ++w)
+    for (size_t g = 0; g < groups; ++g)
+      for (int j = 0; j < ns; ++j) {
+        p.ins.push_back(words[w * ns + j][g]);
+        p.coeff.push_back(cs[w * ns + j][g]);
+      }
 }
 
 uint64_t lcg(uint64_t& s)
@@ -88,10 +91,11 @@ uint64_t lcg(uint64_t& s)
 
 }  // namespace
 
-LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot)
+LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int waves)
 {
   LaneProgram p;
   p.nslot = nslot;
+  p.waves = waves;
   const int A = m.moments_count, B = m.basic_count;
   p.basic_count = B;
   if (m.layer_count > 2) {
@@ -256,7 +260,8 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot)
         et.push_back({0.5, vslot[n], gslot[n]});
         if (mapped[n]) et.push_back({0.5 * d[n], vslot[n], p.ones_slot});
       }
-      const int parts = std::max(1, std::min<int>(nslot, static_cast<int>(et.size() + 3) / 4));
+      const int parts =
+        std::max(1, std::min<int>(nslot * waves, static_cast<int>(et.size() + 3) / 4));
       for (int k = 0; k < parts; ++k) {
         Row r;
         r.target = 0;
@@ -300,7 +305,7 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot)
   const uint32_t nslots = static_cast<uint32_t>(p.slots);
   for (size_t s = 0; s < p.seg_kind.size(); ++s) {
     const size_t b = static_cast<size_t>(p.seg_begin[s]);
-    const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot;
+    const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot * waves;
     if (b % nslot != 0 || e > p.ins.size() || p.seg_groups[s] <= 0) {
       p.reason = "lane program segment table is inconsistent";
       return p;
@@ -312,6 +317,34 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot)
           ((end == kLaneEndStore || end == kLaneEndAdd) && ((w >> 20) & 1023u) >= nslots)) {
         p.reason = "lane program addresses a slot outside the lane's column";
         return p;
+      }
+    }
+  }
+  // waves of a tile run a segment concurrently: within a segment every stored / added slot
+  // is the target of one row only and is never read
+  {
+    std::vector<int> mark(p.slots, -1);
+    for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+      const size_t b = static_cast<size_t>(p.seg_begin[s]);
+      const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot * waves;
+      for (size_t k = b; k < e; ++k) {
+        const uint32_t w = p.ins[k], end = w >> 30;
+        if (end == kLaneEndStore || end == kLaneEndAdd) {
+          const int t = static_cast<int>((w >> 20) & 1023u);
+          if (mark[t] == static_cast<int>(s)) {
+            p.reason = "lane program writes one slot twice in a segment";
+            return p;
+          }
+          mark[t] = static_cast<int>(s);
+        }
+      }
+      for (size_t k = b; k < e; ++k) {
+        const uint32_t w = p.ins[k];
+        if (mark[w & 1023u] == static_cast<int>(s) ||
+            (p.seg_kind[s] == 0 && mark[(w >> 10) & 1023u] == static_cast<int>(s))) {
+          p.reason = "lane program reads a slot its segment writes";
+          return p;
+        }
       }
     }
   }
@@ -352,23 +385,30 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
   double e = 0.0;
   std::vector<double> acc(ns), xv(ns), yv(ns);
   for (size_t s = 0; s < p.seg_kind.size(); ++s) {
-    std::fill(acc.begin(), acc.end(), 0.0);
     const bool bin = p.seg_kind[s] == 0;
-    for (int g = 0; g < p.seg_groups[s]; ++g) {
-      const size_t base = static_cast<size_t>(p.seg_begin[s]) + static_cast<size_t>(g) * ns;
-      for (int j = 0; j < ns; ++j) {
-        const uint32_t w = p.ins[base + j];
-        xv[j] = X[w & 1023u];
-        yv[j] = bin ? X[(w >> 10) & 1023u] : 1.0;
-      }
-      for (int j = 0; j < ns; ++j) acc[j] += p.coeff[base + j] * xv[j] * yv[j];
-      for (int j = 0; j < ns; ++j) {
-        const uint32_t w = p.ins[base + j];
-        const uint32_t end = w >> 30, t = (w >> 20) & 1023u;
-        if (end == kLaneEndStore) X[t] = acc[j];
-        if (end == kLaneEndAdd) X[t] += acc[j];
-        if (end == kLaneEndEnergy) e += acc[j];
-        if (end != kLaneEndNone) acc[j] = 0.0;
+    // waves run concurrently on the device: each wave's stream is replayed against a
+    // snapshot of the slots taken at the start of the segment, and the writes of all waves
+    // are applied afterwards (rows of one segment never read a slot the segment writes)
+    const std::vector<double> X0 = X;
+    for (int w = 0; w < p.waves; ++w) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      for (int g = 0; g < p.seg_groups[s]; ++g) {
+        const size_t base = static_cast<size_t>(p.seg_begin[s]) +
+                            (static_cast<size_t>(w) * p.seg_groups[s] + g) * ns;
+        for (int j = 0; j < ns; ++j) {
+          const uint32_t wd = p.ins[base + j];
+          xv[j] = X0[wd & 1023u];
+          yv[j] = bin ? X0[(wd >> 10) & 1023u] : 1.0;
+        }
+        for (int j = 0; j < ns; ++j) acc[j] += p.coeff[base + j] * xv[j] * yv[j];
+        for (int j = 0; j < ns; ++j) {
+          const uint32_t wd = p.ins[base + j];
+          const uint32_t end = wd >> 30, t = (wd >> 20) & 1023u;
+          if (end == kLaneEndStore) X[t] = acc[j];
+          if (end == kLaneEndAdd) X[t] += acc[j];
+          if (end == kLaneEndEnergy) e += acc[j];
+          if (end != kLaneEndNone) acc[j] = 0.0;
+        }
       }
     }
   }

@@ -16,9 +16,10 @@
 //   ENG  E' += 1/2 sum_n v_n (g_n + d_n)       (exact for a quadratic form)
 //   BWD  G_b[q] += sum c g_n b_other  (+ g_q for a basic node q)
 // Components are packed into batches whose nodes fit the per-lane value region; each batch
-// runs as three segments (FWD, ADJ, ENG+BWD).  Inside a segment rows are independent and
-// are spread longest-first over `nslot` accumulators per lane, interleaved as groups of
-// nslot instructions so that one group issues nslot independent LDS reads per operand.
+// runs as three segments (FWD, ADJ, ENG+BWD).  Inside a segment rows are independent: they
+// are spread longest-first over `waves` x `nslot` accumulators -- `waves` waves share one
+// 64-atom tile and split the rows, each wave's rows interleaved as groups of nslot
+// instructions so that one group issues nslot independent LDS reads per operand.
 //
 // Per-lane slot map: [0, B) basics, [B, 2B) basic adjoints, 2B the constant 1,
 // [v_base, v_base + cap_v) component values, [g_base, g_base + cap_g) component adjoints.
@@ -40,6 +41,7 @@ struct LaneProgram {
   std::string reason;  // why the model cannot use the lane program (when !ok)
   int basic_count = 0;
   int nslot = 8;
+  int waves = 1;  // waves that share one 64-atom tile and split every segment's rows
   int cap_v = 0, cap_g = 0;  // value / adjoint region sizes
   int slots = 0;  // per-lane slots (LDS = slots * 64 * sizeof(Real) per wave)
   int ones_slot = 0, v_base = 0, g_base = 0;
@@ -57,7 +59,8 @@ constexpr int kLaneMaxSlot = 1023;
 // Build the lane program with at most `max_slots` per-lane slots (the value / adjoint
 // region sizes follow from it).  Never throws for an unsuitable graph: returns
 // ok = false with the reason instead.
-LaneProgram compile_lane_program(const Model& model, int max_slots, int nslot = 8);
+LaneProgram compile_lane_program(const Model& model, int max_slots, int nslot = 8,
+                                 int waves = 1);
 
 // Host execution of a lane program for one atom (test and self-check support):
 // returns E' and writes dE'/db into grad_b (B values).

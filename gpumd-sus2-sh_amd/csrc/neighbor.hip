@@ -112,6 +112,7 @@ __device__ __forceinline__ int wrap_cell(int c, int n, int& shift)
 
 template <typename Real, bool BATCH>
 __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double* __restrict__ pos,
+                                                   const int* __restrict__ type_s,
                                                    const int* __restrict__ skey,
                                                    const int* __restrict__ cbeg,
                                                    const int* __restrict__ cend,
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
               e.dx = static_cast<Real>(dx);
               e.dy = static_cast<Real>(dy);
               e.dz = static_cast<Real>(dz);
-              e.j = q;
+              e.j = q | (type_s[q] << kEdgeTypeShift);
               nl[static_cast<size_t>(c) * N + p] = e;
             }
             ++c;
@@ -247,22 +248,24 @@ void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
 }
 
 template <typename Real>
-void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* sorted_key,
-                      const int* cell_begin, const int* cell_end, int* count, Edge<Real>* nl,
-                      int* max_count, hipStream_t s)
+void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* type_s,
+                      const int* sorted_key, const int* cell_begin, const int* cell_end,
+                      int* count, Edge<Real>* nl, int* max_count, hipStream_t s)
 {
   if (a.grids)
-    k_neighbors<Real, true><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
-                                                                 cell_end, count, nl, max_count);
+    k_neighbors<Real, true><<<blocks_for(a.N, 256), 256, 0, s>>>(
+      a, pos_s, type_s, sorted_key, cell_begin, cell_end, count, nl, max_count);
   else
-    k_neighbors<Real, false><<<blocks_for(a.N, 256), 256, 0, s>>>(a, pos_s, sorted_key, cell_begin,
-                                                                  cell_end, count, nl, max_count);
+    k_neighbors<Real, false><<<blocks_for(a.N, 256), 256, 0, s>>>(
+      a, pos_s, type_s, sorted_key, cell_begin, cell_end, count, nl, max_count);
 }
 
 template void launch_neighbors<float>(const NeighborArgs&, const double*, const int*, const int*,
-                                      const int*, int*, Edge<float>*, int*, hipStream_t);
+                                      const int*, const int*, int*, Edge<float>*, int*,
+                                      hipStream_t);
 template void launch_neighbors<double>(const NeighborArgs&, const double*, const int*, const int*,
-                                       const int*, int*, Edge<double>*, int*, hipStream_t);
+                                       const int*, const int*, int*, Edge<double>*, int*,
+                                       hipStream_t);
 
 size_t radix_sort_temp_bytes(int N, int bits)
 {

@@ -36,7 +36,7 @@ def model_check(path):
     return dict(zip(keys, list(dims)))
 
 
-def lane_check(path, basics=None, max_slots=160):
+def lane_check(path, basics=None, max_slots=160, waves=1):
     """Host-only run of the lane program (the one-atom-per-lane graph kernel's instruction
     stream) for basic vectors basics [n, B].  Returns (info dict, e_prime [n], grad [n, B]);
     e_prime / grad are None when the model has no lane program."""
@@ -46,14 +46,14 @@ def lane_check(path, basics=None, max_slots=160):
     P = ctypes.POINTER(ctypes.c_double)
     info = (ctypes.c_int * 6)()
     if basics is None:
-        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), 0, None, None, None,
-                                      info))
+        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), int(waves), 0, None,
+                                      None, None, info))
         e = g = None
     else:
         b = np.ascontiguousarray(np.atleast_2d(basics), dtype=np.float64)
         e = np.zeros(b.shape[0])
         g = np.zeros_like(b)
-        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), b.shape[0],
+        check(lib().sus2sh_lane_check(str(path).encode(), int(max_slots), int(waves), b.shape[0],
                                       b.ctypes.data_as(P), e.ctypes.data_as(P),
                                       g.ctypes.data_as(P), info))
     keys = ("ok", "slots", "instructions", "components", "batches", "largest_component")

@@ -90,12 +90,13 @@ class SUS2_SH:
 
     def tuning(self):
         """Autotuner graph-kernel times {(variant, tile): ms} (variant 0 rows, 1 balanced
-        streams, 2 round-robin streams, 3 lane program with tile 64 atoms)."""
-        buf = (ctypes.c_float * 16)()
-        n = lib().sus2sh_get_tuning(self._h, buf, 16)
+        streams, 2 round-robin streams; variant 3 is the lane program, keyed (3, shape))."""
+        buf = (ctypes.c_float * 32)()
+        n = lib().sus2sh_get_tuning(self._h, buf, 32)
         out = {(k // 5, 1 << (k % 5)): buf[k] for k in range(min(n, 15)) if buf[k] >= 0}
-        if n > 15 and buf[15] >= 0:
-            out[(3, 64)] = buf[15]
+        for k in range(15, n):
+            if buf[k] >= 0:
+                out[(3, k - 15)] = buf[k]
         return out
 
     def mean_neighbors(self):

@@ -202,14 +202,18 @@ def test_parameter_gradient_matches_oracle(make_model, fp32):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_lane_program_matches_oracle(make_model, name, fp32):
     """Graph variant 3 (one atom per lane, wave-uniform program) on every case; the
-    body-order-6 graph is three layers deep and has no lane program: rejected cleanly."""
+    body-order-6 graph is three layers deep, and a graph whose largest component does not
+    fit the LDS of a 64-atom workgroup (l4k4 in FP64) has no lane program: both are
+    rejected cleanly."""
     case = CASES[name]
     path = make_model(name, **case["model"])
     S = case["model"].get("species_count", 2)
     pos, types, cell = _structure(case["struct"], S)
     opts = [f"T{s}" for s in range(S)] + ["sus2_float=%d" % (1 if fp32 else 0),
                                           "sus2_sh_graph_variant=3"]
-    if case["model"]["body_order"] > 5:
+    # largest per-lane slot budget whose 64-atom workgroup fits 160 KiB of LDS
+    feasible = sus2sh.lane_check(path, max_slots=640 if fp32 else 320, waves=4)[0]["ok"]
+    if case["model"]["body_order"] > 5 or not feasible:
         with pytest.raises(sus2sh.Sus2shError) as err:
             _run(path, cell, types, pos, options=opts)
         assert "no lane program" in err.value.message
@@ -217,6 +221,20 @@ def test_lane_program_matches_oracle(make_model, name, fp32):
     ref = Oracle(path).compute(cell, types, pos)
     got = _run(path, cell, types, pos, options=opts)
     _compare(ref, got, fp32)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_lane_program_wave_splits(make_model, waves):
+    """The lane kernel with 1, 2 and 4 waves per 64-atom tile (rows of every segment split
+    between the waves) gives the oracle's result in both precisions; N = 500 is not a
+    multiple of 64."""
+    path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])
+    pos, types, cell = _structure(CASES["l3k3_fcc"]["struct"], 2)
+    ref = Oracle(path).compute(cell, types, pos)
+    for fp32 in (0, 1):
+        opts = ["Cu", "Zr", f"sus2_float={fp32}", "sus2_sh_graph_variant=3",
+                f"sus2_sh_lane_waves={waves}"]
+        _compare(ref, _run(path, cell, types, pos, options=opts), bool(fp32))
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2])

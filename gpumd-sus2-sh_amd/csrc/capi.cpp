@@ -309,7 +309,7 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
                       const double* basics, double* e_prime, double* grad, int* info)
 {
   if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
-      (waves != 1 && waves != 2 && waves != 4))
+      (waves != 1 && waves != 2 && waves != 4 && waves != 8))
     return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);

@@ -14,14 +14,25 @@ namespace sus2sh {
 
 namespace {
 
+// Pair strides of the LDS radial tables: padded to 1 mod 32 words so that lanes reading the
+// same coefficient of different pair tables hit different banks.
+__host__ __device__ constexpr int padded_stride(int n) { return n + ((1 - n % 32) + 32) % 32; }
+
 template <typename Real>
 __device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, Real* s_coeff,
                                              Real* s_scal)
 {
-  const int nc = rt.S * rt.S * Mr * rt.rb;
-  const int ns = rt.S * rt.S * Mr * 2;
-  for (int i = threadIdx.x; i < nc; i += blockDim.x) s_coeff[i] = rt.coeff[i];
-  for (int i = threadIdx.x; i < ns; i += blockDim.x) s_scal[i] = rt.scal[i];
+  const int pairs = rt.S * rt.S;
+  const int nc = Mr * rt.rb, ns = Mr * 2;
+  const int cs = padded_stride(nc), ss = padded_stride(ns);
+  for (int i = threadIdx.x; i < pairs * nc; i += blockDim.x) {
+    const int p = i / nc;
+    s_coeff[p * cs + (i - p * nc)] = rt.coeff[i];
+  }
+  for (int i = threadIdx.x; i < pairs * ns; i += blockDim.x) {
+    const int p = i / ns;
+    s_scal[p * ss + (i - p * ns)] = rt.scal[i];
+  }
   __syncthreads();
 }
 
@@ -33,7 +44,8 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   constexpr int FG = K * NC;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
-  Real* s_scal = s_coeff + a.rt.S * a.rt.S * MR * a.rt.rb;
+  const int cs = padded_stride(MR * a.rt.rb), ss = padded_stride(MR * 2);
+  Real* s_scal = s_coeff + a.rt.S * a.rt.S * cs;
   stage_tables(a.rt, MR, s_coeff, s_scal);
 
   const int N = a.N;
@@ -57,7 +69,7 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
     const Real inv = Real(1) / r;
     const int pair = ti * S + (e.j >> kEdgeTypeShift);
     Real R[MR], dR[MR];
-    eval_radial<Real, MR, false>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+    eval_radial<Real, MR, false>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                  a.rt.kind, a.rt.uniform, r, R, dR);
     Real Sv[NC];
     solid_values<Real, L>(e.dx * inv, e.dy * inv, e.dz * inv, Sv);
@@ -142,9 +154,10 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   const int S = a.rt.S;
   const int rb = a.rt.rb;
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
-  Real* s_scal = s_coeff + S * S * MR * rb;
+  const int cs = padded_stride(MR * rb), ss = padded_stride(MR * 2);
+  Real* s_scal = s_coeff + S * S * cs;
   // PG: per-pair adjoint accumulators [S*S][MR*rb + 2*MR] (double), 16-byte aligned
-  const size_t pg_off = ((static_cast<size_t>(S) * S * MR * (rb + 2) * sizeof(Real)) + 15) & ~size_t(15);
+  const size_t pg_off = ((static_cast<size_t>(S) * S * (cs + ss) * sizeof(Real)) + 15) & ~size_t(15);
   double* s_pg = reinterpret_cast<double*>(smem_raw + pg_off);
   const int pg_stride = MR * rb + 2 * MR;
   if (PG) {
@@ -178,7 +191,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     const int pair = ti * S + (e.j >> kEdgeTypeShift);
     const int jn = e.j & kEdgeIndexMask;
     Real R[MR], dR[MR];
-    eval_radial<Real, MR, true>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+    eval_radial<Real, MR, true>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                 a.rt.kind, a.rt.uniform, r, R, dR);
     Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
     solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
@@ -218,7 +231,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
         }
       }
       double* Dp = s_pg + pair * pg_stride;
-      radial_param_adjoint<Real, MR>(s_coeff + pair * MR * rb, s_scal + pair * MR * 2, rb, rc,
+      radial_param_adjoint<Real, MR>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                      a.rt.kind, r, A, Dp, Dp + MR * rb, Dp + MR * rb + MR);
     }
     const Real g0 = ux * sv + gx * inv;
@@ -260,7 +273,8 @@ template <typename Real>
 size_t table_bytes(const EdgeLaunch<Real>& a)
 {
   const int Mr = a.K * (a.L + 1);
-  return static_cast<size_t>(a.rt.S) * a.rt.S * Mr * (a.rt.rb + 2) * sizeof(Real);
+  return static_cast<size_t>(a.rt.S) * a.rt.S *
+         (padded_stride(Mr * a.rt.rb) + padded_stride(Mr * 2)) * sizeof(Real);
 }
 
 #define SUS2SH_FOR_EACH_LK(X) \

@@ -53,7 +53,8 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.graph_variant = v;
     } else if (key == "sus2_sh_lane_waves") {
       const int w = std::atoi(val.c_str());
-      if (w != 1 && w != 2 && w != 4) throw InputError("sus2_sh_lane_waves must be 1, 2 or 4: " + val);
+      if (w != 1 && w != 2 && w != 4 && w != 8)
+        throw InputError("sus2_sh_lane_waves must be 1, 2, 4 or 8: " + val);
       o.lane_waves = w;  // fixes the lane program shape (4 accumulators per wave)
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
@@ -113,8 +114,12 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
             "hipDeviceGetAttribute");
 
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  // LDS radial tables of the edge kernels: per pair, coefficients and (a, s), each padded to
+  // 1 mod 32 words (edge_kernels.hip)
+  const auto padded = [](size_t n) { return n + ((1 + 32 - n % 32) % 32); };
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
-                       model_.radial_funcs_count * (model_.rb_size + 2) * rsz;
+                       (padded(static_cast<size_t>(model_.radial_funcs_count) * model_.rb_size) +
+                        padded(static_cast<size_t>(model_.radial_funcs_count) * 2)) * rsz;
   if (table > 65536)
     throw UnsupportedError("SUS2_SH: radial tables exceed 64 KiB of LDS (too many species).");
 
@@ -193,14 +198,14 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
 const std::vector<Engine::LaneShape>& Engine::lane_shapes()
 {
   static const std::vector<LaneShape> shapes = {
-    {4, 4, 160}, {2, 4, 160}, {4, 4, 192}, {4, 4, 256}, {1, 8, 160}};
+    {4, 4, 160}, {8, 2, 160}, {4, 2, 160}, {8, 2, 192}, {8, 2, 256}, {2, 4, 160}};
   return shapes;
 }
 
 void Engine::compile_lane()
 {
   if (opt_.lane_waves > 0) {
-    lane_ = build_lane(LaneShape{opt_.lane_waves, 4, 160});
+    lane_ = build_lane(LaneShape{opt_.lane_waves, opt_.lane_waves == 8 ? 2 : 4, 160});
     return;
   }
   lane_ = LaneProgram{};
@@ -212,18 +217,14 @@ void Engine::compile_lane()
 
 void Engine::upload_lane()
 {
+  const LaneRecords rec =
+    encode_lane_records(lane_, opt_.use_float ? static_cast<int>(sizeof(float)) : static_cast<int>(sizeof(double)));
   const size_t nseg = lane_.seg_kind.size();
   std::vector<int4> segs(nseg);
   for (size_t k = 0; k < nseg; ++k)
-    segs[k] = make_int4(lane_.seg_kind[k], lane_.seg_begin[k], lane_.seg_groups[k], 0);
+    segs[k] = make_int4(lane_.seg_kind[k], rec.seg_first[k], lane_.seg_groups[k], 0);
   lane_segs_.upload(segs.data(), segs.size() * sizeof(int4));
-  lane_ins_.upload(lane_.ins.data(), lane_.ins.size() * sizeof(uint32_t));
-  if (opt_.use_float) {
-    std::vector<float> lc(lane_.coeff.begin(), lane_.coeff.end());
-    lane_coeff_.upload(lc.data(), lc.size() * sizeof(float));
-  } else {
-    lane_coeff_.upload(lane_.coeff.data(), lane_.coeff.size() * sizeof(double));
-  }
+  lane_ins_.upload(rec.words.data(), rec.words.size() * sizeof(uint32_t));
 }
 
 template <typename Real>
@@ -429,8 +430,7 @@ bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable
     if (!lane_.ok) return false;
     LaneTable<Real> lt{};
     lt.segs = lane_segs_.as<int4>();
-    lt.ins = lane_ins_.as<uint32_t>();
-    lt.coeff = lane_coeff_.as<Real>();
+    lt.recs = lane_ins_.as<uint32_t>();
     lt.shift = shift_.as<double>();
     lt.sigma = sigma_.as<double>();
     lt.nseg = static_cast<int>(lane_.seg_kind.size());

@@ -144,7 +144,7 @@ class Engine {
   DeviceBuffer fwd_ins_[2], fwd_sbegin_[2], fwd_slen_[2], bwd_ins_[2], bwd_sbegin_[2],
     bwd_slen_[2];
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
-  DeviceBuffer lane_segs_, lane_ins_, lane_coeff_;
+  DeviceBuffer lane_segs_, lane_ins_;
   int cu_count_ = 256;
   bool radial_uniform_ = false;
 

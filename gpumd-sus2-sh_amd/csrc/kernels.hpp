@@ -125,9 +125,8 @@ size_t graph_lds_bytes(int tile, int slots);
 // Lane program tables (see lane_program.hpp): one atom per lane, wave-uniform program.
 template <typename Real>
 struct LaneTable {
-  const int4* segs;        // (kind, first instruction, groups, 0)
-  const uint32_t* ins;     // x | y << 10 | t << 20 | end << 30
-  const Real* coeff;       // one per instruction
+  const int4* segs;        // (kind, first record, groups per wave, 0)
+  const uint32_t* recs;    // one record per group (encode_lane_records)
   const double* shift;     // [S]
   const double* sigma;     // [S]
   int nseg;

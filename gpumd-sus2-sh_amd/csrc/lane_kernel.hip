@@ -7,10 +7,11 @@
 // bank conflicts).  The WPT waves of the workgroup serve the same 64 atoms: the program
 // splits every segment's independent rows between them, so a wave runs one WPT-th of the
 // instructions and the CU holds WPT times more waves for the same LDS.  A barrier separates
-// segments (a segment reads what earlier segments wrote).  Instruction words and
-// coefficients are uniform across a wave and are fetched with scalar loads; each group of NS
-// instructions feeds NS independent accumulators, so a group issues its NS (or 2 NS) LDS
-// reads before the first FMA that needs them.
+// segments (a segment reads what earlier segments wrote).  A group of NS instructions is one
+// pre-decoded record (LDS word offsets, one row-end mask word, coefficients) that is uniform
+// across the wave and fetched with scalar loads, so a group costs no scalar decoding; its NS
+// accumulators are independent, so the group issues its NS (or 2 NS) LDS reads before the
+// first FMA that needs them, and the next record is fetched while they are in flight.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -21,6 +22,70 @@ namespace {
 
 constexpr int kLaneWave = 64;
 
+// Read-only program data is read through the constant address space: such loads stay scalar
+// across the segment barriers (a barrier is a possible clobber of global memory).
+using cu32 = const __attribute__((address_space(4))) uint32_t;
+
+// One segment of one wave: `groups` records starting at R.  The record of group g + 1 is
+// loaded while group g's LDS reads are in flight, so one wait covers both.
+template <typename Real, int NS, bool BINARY>
+__device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
+{
+  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
+  constexpr int RW = HW + NS * static_cast<int>(sizeof(Real)) / 4;
+  using creal = const __attribute__((address_space(4))) Real;
+  Real acc[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) acc[j] = Real(0);
+  uint32_t xo[NS], yo[NS], to[NS], mask;
+  Real c[NS];
+  auto fetch = [&](cu32* r) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      xo[j] = r[j];
+      if (BINARY) yo[j] = r[NS + j];
+      to[j] = r[2 * NS + j];
+      c[j] = reinterpret_cast<creal*>(r + HW)[j];
+    }
+    mask = r[3 * NS];
+  };
+  fetch(R);
+  for (int g = 0; g < groups; ++g) {
+    Real xv[NS], yv[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      xv[j] = X[xo[j]];
+      if (BINARY) yv[j] = X[yo[j]];
+    }
+    Real cc[NS];
+    uint32_t tt[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      cc[j] = c[j];
+      tt[j] = to[j];
+    }
+    const uint32_t mm = mask;
+    if (g + 1 < groups) fetch(R + (g + 1) * RW);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] += BINARY ? (cc[j] * xv[j]) * yv[j] : cc[j] * xv[j];
+    if (mm != 0u) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        if (mm & (1u << j)) {
+          const uint32_t action = (mm >> (8 + 2 * j)) & 3u;
+          if (action == 1u)
+            X[tt[j]] = acc[j];
+          else if (action == 2u)
+            X[tt[j]] += acc[j];
+          else
+            e += acc[j];
+          acc[j] = Real(0);
+        }
+      }
+    }
+  }
+}
+
 template <typename Real, int NS, int WPT>
 __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Real* __restrict__ basics,
                                                                const int* __restrict__ type_s,
@@ -28,9 +93,11 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
                                                                double* __restrict__ e_out,
                                                                Real* __restrict__ grad_out)
 {
+  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
+  constexpr int RW = HW + NS * static_cast<int>(sizeof(Real)) / 4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const unsigned lane = threadIdx.x & (kLaneWave - 1);
-  const int wave = static_cast<int>(threadIdx.x / kLaneWave);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLaneWave));
   Real* X = reinterpret_cast<Real*>(smem_raw) + lane;  // slot s of this lane: X[s * 64]
   const unsigned atom = blockIdx.x * static_cast<unsigned>(kLaneWave) + lane;
   const bool active = atom < static_cast<unsigned>(N);
@@ -45,48 +112,13 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
   Real e = 0;
   for (int s = 0; s < lt.nseg; ++s) {
     __syncthreads();  // every wave runs every segment: uniform barrier
-    const int4 seg = lt.segs[s];
-    const size_t first = static_cast<size_t>(seg.y) + static_cast<size_t>(wave) * seg.z * NS;
-    const uint32_t* __restrict__ I = lt.ins + first;
-    const Real* __restrict__ C = lt.coeff + first;
-    Real acc[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) acc[j] = Real(0);
-    const bool binary = seg.x == 0;
-    for (int g = 0; g < seg.z; ++g, I += NS, C += NS) {
-      uint32_t w[NS];
-      Real c[NS], xv[NS], yv[NS];
-#pragma unroll
-      for (int j = 0; j < NS; ++j) {
-        w[j] = I[j];
-        c[j] = C[j];
-      }
-#pragma unroll
-      for (int j = 0; j < NS; ++j) xv[j] = X[(w[j] & 1023u) * kLaneWave];
-      if (binary) {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) yv[j] = X[((w[j] >> 10) & 1023u) * kLaneWave];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) acc[j] += (c[j] * xv[j]) * yv[j];
-      } else {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) acc[j] += c[j] * xv[j];
-      }
-#pragma unroll
-      for (int j = 0; j < NS; ++j) {
-        const uint32_t end = w[j] >> 30;
-        if (end != 0u) {
-          const uint32_t t = (w[j] >> 20) & 1023u;
-          if (end == 1u)
-            X[t * kLaneWave] = acc[j];
-          else if (end == 2u)
-            X[t * kLaneWave] += acc[j];
-          else
-            e += acc[j];
-          acc[j] = Real(0);
-        }
-      }
-    }
+    cu32* sg = (cu32*)lt.segs + 4 * s;  // (kind, first record, groups, 0)
+    const uint32_t kind = sg[0], first = sg[1], groups = sg[2];
+    cu32* R = (cu32*)lt.recs + (static_cast<size_t>(first) + static_cast<size_t>(wave) * groups) * RW;
+    if (kind == 0)
+      run_segment<Real, NS, true>(R, static_cast<int>(groups), X, e);
+    else
+      run_segment<Real, NS, false>(R, static_cast<int>(groups), X, e);
   }
   __syncthreads();
 
@@ -136,12 +168,12 @@ bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneT
     launch_lane_shape<Real, NS, WPT>(N, basics, type_s, lt, e_s, grads, s, lds);     \
     return true;                                                                     \
   }
+  SUS2SH_LANE_CASE(2, 4)
+  SUS2SH_LANE_CASE(2, 8)
   SUS2SH_LANE_CASE(4, 1)
   SUS2SH_LANE_CASE(4, 2)
   SUS2SH_LANE_CASE(4, 4)
   SUS2SH_LANE_CASE(8, 1)
-  SUS2SH_LANE_CASE(8, 2)
-  SUS2SH_LANE_CASE(8, 4)
 #undef SUS2SH_LANE_CASE
   return false;
 }

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace sus2sh {
@@ -355,16 +356,17 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
 
   // self-check against the flat graph on two seeded basic vectors
   uint64_t seed = 0x5eed5eedULL;
-  std::vector<double> b(B), g1(B), g2(B);
+  std::vector<double> b(B), g1(B), g2(B), g3(B);
   for (int trial = 0; trial < 2; ++trial) {
     for (int i = 0; i < B; ++i) b[i] = (static_cast<double>(lcg(seed) % 2000001) / 1e6 - 1.0) * 0.5;
     p.ok = true;
     const double e1 = run_lane_program_host(p, b.data(), g1.data());
     const double e2 = run_flat_graph_host(m, b.data(), g2.data());
-    double scale = std::abs(e2), err = std::abs(e1 - e2);
+    const double e3 = run_lane_records_host(p, encode_lane_records(p, 8), b.data(), g3.data());
+    double scale = std::abs(e2), err = std::max(std::abs(e1 - e2), std::abs(e3 - e2));
     for (int i = 0; i < B; ++i) {
       scale = std::max(scale, std::abs(g2[i]));
-      err = std::max(err, std::abs(g1[i] - g2[i]));
+      err = std::max(err, std::max(std::abs(g1[i] - g2[i]), std::abs(g3[i] - g2[i])));
     }
     if (!(err <= 1e-9 * std::max(1.0, scale))) {
       p.ok = false;
@@ -408,6 +410,92 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
           if (end == kLaneEndAdd) X[t] += acc[j];
           if (end == kLaneEndEnergy) e += acc[j];
           if (end != kLaneEndNone) acc[j] = 0.0;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < B; ++i) grad_b[i] = X[B + i];
+  return e;
+}
+
+LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
+{
+  LaneRecords r;
+  r.nslot = p.nslot;
+  r.real_bytes = real_bytes;
+  const int ns = p.nslot, hw = r.header_words(), rw = r.rec_words();
+  const uint32_t col = 64u;  // words per slot row: slot s of lane i is word s * 64 + i
+  size_t nrec = 0;
+  for (size_t s = 0; s < p.seg_kind.size(); ++s) nrec += static_cast<size_t>(p.seg_groups[s]) * p.waves;
+  r.words.assign(nrec * rw, 0u);
+  size_t rec = 0;
+  for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+    r.seg_first.push_back(static_cast<int>(rec));
+    const size_t groups = static_cast<size_t>(p.seg_groups[s]) * p.waves;
+    for (size_t q = 0; q < groups; ++q, ++rec) {
+      uint32_t* R = r.words.data() + rec * rw;
+      const size_t base = static_cast<size_t>(p.seg_begin[s]) + q * ns;
+      uint32_t mask = 0;
+      for (int j = 0; j < ns; ++j) {
+        const uint32_t w = p.ins[base + j];
+        const uint32_t end = w >> 30;
+        R[j] = (w & 1023u) * col;
+        R[ns + j] = ((w >> 10) & 1023u) * col;
+        R[2 * ns + j] = ((w >> 20) & 1023u) * col;
+        if (end != kLaneEndNone) mask |= (1u << j) | (end << (8 + 2 * j));
+        if (real_bytes == 4) {
+          const float c = static_cast<float>(p.coeff[base + j]);
+          std::memcpy(R + hw + j, &c, 4);
+        } else {
+          const double c = p.coeff[base + j];
+          std::memcpy(R + hw + 2 * j, &c, 8);
+        }
+      }
+      R[3 * ns] = mask;
+    }
+  }
+  return r;
+}
+
+double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const double* basics,
+                             double* grad_b)
+{
+  const int B = p.basic_count, ns = p.nslot, hw = r.header_words(), rw = r.rec_words();
+  const uint32_t col = 64u;
+  std::vector<double> X(p.slots, 0.0);
+  for (int i = 0; i < B; ++i) X[i] = basics[i];
+  X[p.ones_slot] = 1.0;
+  double e = 0.0;
+  std::vector<double> acc(ns), xv(ns), yv(ns);
+  for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+    const bool bin = p.seg_kind[s] == 0;
+    const std::vector<double> X0 = X;  // waves of a segment read what earlier segments wrote
+    for (int w = 0; w < p.waves; ++w) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      for (int g = 0; g < p.seg_groups[s]; ++g) {
+        const uint32_t* R = r.words.data() +
+                            (static_cast<size_t>(r.seg_first[s]) + static_cast<size_t>(w) * p.seg_groups[s] + g) * rw;
+        for (int j = 0; j < ns; ++j) {
+          double c;
+          if (r.real_bytes == 4) {
+            float f;
+            std::memcpy(&f, R + hw + j, 4);
+            c = f;
+          } else {
+            std::memcpy(&c, R + hw + 2 * j, 8);
+          }
+          xv[j] = X0[R[j] / col];
+          yv[j] = bin ? X0[R[ns + j] / col] : 1.0;
+          acc[j] += c * xv[j] * yv[j];
+        }
+        const uint32_t mask = R[3 * ns];
+        for (int j = 0; j < ns; ++j) {
+          if (!(mask & (1u << j))) continue;
+          const uint32_t action = (mask >> (8 + 2 * j)) & 3u, t = R[2 * ns + j] / col;
+          if (action == kLaneEndStore) X[t] = acc[j];
+          if (action == kLaneEndAdd) X[t] += acc[j];
+          if (action == kLaneEndEnergy) e += acc[j];
+          acc[j] = 0.0;
         }
       }
     }

@@ -66,6 +66,27 @@ LaneProgram compile_lane_program(const Model& model, int max_slots, int nslot = 
 // returns E' and writes dE'/db into grad_b (B values).
 double run_lane_program_host(const LaneProgram& p, const double* basics, double* grad_b);
 
+// Device encoding of a lane program: one record of rec_words() 32-bit words per group of
+// nslot instructions, wave-major inside a segment (record (w * groups + g) of the segment).
+//   words [0, nslot)          x word offsets in the lane column layout (slot * 64)
+//   words [nslot, 2 nslot)    y word offsets
+//   words [2 nslot, 3 nslot)  row-end target word offsets
+//   word  3 nslot             row-end mask: bit j = instruction j ends its row, bits
+//                             8 + 2j .. 9 + 2j = its action (1 store, 2 add, 3 energy)
+//   words [hdr, hdr + nslot * real_bytes / 4)  coefficients (Real), hdr = header_words()
+struct LaneRecords {
+  int nslot = 0, real_bytes = 0;
+  int header_words() const { return (3 * nslot + 1 + 3) / 4 * 4; }
+  int rec_words() const { return header_words() + nslot * real_bytes / 4; }
+  std::vector<uint32_t> words;
+  std::vector<int> seg_first;  // first record of each segment
+};
+LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes);
+
+// Host execution of the device encoding (same arithmetic order as the kernel, in double).
+double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const double* basics,
+                             double* grad_b);
+
 // Host execution of the flat graph for one atom (same outputs), the reference the lane
 // program is checked against when it is compiled.
 double run_flat_graph_host(const Model& model, const double* basics, double* grad_b);

@@ -72,7 +72,7 @@ int sus2sh_probe(const char* path);
  * src/force/force.cu:114).  `opts` are the run.in tokens after the model file: the
  * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
  * Honoured options: sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
- * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4>,
+ * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8>,
  * sus2_sh_autotune=<0|1>,
  * sus2_sh_profile=<0|1>; other sus2_* tokens are accepted and ignored.  Uses the current HIP device. */
 int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
@@ -151,8 +151,8 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
 
 /* Autotuner measurements of the graph kernel (ms per launch, -1 when not tried):
  * ms[v * 5 + log2(tile)] for variant v = 0..2 and tile 1..16, and ms[15 + c] for lane-program
- * (variant 3) shape c = 0..4: (waves per 64-atom tile, accumulators per wave, minimum
- * per-lane slots) = (4,4,160), (2,4,160), (4,4,192), (4,4,256), (1,8,160).
+ * (variant 3) shape c = 0..5: (waves per 64-atom tile, accumulators per wave, minimum
+ * per-lane slots) = (4,4,160), (8,2,160), (4,2,160), (8,2,192), (8,2,256), (2,4,160).
  * Returns the count written. */
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
@@ -191,7 +191,7 @@ int sus2sh_model_check(const char* mtp_path, int* dims);
 
 /* Host-only check of the lane program (variant 3 of the graph kernel; no reference
  * counterpart): compile it for the model with at most max_slots per-lane slots and `waves`
- * (1, 2 or 4) waves per 64-atom tile, and run it on the host for n basic vectors
+ * (1, 2, 4 or 8) waves per 64-atom tile, and run it on the host for n basic vectors
  * basics[n][B] -> e_prime[n] = sum_s w_s M[map[s]] and grad[n][B] = de_prime/dbasic (without
  * the species factor).  info[0..6) = ok, slots, instructions per atom, graph components,
  * batches, largest component.  When the model has no lane program (info[0] = 0) nothing is

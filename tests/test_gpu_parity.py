@@ -223,9 +223,9 @@ def test_lane_program_matches_oracle(make_model, name, fp32):
     _compare(ref, got, fp32)
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
 def test_lane_program_wave_splits(make_model, waves):
-    """The lane kernel with 1, 2 and 4 waves per 64-atom tile (rows of every segment split
+    """The lane kernel with 1, 2, 4 and 8 waves per 64-atom tile (rows of every segment split
     between the waves) gives the oracle's result in both precisions; N = 500 is not a
     multiple of 64."""
     path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])

@@ -31,14 +31,14 @@ STANDARD = [
 
 
 def _check(path, max_slots=1024, n=5, seed=0, expect_ok=True):
-    """Every wave split (1, 2, 4 waves per tile) of the program against the oracle."""
+    """Every wave split (1, 2, 4, 8 waves per tile) of the program against the oracle."""
     orc = Oracle(path)
     B = orc.dims["B"]
     rng = np.random.default_rng(seed)
     basics = rng.normal(0.0, 0.4, size=(n, B))
     e0, g0 = orc.graph(basics)
     scale = max(1.0, np.abs(e0).max(), np.abs(g0).max())
-    for waves in (1, 2, 4):
+    for waves in (1, 2, 4, 8):
         info, e, g = sus2sh.lane_check(path, basics, max_slots=max_slots, waves=waves)
         print(f"[lane] {path} waves={waves}: {info}", flush=True)
         if not expect_ok:

@@ -313,7 +313,8 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
     return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);
-    const sus2sh::LaneProgram p = sus2sh::compile_lane_program(m, max_slots, 8, waves);
+    const sus2sh::LaneProgram p =
+      sus2sh::compile_lane_program(m, max_slots, waves == 8 ? 2 : 4, waves);
     if (info) {
       info[0] = p.ok ? 1 : 0;
       info[1] = p.slots;

@@ -8,10 +8,12 @@
 // splits every segment's independent rows between them, so a wave runs one WPT-th of the
 // instructions and the CU holds WPT times more waves for the same LDS.  A barrier separates
 // segments (a segment reads what earlier segments wrote).  A group of NS instructions is one
-// pre-decoded record (LDS word offsets, one row-end mask word, coefficients) that is uniform
-// across the wave and fetched with scalar loads, so a group costs no scalar decoding; its NS
-// accumulators are independent, so the group issues its NS (or 2 NS) LDS reads before the
-// first FMA that needs them, and the next record is fetched while they are in flight.
+// pre-decoded record (LDS word offsets, row-end mask, keep factors, coefficients) that is
+// uniform across the wave and fetched with scalar loads, so a group costs no scalar decoding;
+// its NS accumulators are independent, so the group issues its NS (or 2 NS) LDS reads before
+// the first FMA that needs them, and the next record is fetched while they are in flight.
+// Segment kinds: 0 binary rows stored, 1 unary rows stored, 2 binary rows summed into the
+// site energy, 3 binary rows added to their targets.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -26,63 +28,97 @@ constexpr int kLaneWave = 64;
 // across the segment barriers (a barrier is a possible clobber of global memory).
 using cu32 = const __attribute__((address_space(4))) uint32_t;
 
-// One segment of one wave: `groups` records starting at R.  The record of group g + 1 is
-// loaded while group g's LDS reads are in flight, so one wait covers both.
+// One group record in scalar registers.
 template <typename Real, int NS, bool BINARY>
-__device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
+struct LaneRec {
+  uint32_t xo[NS], yo[NS], to[NS], mask;
+  Real keep[NS], c[NS];
+};
+
+template <typename Real, int NS, bool BINARY>
+__device__ __forceinline__ void fetch_rec(cu32* r, LaneRec<Real, NS, BINARY>& q)
 {
   constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
-  constexpr int RW = HW + NS * static_cast<int>(sizeof(Real)) / 4;
   using creal = const __attribute__((address_space(4))) Real;
+  creal* f = reinterpret_cast<creal*>(r + HW);
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    q.xo[j] = r[j];
+    if (BINARY) q.yo[j] = r[NS + j];
+    q.to[j] = r[2 * NS + j];
+    q.keep[j] = f[j];
+    q.c[j] = f[NS + j];
+  }
+  q.mask = r[3 * NS];
+}
+
+// Issue the LDS reads of a group.
+template <typename Real, int NS, bool BINARY>
+__device__ __forceinline__ void read_rec(const LaneRec<Real, NS, BINARY>& q, const Real* X,
+                                         Real (&xv)[NS], Real (&yv)[NS])
+{
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    xv[j] = X[q.xo[j]];
+    if (BINARY) yv[j] = X[q.yo[j]];
+  }
+}
+
+// Accumulate a group and end its rows.  Store and energy segments are branch-free: every group
+// writes acc to its target word (the trash slot unless the row ends) and scales acc by keep.
+template <typename Real, int NS, int KIND>
+__device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q,
+                                           const Real (&xv)[NS], const Real (&yv)[NS],
+                                           Real (&acc)[NS], Real* X, Real& e)
+{
+#pragma unroll
+  for (int j = 0; j < NS; ++j) acc[j] += KIND != 1 ? (q.c[j] * xv[j]) * yv[j] : q.c[j] * xv[j];
+  if (KIND == 0 || KIND == 1) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      X[q.to[j]] = acc[j];
+      acc[j] *= q.keep[j];
+    }
+  } else if (KIND == 2) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      e += acc[j] - acc[j] * q.keep[j];
+      acc[j] *= q.keep[j];
+    }
+  } else if (q.mask != 0u) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      if (q.mask & (1u << j)) {
+        X[q.to[j]] += acc[j];
+        acc[j] = Real(0);
+      }
+    }
+  }
+}
+
+// One segment of one wave: `groups` records from R.  Two record sets alternate, so the
+// record of group g + 1 is fetched while group g's LDS reads are in flight and nothing is
+// copied between iterations.
+template <typename Real, int NS, int KIND>
+__device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
+{
+  constexpr bool BIN = KIND != 1;
+  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
+  constexpr int RW = HW + 2 * NS * static_cast<int>(sizeof(Real)) / 4;
   Real acc[NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = Real(0);
-  uint32_t xo[NS], yo[NS], to[NS], mask;
-  Real c[NS];
-  auto fetch = [&](cu32* r) {
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      xo[j] = r[j];
-      if (BINARY) yo[j] = r[NS + j];
-      to[j] = r[2 * NS + j];
-      c[j] = reinterpret_cast<creal*>(r + HW)[j];
-    }
-    mask = r[3 * NS];
-  };
-  fetch(R);
-  for (int g = 0; g < groups; ++g) {
-    Real xv[NS], yv[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      xv[j] = X[xo[j]];
-      if (BINARY) yv[j] = X[yo[j]];
-    }
-    Real cc[NS];
-    uint32_t tt[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      cc[j] = c[j];
-      tt[j] = to[j];
-    }
-    const uint32_t mm = mask;
-    if (g + 1 < groups) fetch(R + (g + 1) * RW);
-#pragma unroll
-    for (int j = 0; j < NS; ++j) acc[j] += BINARY ? (cc[j] * xv[j]) * yv[j] : cc[j] * xv[j];
-    if (mm != 0u) {
-#pragma unroll
-      for (int j = 0; j < NS; ++j) {
-        if (mm & (1u << j)) {
-          const uint32_t action = (mm >> (8 + 2 * j)) & 3u;
-          if (action == 1u)
-            X[tt[j]] = acc[j];
-          else if (action == 2u)
-            X[tt[j]] += acc[j];
-          else
-            e += acc[j];
-          acc[j] = Real(0);
-        }
-      }
-    }
+  LaneRec<Real, NS, BIN> a, b;
+  Real xv[NS], yv[NS];
+  fetch_rec<Real, NS, BIN>(R, a);
+  for (int g = 0; g < groups; g += 2) {
+    read_rec<Real, NS, BIN>(a, X, xv, yv);
+    if (g + 1 < groups) fetch_rec<Real, NS, BIN>(R + (g + 1) * RW, b);
+    finish_rec<Real, NS, KIND>(a, xv, yv, acc, X, e);
+    if (g + 1 >= groups) break;
+    read_rec<Real, NS, BIN>(b, X, xv, yv);
+    if (g + 2 < groups) fetch_rec<Real, NS, BIN>(R + (g + 2) * RW, a);
+    finish_rec<Real, NS, KIND>(b, xv, yv, acc, X, e);
   }
 }
 
@@ -94,7 +130,7 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
                                                                Real* __restrict__ grad_out)
 {
   constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
-  constexpr int RW = HW + NS * static_cast<int>(sizeof(Real)) / 4;
+  constexpr int RW = HW + 2 * NS * static_cast<int>(sizeof(Real)) / 4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const unsigned lane = threadIdx.x & (kLaneWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLaneWave));
@@ -115,10 +151,12 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
     cu32* sg = (cu32*)lt.segs + 4 * s;  // (kind, first record, groups, 0)
     const uint32_t kind = sg[0], first = sg[1], groups = sg[2];
     cu32* R = (cu32*)lt.recs + (static_cast<size_t>(first) + static_cast<size_t>(wave) * groups) * RW;
-    if (kind == 0)
-      run_segment<Real, NS, true>(R, static_cast<int>(groups), X, e);
-    else
-      run_segment<Real, NS, false>(R, static_cast<int>(groups), X, e);
+    switch (kind) {
+      case 0: run_segment<Real, NS, 0>(R, static_cast<int>(groups), X, e); break;
+      case 1: run_segment<Real, NS, 1>(R, static_cast<int>(groups), X, e); break;
+      case 2: run_segment<Real, NS, 2>(R, static_cast<int>(groups), X, e); break;
+      default: run_segment<Real, NS, 3>(R, static_cast<int>(groups), X, e); break;
+    }
   }
   __syncthreads();
 

@@ -36,6 +36,7 @@ int find_root(std::vector<int>& parent, int x)
 void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
 {
   const int ns = p.nslot, bins = p.nslot * p.waves;
+  const bool store = kind == kLaneBinaryStore || kind == kLaneUnaryStore;
   rows.erase(std::remove_if(rows.begin(), rows.end(), [](const Row& r) { return r.terms.empty(); }),
              rows.end());
   if (rows.empty()) return;
@@ -53,9 +54,18 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
   const size_t groups = *std::max_element(load.begin(), load.end());
   std::vector<std::vector<uint32_t>> words(bins);
   std::vector<std::vector<double>> cs(bins);
+  std::vector<std::vector<uint16_t>> wt(bins);
+  std::vector<std::vector<uint8_t>> kp(bins);
   for (int j = 0; j < bins; ++j) {
-    for (int r : per[j]) {
-      const Row& row = rows[r];
+    // store segments write the accumulator after every group: partial sums go to the row's
+    // own target (never read inside the segment), the last row of a bin keeps its sum so
+    // that the padding after it rewrites the same value; a bin with no row writes the trash
+    // slot
+    uint16_t last_target = static_cast<uint16_t>(p.trash_slot);
+    if (store && per[j].empty()) p.needs_trash = true;
+    for (size_t ri = 0; ri < per[j].size(); ++ri) {
+      const Row& row = rows[per[j][ri]];
+      const bool last_row = ri + 1 == per[j].size();
       for (size_t k = 0; k < row.terms.size(); ++k) {
         const Term& t = row.terms[k];
         const bool last = k + 1 == row.terms.size();
@@ -64,13 +74,18 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
         words[j].push_back(static_cast<uint32_t>(t.x) | (static_cast<uint32_t>(t.y) << 10) |
                            (tgt << 20) | (end << 30));
         cs[j].push_back(t.c);
+        wt[j].push_back(static_cast<uint16_t>(row.target));
+        kp[j].push_back(last && !(store && last_row) ? 0 : 1);
         ++p.useful_terms;
       }
+      last_target = static_cast<uint16_t>(row.target);
     }
     const uint32_t one = static_cast<uint32_t>(p.ones_slot);
     while (words[j].size() < groups) {
       words[j].push_back(one | (one << 10));
       cs[j].push_back(0.0);
+      wt[j].push_back(last_target);
+      kp[j].push_back(1);
     }
   }
   p.seg_kind.push_back(kind);
@@ -81,6 +96,8 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
       for (int j = 0; j < ns; ++j) {
         p.ins.push_back(words[w * ns + j][g]);
         p.coeff.push_back(cs[w * ns + j][g]);
+        p.write_slot.push_back(wt[w * ns + j][g]);
+        p.keep.push_back(kp[w * ns + j][g]);
       }
 }
 
@@ -92,7 +109,20 @@ uint64_t lcg(uint64_t& s)
 
 }  // namespace
 
+namespace {
+LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bool with_trash);
+}
+
 LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int waves)
+{
+  LaneProgram p = compile_impl(m, max_slots, nslot, waves, false);
+  if (!p.ok && p.needs_trash) p = compile_impl(m, max_slots, nslot, waves, true);
+  return p;
+}
+
+namespace {
+
+LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bool with_trash)
 {
   LaneProgram p;
   p.nslot = nslot;
@@ -104,12 +134,14 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
     return p;
   }
   max_slots = std::min(max_slots, kLaneMaxSlot + 1);
-  if (max_slots - 2 * B - 1 < 2) {
+  const int reserved = 2 * B + (with_trash ? 2 : 1);
+  if (max_slots - reserved < 2) {
     p.reason = "too many basic moments for the per-lane slot budget";
     return p;
   }
   p.ones_slot = 2 * B;
-  p.v_base = 2 * B + 1;
+  p.trash_slot = with_trash ? 2 * B + 1 : 2 * B;  // without a trash slot no bin may be empty
+  p.v_base = reserved;
 
   // direct weights d_n and the structural "mapped" flag
   std::vector<double> d(A, 0.0);
@@ -231,7 +263,7 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
       if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, p.ones_slot});
       fwd.push_back(std::move(r));
     }
-    emit_segment(p, 0, fwd);
+    emit_segment(p, kLaneBinaryStore, fwd);
     // ADJ: g_n = d_n + sum k_p v_other
     std::vector<Row> adj;
     for (int n : nodes) {
@@ -252,7 +284,7 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
       if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, 0});
       adj.push_back(std::move(r));
     }
-    emit_segment(p, 1, adj);
+    emit_segment(p, kLaneUnaryStore, adj);
     // ENG + BWD
     std::vector<Row> eb;
     {
@@ -271,6 +303,8 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
         eb.push_back(std::move(r));
       }
     }
+    emit_segment(p, kLaneBinaryEnergy, eb);
+    eb.clear();
     std::vector<std::vector<Term>> gb(B);
     for (int n : nodes) {
       if (n < B) {
@@ -295,7 +329,11 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
       r.terms = std::move(gb[q]);
       eb.push_back(std::move(r));
     }
-    emit_segment(p, 0, eb);
+    emit_segment(p, kLaneBinaryAdd, eb);
+  }
+  if (p.needs_trash && !with_trash) {
+    p.reason = "a store segment has an empty accumulator: needs a trash slot";
+    return p;
   }
   if (p.slots > kLaneMaxSlot + 1) {
     p.reason = "per-lane slot count exceeds the instruction encoding";
@@ -314,7 +352,7 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
     for (size_t k = b; k < e; ++k) {
       const uint32_t w = p.ins[k];
       const uint32_t end = w >> 30;
-      if ((w & 1023u) >= nslots || (p.seg_kind[s] == 0 && ((w >> 10) & 1023u) >= nslots) ||
+      if ((w & 1023u) >= nslots || (p.seg_kind[s] != kLaneUnaryStore && ((w >> 10) & 1023u) >= nslots) ||
           ((end == kLaneEndStore || end == kLaneEndAdd) && ((w >> 20) & 1023u) >= nslots)) {
         p.reason = "lane program addresses a slot outside the lane's column";
         return p;
@@ -332,6 +370,10 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
         const uint32_t w = p.ins[k], end = w >> 30;
         if (end == kLaneEndStore || end == kLaneEndAdd) {
           const int t = static_cast<int>((w >> 20) & 1023u);
+          if ((with_trash && t == p.trash_slot) || t == p.ones_slot || t < B) {
+            p.reason = "lane program writes a reserved slot";
+            return p;
+          }
           if (mark[t] == static_cast<int>(s)) {
             p.reason = "lane program writes one slot twice in a segment";
             return p;
@@ -342,7 +384,7 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
       for (size_t k = b; k < e; ++k) {
         const uint32_t w = p.ins[k];
         if (mark[w & 1023u] == static_cast<int>(s) ||
-            (p.seg_kind[s] == 0 && mark[(w >> 10) & 1023u] == static_cast<int>(s))) {
+            (p.seg_kind[s] != kLaneUnaryStore && mark[(w >> 10) & 1023u] == static_cast<int>(s))) {
           p.reason = "lane program reads a slot its segment writes";
           return p;
         }
@@ -378,6 +420,8 @@ LaneProgram compile_lane_program(const Model& m, int max_slots, int nslot, int w
   return p;
 }
 
+}  // namespace
+
 double run_lane_program_host(const LaneProgram& p, const double* basics, double* grad_b)
 {
   const int B = p.basic_count, ns = p.nslot;
@@ -387,7 +431,7 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
   double e = 0.0;
   std::vector<double> acc(ns), xv(ns), yv(ns);
   for (size_t s = 0; s < p.seg_kind.size(); ++s) {
-    const bool bin = p.seg_kind[s] == 0;
+    const bool bin = p.seg_kind[s] != kLaneUnaryStore;
     // waves run concurrently on the device: each wave's stream is replayed against a
     // snapshot of the slots taken at the start of the segment, and the writes of all waves
     // are applied afterwards (rows of one segment never read a slot the segment writes)
@@ -428,6 +472,15 @@ LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
   size_t nrec = 0;
   for (size_t s = 0; s < p.seg_kind.size(); ++s) nrec += static_cast<size_t>(p.seg_groups[s]) * p.waves;
   r.words.assign(nrec * rw, 0u);
+  auto put_real = [&](uint32_t* at, double v) {
+    if (real_bytes == 4) {
+      const float f = static_cast<float>(v);
+      std::memcpy(at, &f, 4);
+    } else {
+      std::memcpy(at, &v, 8);
+    }
+  };
+  const int rs = real_bytes / 4;
   size_t rec = 0;
   for (size_t s = 0; s < p.seg_kind.size(); ++s) {
     r.seg_first.push_back(static_cast<int>(rec));
@@ -441,15 +494,10 @@ LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
         const uint32_t end = w >> 30;
         R[j] = (w & 1023u) * col;
         R[ns + j] = ((w >> 10) & 1023u) * col;
-        R[2 * ns + j] = ((w >> 20) & 1023u) * col;
-        if (end != kLaneEndNone) mask |= (1u << j) | (end << (8 + 2 * j));
-        if (real_bytes == 4) {
-          const float c = static_cast<float>(p.coeff[base + j]);
-          std::memcpy(R + hw + j, &c, 4);
-        } else {
-          const double c = p.coeff[base + j];
-          std::memcpy(R + hw + 2 * j, &c, 8);
-        }
+        R[2 * ns + j] = static_cast<uint32_t>(p.write_slot[base + j]) * col;
+        if (end != kLaneEndNone) mask |= 1u << j;
+        put_real(R + hw + j * rs, p.keep[base + j] ? 1.0 : 0.0);
+        put_real(R + hw + (ns + j) * rs, p.coeff[base + j]);           // coefficient
       }
       R[3 * ns] = mask;
     }
@@ -461,14 +509,26 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
                              double* grad_b)
 {
   const int B = p.basic_count, ns = p.nslot, hw = r.header_words(), rw = r.rec_words();
+  const int rs = r.real_bytes / 4;
   const uint32_t col = 64u;
+  auto get_real = [&](const uint32_t* at) {
+    if (r.real_bytes == 4) {
+      float f;
+      std::memcpy(&f, at, 4);
+      return static_cast<double>(f);
+    }
+    double d;
+    std::memcpy(&d, at, 8);
+    return d;
+  };
   std::vector<double> X(p.slots, 0.0);
   for (int i = 0; i < B; ++i) X[i] = basics[i];
   X[p.ones_slot] = 1.0;
   double e = 0.0;
   std::vector<double> acc(ns), xv(ns), yv(ns);
   for (size_t s = 0; s < p.seg_kind.size(); ++s) {
-    const bool bin = p.seg_kind[s] == 0;
+    const int kind = p.seg_kind[s];
+    const bool bin = kind != kLaneUnaryStore;
     const std::vector<double> X0 = X;  // waves of a segment read what earlier segments wrote
     for (int w = 0; w < p.waves; ++w) {
       std::fill(acc.begin(), acc.end(), 0.0);
@@ -476,26 +536,24 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
         const uint32_t* R = r.words.data() +
                             (static_cast<size_t>(r.seg_first[s]) + static_cast<size_t>(w) * p.seg_groups[s] + g) * rw;
         for (int j = 0; j < ns; ++j) {
-          double c;
-          if (r.real_bytes == 4) {
-            float f;
-            std::memcpy(&f, R + hw + j, 4);
-            c = f;
-          } else {
-            std::memcpy(&c, R + hw + 2 * j, 8);
-          }
           xv[j] = X0[R[j] / col];
           yv[j] = bin ? X0[R[ns + j] / col] : 1.0;
-          acc[j] += c * xv[j] * yv[j];
+          acc[j] += get_real(R + hw + (ns + j) * rs) * xv[j] * yv[j];
         }
         const uint32_t mask = R[3 * ns];
         for (int j = 0; j < ns; ++j) {
-          if (!(mask & (1u << j))) continue;
-          const uint32_t action = (mask >> (8 + 2 * j)) & 3u, t = R[2 * ns + j] / col;
-          if (action == kLaneEndStore) X[t] = acc[j];
-          if (action == kLaneEndAdd) X[t] += acc[j];
-          if (action == kLaneEndEnergy) e += acc[j];
-          acc[j] = 0.0;
+          const double keep = get_real(R + hw + j * rs);
+          const uint32_t t = R[2 * ns + j] / col;
+          if (kind == kLaneBinaryStore || kind == kLaneUnaryStore) {
+            X[t] = acc[j];
+            acc[j] *= keep;
+          } else if (kind == kLaneBinaryEnergy) {
+            e += acc[j] - acc[j] * keep;
+            acc[j] *= keep;
+          } else if (mask & (1u << j)) {
+            X[t] += acc[j];
+            acc[j] = 0.0;
+          }
         }
       }
     }

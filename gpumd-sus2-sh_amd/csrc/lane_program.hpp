@@ -16,12 +16,13 @@
 //   ENG  E' += 1/2 sum_n v_n (g_n + d_n)       (exact for a quadratic form)
 //   BWD  G_b[q] += sum c g_n b_other  (+ g_q for a basic node q)
 // Components are packed into batches whose nodes fit the per-lane value region; each batch
-// runs as three segments (FWD, ADJ, ENG+BWD).  Inside a segment rows are independent: they
+// runs as four segments (FWD, ADJ, ENG, BWD).  Inside a segment rows are independent: they
 // are spread longest-first over `waves` x `nslot` accumulators -- `waves` waves share one
 // 64-atom tile and split the rows, each wave's rows interleaved as groups of nslot
 // instructions so that one group issues nslot independent LDS reads per operand.
 //
-// Per-lane slot map: [0, B) basics, [B, 2B) basic adjoints, 2B the constant 1,
+// Per-lane slot map: [0, B) basics, [B, 2B) basic adjoints, 2B the constant 1, (2B + 1 a
+// write-only trash slot, only when a store segment has an accumulator without rows),
 // [v_base, v_base + cap_v) component values, [g_base, g_base + cap_g) component adjoints.
 // Instruction word: x | y << 10 | t << 20 | end << 30 with end 0 (none), 1 (slot t = acc),
 // 2 (slot t += acc), 3 (site energy += acc).  Binary segments add c X[x] X[y], unary
@@ -44,16 +45,25 @@ struct LaneProgram {
   int waves = 1;  // waves that share one 64-atom tile and split every segment's rows
   int cap_v = 0, cap_g = 0;  // value / adjoint region sizes
   int slots = 0;  // per-lane slots (LDS = slots * 64 * sizeof(Real) per wave)
-  int ones_slot = 0, v_base = 0, g_base = 0;
+  int ones_slot = 0, trash_slot = 0, v_base = 0, g_base = 0;
   int components = 0, largest_component = 0, batches = 0;
   long long useful_terms = 0;  // instructions that are not padding
   // segments: {kind (0 binary, 1 unary), first instruction, groups}
   std::vector<int> seg_kind, seg_begin, seg_groups;
   std::vector<uint32_t> ins;
   std::vector<double> coeff;
+  // device form of the row ends: slot each instruction's group writes its accumulator to
+  // (store segments: the row's target; add segments: the row's target at its end) and
+  // whether the accumulator is kept after the group (0 at a row end, except the last row of
+  // a store accumulator, whose sum is rewritten unchanged by the padding after it)
+  std::vector<uint16_t> write_slot;
+  std::vector<uint8_t> keep;
+  bool needs_trash = false;  // a store segment has an accumulator without rows
 };
 
 constexpr uint32_t kLaneEndNone = 0, kLaneEndStore = 1, kLaneEndAdd = 2, kLaneEndEnergy = 3;
+// segment kinds: every row of a segment ends the same way
+constexpr int kLaneBinaryStore = 0, kLaneUnaryStore = 1, kLaneBinaryEnergy = 2, kLaneBinaryAdd = 3;
 constexpr int kLaneMaxSlot = 1023;
 
 // Build the lane program with at most `max_slots` per-lane slots (the value / adjoint
@@ -70,14 +80,17 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
 // nslot instructions, wave-major inside a segment (record (w * groups + g) of the segment).
 //   words [0, nslot)          x word offsets in the lane column layout (slot * 64)
 //   words [nslot, 2 nslot)    y word offsets
-//   words [2 nslot, 3 nslot)  row-end target word offsets
-//   word  3 nslot             row-end mask: bit j = instruction j ends its row, bits
-//                             8 + 2j .. 9 + 2j = its action (1 store, 2 add, 3 energy)
-//   words [hdr, hdr + nslot * real_bytes / 4)  coefficients (Real), hdr = header_words()
+//   words [2 nslot, 3 nslot)  target word offsets: store segments write acc to them after
+//                             every group (partial sums land in the row's own target,
+//                             which no instruction of the segment reads), add segments
+//                             add at the row's end
+//   word  3 nslot             row-end mask: bit j = instruction j ends its row
+//   then nslot keep factors (Real: 0 at a row end, else 1; acc *= keep after each group)
+//   and nslot coefficients (Real), starting at word header_words()
 struct LaneRecords {
   int nslot = 0, real_bytes = 0;
   int header_words() const { return (3 * nslot + 1 + 3) / 4 * 4; }
-  int rec_words() const { return header_words() + nslot * real_bytes / 4; }
+  int rec_words() const { return header_words() + 2 * nslot * real_bytes / 4; }
   std::vector<uint32_t> words;
   std::vector<int> seg_first;  // first record of each segment
 };

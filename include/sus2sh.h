@@ -191,7 +191,8 @@ int sus2sh_model_check(const char* mtp_path, int* dims);
 
 /* Host-only check of the lane program (variant 3 of the graph kernel; no reference
  * counterpart): compile it for the model with at most max_slots per-lane slots and `waves`
- * (1, 2, 4 or 8) waves per 64-atom tile, and run it on the host for n basic vectors
+ * (1, 2, 4 or 8) waves per 64-atom tile (4 accumulators per wave, 2 with 8 waves: the shapes
+ * sus2_sh_lane_waves selects), and run it on the host for n basic vectors
  * basics[n][B] -> e_prime[n] = sum_s w_s M[map[s]] and grad[n][B] = de_prime/dbasic (without
  * the species factor).  info[0..6) = ok, slots, instructions per atom, graph components,
  * batches, largest component.  When the model has no lane program (info[0] = 0) nothing is

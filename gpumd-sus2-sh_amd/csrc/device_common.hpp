@@ -68,17 +68,42 @@ __device__ __forceinline__ Real dev_sqrt(Real x)
   return sqrt(x);
 }
 
+// 1/sqrt(x) and tanh(z).  FP32 uses the hardware reciprocal square root and exponential
+// (tanh z = 1 - 2 / (exp(2z) + 1), absolute error ~1e-7 on the |z| < 10 range of the radial
+// map); FP64 uses the correctly rounded library functions.
+template <typename Real>
+__device__ __forceinline__ Real dev_rsqrt(Real x)
+{
+  return Real(1) / sqrt(x);
+}
+template <>
+__device__ __forceinline__ float dev_rsqrt<float>(float x)
+{
+  return __builtin_amdgcn_rsqf(x);
+}
+template <typename Real>
+__device__ __forceinline__ Real dev_tanh(Real z)
+{
+  return tanh(z);
+}
+template <>
+__device__ __forceinline__ float dev_tanh<float>(float z)
+{
+  const float t = __builtin_amdgcn_exp2f(z * 2.8853900817779268f);  // exp(2z) = 2^(2 z log2 e)
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
+}
+
 // xi(r) and dxi/dr for one (a, s)
 template <typename Real>
 __device__ __forceinline__ void radial_map(int kind, Real a, Real s, Real r, Real& xi, Real& dxi)
 {
   const Real z = Real(0.5) * a * (r - s);
   if (kind == 1) {
-    const Real q = Real(1) / sqrt(Real(1) + z * z);
+    const Real q = dev_rsqrt<Real>(Real(1) + z * z);
     xi = z * q;
     dxi = Real(0.5) * a * q * q * q;
   } else {
-    xi = tanh(z);
+    xi = dev_tanh<Real>(z);
     dxi = Real(0.5) * a * (Real(1) - xi * xi);
   }
 }
@@ -86,43 +111,39 @@ __device__ __forceinline__ void radial_map(int kind, Real a, Real s, Real r, Rea
 // R_mu(r) (and R'_mu(r)) for all MR radial functions of one pair: Chebyshev recurrence on
 // xi with the (r - rc)^2 envelope (src/force/sus2_sh.cu:3171-3220).  `coeff` and `scal`
 // point at the pair's block of the tables (usually in LDS).
-template <typename Real, int MR, bool DERIV>
+// NB > 0: every mu shares (a, s) ("uniform"); coefficient rows are NB long, zero beyond rb,
+//         so T_n(xi), T'_n(xi) are computed once per edge and the sums need no bound test.
+// NB = 0: (a, s) per mu; coefficient rows are rb long.
+template <typename Real, int MR, bool DERIV, int NB>
 __device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
                                             const Real* __restrict__ scal, int rb, Real rc,
-                                            int kind, int uniform, Real r, Real (&R)[MR],
-                                            Real (&dR)[MR])
+                                            int kind, Real r, Real (&R)[MR], Real (&dR)[MR])
 {
   const Real dr = r - rc;
   const Real env = dr * dr;
   const Real denv = Real(2) * dr;
   Real xi = 0, dxi = 0;
-  if (uniform) {
-    // every mu shares xi: the Chebyshev values T_n(xi), T'_n(xi) are computed once per edge
+  if constexpr (NB > 0) {
     radial_map<Real>(kind, scal[0], scal[1], r, xi, dxi);
-    constexpr int kMaxN = 16;
-    Real T[kMaxN], dT[kMaxN];
+    Real T[NB], dT[NB];
     T[0] = Real(1);
     dT[0] = Real(0);
     T[1] = xi;
     dT[1] = Real(1);
 #pragma unroll
-    for (int n = 2; n < kMaxN; ++n) {
-      if (n < rb) {
-        T[n] = Real(2) * xi * T[n - 1] - T[n - 2];
-        if (DERIV) dT[n] = Real(2) * (T[n - 1] + xi * dT[n - 1]) - dT[n - 2];
-      }
+    for (int n = 2; n < NB; ++n) {
+      T[n] = Real(2) * xi * T[n - 1] - T[n - 2];
+      if (DERIV) dT[n] = Real(2) * (T[n - 1] + xi * dT[n - 1]) - dT[n - 2];
     }
 #pragma unroll
     for (int mu = 0; mu < MR; ++mu) {
-      const Real* c = coeff + mu * rb;
+      const Real* c = coeff + mu * NB;
       Real acc = 0, accd = 0;
 #pragma unroll
-      for (int n = 0; n < kMaxN; ++n) {
-        if (n < rb) {
-          const Real cn = c[n];
-          acc += cn * T[n];
-          if (DERIV) accd += cn * dT[n];
-        }
+      for (int n = 0; n < NB; ++n) {
+        const Real cn = c[n];
+        acc += cn * T[n];
+        if (DERIV) accd += cn * dT[n];
       }
       R[mu] = env * acc;
       if (DERIV) dR[mu] = denv * acc + env * dxi * accd;

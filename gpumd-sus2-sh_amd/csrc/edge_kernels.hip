@@ -18,16 +18,19 @@ namespace {
 // same coefficient of different pair tables hit different banks.
 __host__ __device__ constexpr int padded_stride(int n) { return n + ((1 - n % 32) + 32) % 32; }
 
+// Coefficient row length in LDS: NB (zero-padded) for the uniform kernels, rb otherwise.
+__host__ __device__ constexpr int coeff_row(int NB, int rb) { return NB > 0 ? NB : rb; }
+
 template <typename Real>
-__device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, Real* s_coeff,
-                                             Real* s_scal)
+__device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, int row,
+                                             Real* s_coeff, Real* s_scal)
 {
   const int pairs = rt.S * rt.S;
-  const int nc = Mr * rt.rb, ns = Mr * 2;
+  const int nc = Mr * row, ns = Mr * 2;
   const int cs = padded_stride(nc), ss = padded_stride(ns);
   for (int i = threadIdx.x; i < pairs * nc; i += blockDim.x) {
-    const int p = i / nc;
-    s_coeff[p * cs + (i - p * nc)] = rt.coeff[i];
+    const int p = i / nc, k = i - p * nc, mu = k / row, n = k - mu * row;
+    s_coeff[p * cs + k] = n < rt.rb ? rt.coeff[(p * Mr + mu) * rt.rb + n] : Real(0);
   }
   for (int i = threadIdx.x; i < pairs * ns; i += blockDim.x) {
     const int p = i / ns;
@@ -36,7 +39,7 @@ __device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr
   __syncthreads();
 }
 
-template <typename Real, int L, int K>
+template <typename Real, int L, int K, int NB>
 __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __restrict__ out)
 {
   constexpr int MR = K * (L + 1);
@@ -44,9 +47,10 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   constexpr int FG = K * NC;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
-  const int cs = padded_stride(MR * a.rt.rb), ss = padded_stride(MR * 2);
+  const int row = coeff_row(NB, a.rt.rb);
+  const int cs = padded_stride(MR * row), ss = padded_stride(MR * 2);
   Real* s_scal = s_coeff + a.rt.S * a.rt.S * cs;
-  stage_tables(a.rt, MR, s_coeff, s_scal);
+  stage_tables(a.rt, MR, row, s_coeff, s_scal);
 
   const int N = a.N;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,12 +69,12 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
     const Edge<Real> e = next;
     if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
-    const Real r = sqrt(r2);
-    const Real inv = Real(1) / r;
+    const Real inv = dev_rsqrt<Real>(r2);
+    const Real r = r2 * inv;
     const int pair = ti * S + (e.j >> kEdgeTypeShift);
     Real R[MR], dR[MR];
-    eval_radial<Real, MR, false>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
-                                 a.rt.kind, a.rt.uniform, r, R, dR);
+    eval_radial<Real, MR, false, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
+                                     a.rt.kind, r, R, dR);
     Real Sv[NC];
     solid_values<Real, L>(e.dx * inv, e.dy * inv, e.dz * inv, Sv);
 #pragma unroll
@@ -110,11 +114,11 @@ __device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ co
     const Real z = Real(0.5) * a * (r - s);
     Real xi, dxdz;
     if (kind == 1) {
-      const Real q = Real(1) / sqrt(Real(1) + z * z);
+      const Real q = dev_rsqrt<Real>(Real(1) + z * z);
       xi = z * q;
       dxdz = q * q * q;
     } else {
-      xi = tanh(z);
+      xi = dev_tanh<Real>(z);
       dxdz = Real(1) - xi * xi;
     }
     const Real* c = coeff + mu * rb;
@@ -141,7 +145,7 @@ __device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ co
   }
 }
 
-template <typename Real, int L, int K, bool PG>
+template <typename Real, int L, int K, bool PG, int NB>
 __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
                                                 double* __restrict__ v_s,
@@ -154,7 +158,8 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   const int S = a.rt.S;
   const int rb = a.rt.rb;
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
-  const int cs = padded_stride(MR * rb), ss = padded_stride(MR * 2);
+  const int row = coeff_row(NB, rb);
+  const int cs = padded_stride(MR * row), ss = padded_stride(MR * 2);
   Real* s_scal = s_coeff + S * S * cs;
   // PG: per-pair adjoint accumulators [S*S][MR*rb + 2*MR] (double), 16-byte aligned
   const size_t pg_off = ((static_cast<size_t>(S) * S * (cs + ss) * sizeof(Real)) + 15) & ~size_t(15);
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   if (PG) {
     for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) s_pg[i] = 0.0;
   }
-  stage_tables(a.rt, MR, s_coeff, s_scal);
+  stage_tables(a.rt, MR, row, s_coeff, s_scal);
 
   const int N = a.N;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,14 +190,14 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     const Edge<Real> e = next;
     if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
-    const Real r = sqrt(r2);
-    const Real inv = Real(1) / r;
+    const Real inv = dev_rsqrt<Real>(r2);
+    const Real r = r2 * inv;
     const Real ux = e.dx * inv, uy = e.dy * inv, uz = e.dz * inv;
     const int pair = ti * S + (e.j >> kEdgeTypeShift);
     const int jn = e.j & kEdgeIndexMask;
     Real R[MR], dR[MR];
-    eval_radial<Real, MR, true>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
-                                a.rt.kind, a.rt.uniform, r, R, dR);
+    eval_radial<Real, MR, true, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
+                                    a.rt.kind, r, R, dR);
     Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
     solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
     // g = u * sum (V - l U / r) Y + (1/r) sum U N_l grad S,  with U = sum_k G R, V = sum_k G R'
@@ -270,11 +275,19 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
 }
 
 template <typename Real>
-size_t table_bytes(const EdgeLaunch<Real>& a)
+size_t table_bytes(const EdgeLaunch<Real>& a, int NB)
 {
   const int Mr = a.K * (a.L + 1);
   return static_cast<size_t>(a.rt.S) * a.rt.S *
-         (padded_stride(Mr * a.rt.rb) + padded_stride(Mr * 2)) * sizeof(Real);
+         (padded_stride(Mr * coeff_row(NB, a.rt.rb)) + padded_stride(Mr * 2)) * sizeof(Real);
+}
+
+// NB of the uniform kernels: coefficient rows padded to 8 or 16; 0 selects the per-mu path.
+template <typename Real>
+int uniform_nb(const EdgeLaunch<Real>& a)
+{
+  if (!a.rt.uniform) return 0;
+  return a.rt.rb <= 8 ? 8 : 16;
 }
 
 #define SUS2SH_FOR_EACH_LK(X) \
@@ -291,11 +304,17 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
 {
   if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
-  const size_t lds = table_bytes(a);
-#define SUS2SH_BASICS_CASE(LL, KK)                                           \
-  if (a.L == LL && a.K == KK) {                                              \
-    k_basics<Real, LL, KK><<<blocks, 256, lds, s>>>(a, basics);               \
-    return true;                                                             \
+  const int nb = uniform_nb(a);
+  const size_t lds = table_bytes(a, nb);
+#define SUS2SH_BASICS_CASE(LL, KK)                                                \
+  if (a.L == LL && a.K == KK) {                                                   \
+    if (nb == 8)                                                                  \
+      k_basics<Real, LL, KK, 8><<<blocks, 256, lds, s>>>(a, basics);               \
+    else if (nb == 16)                                                            \
+      k_basics<Real, LL, KK, 16><<<blocks, 256, lds, s>>>(a, basics);              \
+    else                                                                          \
+      k_basics<Real, LL, KK, 0><<<blocks, 256, lds, s>>>(a, basics);               \
+    return true;                                                                  \
   }
   SUS2SH_FOR_EACH_LK(SUS2SH_BASICS_CASE)
 #undef SUS2SH_BASICS_CASE
@@ -308,19 +327,24 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
 {
   if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
-  size_t lds = table_bytes(a);
+  const int nb = pg_rad ? 0 : uniform_nb(a);  // the parameter gradient uses the per-mu path
+  size_t lds = table_bytes(a, nb);
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
     lds = ((lds + 15) & ~size_t(15)) +
           sizeof(double) * a.rt.S * a.rt.S * (Mr * a.rt.rb + 2 * Mr);
   }
-#define SUS2SH_FORCES_CASE(LL, KK)                                                         \
-  if (a.L == LL && a.K == KK) {                                                            \
-    if (pg_rad)                                                                            \
-      k_forces<Real, LL, KK, true><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);   \
-    else                                                                                   \
-      k_forces<Real, LL, KK, false><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);  \
-    return true;                                                                           \
+#define SUS2SH_FORCES_CASE(LL, KK)                                                           \
+  if (a.L == LL && a.K == KK) {                                                              \
+    if (pg_rad)                                                                              \
+      k_forces<Real, LL, KK, true, 0><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);  \
+    else if (nb == 8)                                                                        \
+      k_forces<Real, LL, KK, false, 8><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad); \
+    else if (nb == 16)                                                                       \
+      k_forces<Real, LL, KK, false, 16><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);\
+    else                                                                                     \
+      k_forces<Real, LL, KK, false, 0><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad); \
+    return true;                                                                             \
   }
   SUS2SH_FOR_EACH_LK(SUS2SH_FORCES_CASE)
 #undef SUS2SH_FORCES_CASE

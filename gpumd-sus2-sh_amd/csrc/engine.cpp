@@ -117,8 +117,10 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   // LDS radial tables of the edge kernels: per pair, coefficients and (a, s), each padded to
   // 1 mod 32 words (edge_kernels.hip)
   const auto padded = [](size_t n) { return n + ((1 + 32 - n % 32) % 32); };
+  // (rows of the uniform kernels are zero-padded to 8 or 16 coefficients)
+  const size_t row = model_.rb_size <= 8 ? 8 : 16;
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
-                       (padded(static_cast<size_t>(model_.radial_funcs_count) * model_.rb_size) +
+                       (padded(static_cast<size_t>(model_.radial_funcs_count) * row) +
                         padded(static_cast<size_t>(model_.radial_funcs_count) * 2)) * rsz;
   if (table > 65536)
     throw UnsupportedError("SUS2_SH: radial tables exceed 64 KiB of LDS (too many species).");

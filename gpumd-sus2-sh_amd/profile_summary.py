@@ -1,0 +1,92 @@
+#!/usr/bin/env python3
+"""Summarise rocprofv3 output of bench.py runs into the short text files kept in profiles/.
+
+  python gpumd-sus2-sh_amd/profile_summary.py stats  <kernel_stats.csv> [--top 12]
+  python gpumd-sus2-sh_amd/profile_summary.py traffic <fetch_counter_collection.csv>
+         <write_counter_collection.csv> --kernel <substring> --copy-bytes <bytes>
+
+`stats` prints the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (calls,
+average / total duration, share of GPU time).  `traffic` reads one `--pmc FETCH_SIZE` and one
+`--pmc WRITE_SIZE` run, takes the per-dispatch averages (KiB) of the kernel whose name
+contains --kernel, and scales each by the factor that makes the plain copy kernel of the same
+run (bench.py --copy-check) read and write its known --copy-bytes: HBM bytes per launch =
+counter * copy_bytes / counter(copy).  It prints a JSON object that bench.py can report as
+roofline.traffic.
+"""
+import argparse
+import collections
+import csv
+import json
+import sys
+
+
+def stats(path, top):
+    rows = list(csv.DictReader(open(path)))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print(f"{'kernel':70s} {'calls':>6s} {'avg us':>10s} {'total ms':>10s} {'share':>6s}")
+    for r in rows[:top]:
+        name = r["Name"][:70]
+        t = float(r["TotalDurationNs"])
+        print(f"{name:70s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:10.1f} "
+              f"{t / 1e6:10.3f} {100 * t / total:5.1f}%")
+
+
+def per_kernel(path, counter):
+    acc = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def find(d, sub):
+    hits = [k for k in d if sub in k]
+    if not hits:
+        raise SystemExit(f"no kernel matching {sub!r}")
+    return hits[0]
+
+
+def traffic(fetch_csv, write_csv, kernel, copy_bytes):
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    # the largest copyBuffer dispatch is the calibration copy (bench.py --copy-check)
+    copies = [k for k in fetch if "copyBuffer" in k or "elementwise_kernel" in k]
+    ck = max(copies, key=lambda k: fetch[k])
+    cw = max((k for k in write if k in copies), key=lambda k: write[k])
+    f_scale = copy_bytes / (fetch[ck] * 1024.0)
+    w_scale = copy_bytes / (write[cw] * 1024.0)
+    k = find(fetch, kernel)
+    out = {
+        "kernel": k[:120],
+        "fetch_kib": fetch[k],
+        "write_kib": write[find(write, kernel)],
+        "copy_kernel": ck[:80],
+        "copy_bytes": copy_bytes,
+        "fetch_scale": f_scale,
+        "write_scale": w_scale,
+    }
+    out["hbm_bytes"] = out["fetch_kib"] * 1024.0 * f_scale + out["write_kib"] * 1024.0 * w_scale
+    print(json.dumps(out, indent=1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--top", type=int, default=12)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv")
+    t.add_argument("write_csv")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-bytes", type=float, required=True)
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        stats(a.csv, a.top)
+    else:
+        traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -139,6 +139,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="before the runs, copy MIB MiB device-to-device (untimed): the plain "
+                         "copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c3_fp32_traffic.json"),
+                    help="HBM bytes per launch of the dominant kernel measured with rocprofv3 "
+                         "(profile_summary.py traffic); reported as roofline.traffic when its "
+                         "kernel is the one this run's dominant stage launched")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part "
@@ -149,6 +156,16 @@ def main():
     import torch
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
+
+    if args.copy_check > 0:
+        n_copy = args.copy_check * (1 << 20) // 4
+        src = torch.ones(n_copy, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy check: {args.copy_check} MiB device-to-device, twice")
+        del src, dst
 
     nx, ny, nz = args.cells
     log(f"[bench] building workload {nx}x{ny}x{nz} fcc")
@@ -169,7 +186,14 @@ def main():
     dominant = max(("basic", "graph", "force"), key=lambda k: stages[k])
     achieved = flops[dominant] * N / (stages[dominant] * 1e-3) / 1e12
     total_flops = (flops["basic"] + flops["graph"] + flops["force"]) * N
-
+    kernel_name = {"basic": "k_basics", "force": "k_forces",
+                   "graph": "k_lane_graph" if info["graph_variant"] == 3 else "k_graph"}[dominant]
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            prof = json.load(fh)
+        if kernel_name + "<" in prof.get("kernel", "") and prof.get("atoms") == N:
+            traffic = prof["hbm_bytes"]
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (FP64 oracle, bounded sample)")
@@ -206,10 +230,13 @@ def main():
         "graph_tuning_ms_fp32": r32["tuning"],
         "graph_tuning_ms_fp64": r64["tuning"],
         "step_tflops_fp32": total_flops / (r32["ms_per_step"] * 1e-3) / 1e12,
-        "roofline": {"bound": "valu", "kernel": dominant, "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "flops_per_atom": flops[dominant]},
+        # the dominant kernel is FP32 vector-ALU / LDS work (no matrix-core shape): it is
+        # priced against the published FP32 compute peak; traffic is HBM bytes per launch
+        "roofline": {"bound": "mfma", "kernel": kernel_name, "stage": dominant,
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "flops_per_atom": flops[dominant],
+                     "launch_ms": stages[dominant]},
         "cpu_baseline": cpu,
         "vs_baseline_note": "reference A100 figure 5.23633e6 is a whole GPUMD NPT step with an "
                             "rc=7.5 rb=10 l3333 glass model",

@@ -145,11 +145,33 @@ __device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ co
   }
 }
 
-template <typename Real, int L, int K, bool PG, int NB>
+// Position of sorted atom j in the LDS accumulators of a tile (-1: outside the windows):
+// windows are disjoint and sorted by first atom; w_off is their prefix size.
+__device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
+                                           const int* __restrict__ w_off, int count, int j)
+{
+  if (count <= 0) return -1;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (w_first[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int d = j - w_first[lo];
+  return (d >= 0 && d < w_off[lo + 1] - w_off[lo]) ? w_off[lo] + d : -1;
+}
+
+// WIN: neighbour forces of the tile (kWinTile atoms) are summed in LDS over the tile's
+// neighbour windows (launch_windows) and flushed with one global atomic per neighbour and
+// component; a tile without windows (count < 0) falls back to per-edge global atomics.
+template <typename Real, int L, int K, bool PG, int NB, bool WIN>
 __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
                                                 double* __restrict__ v_s,
-                                                double* __restrict__ pg_rad)
+                                                double* __restrict__ pg_rad,
+                                                const int* __restrict__ win)
 {
   constexpr int MR = K * (L + 1);
   constexpr int NC = (L + 1) * (L + 1);
@@ -168,12 +190,34 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   if (PG) {
     for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) s_pg[i] = 0.0;
   }
-  stage_tables(a.rt, MR, row, s_coeff, s_scal);
+  // WIN: window table (first atom, prefix size) and the [3][total] accumulators
+  int* w_first = reinterpret_cast<int*>(smem_raw + pg_off);
+  int* w_off = w_first + kWinMax;
+  Real* acc_w = reinterpret_cast<Real*>(smem_raw + pg_off + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16));
+  int w_count = -1, w_total = 0;
+  if (WIN) {
+    const int* tw = win + static_cast<size_t>(blockIdx.x) * kWinStride;
+    w_count = tw[0];
+    w_total = tw[1];
+    if (w_count >= 0) {
+      if (threadIdx.x < w_count) w_first[threadIdx.x] = tw[2 + 2 * threadIdx.x];
+      if (threadIdx.x == 0) {
+        int o = 0;
+        for (int w = 0; w < w_count; ++w) {
+          w_off[w] = o;
+          o += tw[3 + 2 * w];
+        }
+        w_off[w_count] = o;
+      }
+      for (int i = threadIdx.x; i < 3 * w_total; i += blockDim.x) acc_w[i] = Real(0);
+    }
+  }
+  stage_tables(a.rt, MR, row, s_coeff, s_scal);  // ends with a barrier
 
   const int N = a.N;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p < N;
-  if (!PG && !valid) return;
+  if (!PG && !WIN && !valid) return;
   Real G[FG];
 #pragma unroll
   for (int f = 0; f < FG; ++f) {
@@ -245,9 +289,16 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     fx += g0;
     fy += g1;
     fz += g2;
-    unsafeAtomicAdd(f_s + jn, -static_cast<double>(g0));
-    unsafeAtomicAdd(f_s + N + jn, -static_cast<double>(g1));
-    unsafeAtomicAdd(f_s + 2 * N + jn, -static_cast<double>(g2));
+    const int slot = (WIN && w_count >= 0) ? window_slot(w_first, w_off, w_count, jn) : -1;
+    if (slot >= 0) {
+      atomicAdd(acc_w + slot, -g0);
+      atomicAdd(acc_w + w_total + slot, -g1);
+      atomicAdd(acc_w + 2 * w_total + slot, -g2);
+    } else {
+      unsafeAtomicAdd(f_s + jn, -static_cast<double>(g0));
+      unsafeAtomicAdd(f_s + N + jn, -static_cast<double>(g1));
+      unsafeAtomicAdd(f_s + 2 * N + jn, -static_cast<double>(g2));
+    }
     w[0] -= e.dx * g0;
     w[1] -= e.dy * g1;
     w[2] -= e.dz * g2;
@@ -264,6 +315,25 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
     unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
 #pragma unroll
     for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+  }
+  if (WIN && w_count >= 0) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < w_total; i += blockDim.x) {
+      int lo = 0, hi = w_count - 1;  // window of slot i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (w_off[mid] <= i)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int j = w_first[lo] + (i - w_off[lo]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Real v = acc_w[c * w_total + i];
+        if (v != Real(0)) unsafeAtomicAdd(f_s + static_cast<size_t>(c) * N + j, static_cast<double>(v));
+      }
+    }
   }
   if (PG) {
     __syncthreads();
@@ -323,39 +393,56 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
 
 template <typename Real>
 bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
-                   double* pg_rad, hipStream_t s)
+                   double* pg_rad, const int* win, int win_cap, hipStream_t s)
 {
   if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
+  static_assert(kWinTile == 256, "force tiles are the 256-thread workgroups");
   const int nb = pg_rad ? 0 : uniform_nb(a);  // the parameter gradient uses the per-mu path
+  const bool use_win = win && !pg_rad;
   size_t lds = table_bytes(a, nb);
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
     lds = ((lds + 15) & ~size_t(15)) +
           sizeof(double) * a.rt.S * a.rt.S * (Mr * a.rt.rb + 2 * Mr);
   }
+  if (use_win)
+    lds = ((lds + 15) & ~size_t(15)) + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16) +
+          3 * sizeof(Real) * static_cast<size_t>(win_cap);
+#define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV)                                           \
+  {                                                                                            \
+    auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV>;                                       \
+    if (lds > 65536)                                                                           \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                           \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)); \
+    kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
+  }
 #define SUS2SH_FORCES_CASE(LL, KK)                                                           \
   if (a.L == LL && a.K == KK) {                                                              \
-    if (pg_rad)                                                                              \
-      k_forces<Real, LL, KK, true, 0><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);  \
-    else if (nb == 8)                                                                        \
-      k_forces<Real, LL, KK, false, 8><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad); \
-    else if (nb == 16)                                                                       \
-      k_forces<Real, LL, KK, false, 16><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad);\
-    else                                                                                     \
-      k_forces<Real, LL, KK, false, 0><<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad); \
+    if (pg_rad) {                                                                            \
+      SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false)                                           \
+    } else if (use_win) {                                                                    \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true)                              \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true)                       \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true)                                      \
+    } else {                                                                                 \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false)                             \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false)                      \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false)                                     \
+    }                                                                                        \
     return true;                                                                             \
   }
   SUS2SH_FOR_EACH_LK(SUS2SH_FORCES_CASE)
 #undef SUS2SH_FORCES_CASE
+#undef SUS2SH_FORCES_LAUNCH
   return false;
 }
 
 template bool launch_basics<float>(const EdgeLaunch<float>&, float*, hipStream_t);
 template bool launch_basics<double>(const EdgeLaunch<double>&, double*, hipStream_t);
 template bool launch_forces<float>(const EdgeLaunch<float>&, const float*, double*, double*,
-                                   double*, hipStream_t);
+                                   double*, const int*, int, hipStream_t);
 template bool launch_forces<double>(const EdgeLaunch<double>&, const double*, double*, double*,
-                                    double*, hipStream_t);
+                                    double*, const int*, int, hipStream_t);
 
 }  // namespace sus2sh

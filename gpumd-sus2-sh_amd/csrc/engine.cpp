@@ -669,9 +669,20 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
+  // neighbour windows of the force tiles (single structure): LDS accumulation of the
+  // neighbour forces, budget 48 KiB (FP32) / 64 KiB (FP64) of accumulators per tile
+  const int* win = nullptr;
+  int win_cap = 0;
+  if (!batch && !param_grad) {
+    win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * sizeof(Real)));
+    const size_t tiles = (static_cast<size_t>(N) + kWinTile - 1) / kWinTile;
+    win_.resize(tiles * kWinStride * sizeof(int));
+    launch_windows(N, grid, skey_.as<int>(), win_.as<int>(), win_cap, s);
+    win = win_.as<int>();
+  }
   launch_fill_double(f_s_.as<double>(), 0.0, 3 * static_cast<size_t>(N), s);
   if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(),
-                           param_grad ? pg_rad_.as<double>() : nullptr, s))
+                           param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
   if (opt_.profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 

@@ -145,6 +145,7 @@ class Engine {
     bwd_slen_[2];
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
   DeviceBuffer lane_segs_, lane_ins_;
+  DeviceBuffer win_;  // neighbour windows of the force tiles
   int cu_count_ = 256;
   bool radial_uniform_ = false;
 

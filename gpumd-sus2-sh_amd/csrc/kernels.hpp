@@ -82,6 +82,15 @@ template <typename Real>
 void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* type_s,
                       const int* sorted_key, const int* cell_begin, const int* cell_end,
                       int* count, Edge<Real>* nl, int* max_count, hipStream_t s);
+// Neighbour index windows of each force tile of kWinTile consecutive cell-sorted atoms: the
+// neighbours of a tile lie in few contiguous ranges of the sorted order (the cells within
+// +-R of the tile's cells, row by row).  win[tile * kWinStride + ...] = count (-1: no window
+// set, the tile uses global atomics), total atoms, then count pairs (first atom, atoms).
+constexpr int kWinTile = 256;
+constexpr int kWinMax = 64;
+constexpr int kWinStride = 2 + 2 * kWinMax;
+void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,
+                    hipStream_t s);
 size_t radix_sort_temp_bytes(int N, int bits);
 hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
                             const int* vals_in, int* vals_out, int N, int bits, hipStream_t s);
@@ -106,9 +115,11 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);
 // force/virial: grads[B][N] -> f_s[3][N] (atomic, double) and v_s[9][N].
 // pg_rad (optional): accumulates dE/d(radial coeff, type coeff, scal) per pair into
 // pg_rad[S*S][Mr*rb + Mr*2] (double, atomic).
+// win (optional, single structure): tile windows from launch_windows with capacity win_cap;
+// neighbour forces are then summed per tile in LDS and flushed once per neighbour.
 template <typename Real>
 bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
-                   double* pg_rad, hipStream_t s);
+                   double* pg_rad, const int* win, int win_cap, hipStream_t s);
 
 // --- graph kernel (graph_kernel.hip) ---
 // mode 0: rows / back chunks read per thread; mode 1: interleaved per-thread instruction

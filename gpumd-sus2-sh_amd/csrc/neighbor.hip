@@ -210,6 +210,136 @@ __global__ void k_scatter(int N, const int* __restrict__ satom, const double* __
   for (int c = 0; c < 9; ++c) vir[static_cast<size_t>(c) * N + i] += v_s[static_cast<size_t>(c) * N + p];
 }
 
+// First sorted atom whose cell key is >= k.
+__device__ __forceinline__ int lower_bound_key(const int* __restrict__ skey, int N, int k)
+{
+  int lo = 0, hi = N;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (skey[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One wave per force tile.  Candidate intervals: for every cell row (y, z) the tile touches
+// and every row offset (oy, oz) within +-R, the x range of the tile's cells widened by +-R,
+// wrapped (two pieces) or clipped like the neighbour search; each cell range becomes an atom
+// range by binary search on the sorted keys.  Lane 0 sorts and merges them.
+constexpr int kWinCand = 128;
+
+__global__ __launch_bounds__(64) void k_windows(int N, CellGrid g, const int* __restrict__ skey,
+                                                int* __restrict__ win, int cap)
+{
+  __shared__ int cs[kWinCand], ce[kWinCand];
+  const int tile = blockIdx.x;
+  const int p0 = tile * kWinTile;
+  const int p1 = min(p0 + kWinTile, N) - 1;
+  int* out = win + static_cast<size_t>(tile) * kWinStride;
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const int k_lo = skey[p0], k_hi = skey[p1];
+  const int r_lo = k_lo / n0, r_hi = k_hi / n0;
+  const int ny = 2 * g.R[1] + 1, nz = 2 * g.R[2] + 1;
+  const int total = (r_hi - r_lo + 1) * ny * nz * 2;
+  if (total > kWinCand) {
+    if (threadIdx.x == 0) out[0] = -1;
+    return;
+  }
+  for (int c = threadIdx.x; c < total; c += 64) {
+    const int piece = c & 1;
+    int t = c >> 1;
+    const int oz = t % nz - g.R[2];
+    t /= nz;
+    const int oy = t % ny - g.R[1];
+    const int r = r_lo + t / ny;
+    const int y = r % n1, z = r / n1;
+    const int xa = r == r_lo ? k_lo % n0 : 0;
+    const int xb = r == r_hi ? k_hi % n0 : n0 - 1;
+    int yy = y + oy, zz = z + oz;
+    bool ok = true;
+    if (g.pbc[1]) yy = ((yy % n1) + n1) % n1; else ok = ok && yy >= 0 && yy < n1;
+    if (g.pbc[2]) zz = ((zz % n2) + n2) % n2; else ok = ok && zz >= 0 && zz < n2;
+    int x1 = xa - g.R[0], x2 = xb + g.R[0];
+    int a = 0, b = -1;  // cell x range of this piece (empty when b < a)
+    if (!g.pbc[0]) {
+      if (piece == 0) {
+        a = max(x1, 0);
+        b = min(x2, n0 - 1);
+      }
+    } else if (x2 - x1 + 1 >= n0) {
+      if (piece == 0) {
+        a = 0;
+        b = n0 - 1;
+      }
+    } else if (x1 < 0) {
+      a = piece == 0 ? x1 + n0 : 0;
+      b = piece == 0 ? n0 - 1 : x2;
+    } else if (x2 >= n0) {
+      a = piece == 0 ? x1 : 0;
+      b = piece == 0 ? n0 - 1 : x2 - n0;
+    } else if (piece == 0) {
+      a = x1;
+      b = x2;
+    }
+    int s0 = 0, s1 = 0;
+    if (ok && b >= a) {
+      const int row = (zz * n1 + yy) * n0;
+      s0 = lower_bound_key(skey, N, row + a);
+      s1 = lower_bound_key(skey, N, row + b + 1);
+    }
+    cs[c] = s0;
+    ce[c] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  // insertion sort by start, then merge overlapping / touching ranges
+  for (int i = 1; i < total; ++i) {
+    const int a = cs[i], b = ce[i];
+    int j = i - 1;
+    while (j >= 0 && cs[j] > a) {
+      cs[j + 1] = cs[j];
+      ce[j + 1] = ce[j];
+      --j;
+    }
+    cs[j + 1] = a;
+    ce[j + 1] = b;
+  }
+  int count = 0, size = 0, cur_s = 0, cur_e = -1;
+  for (int i = 0; i < total; ++i) {
+    if (ce[i] <= cs[i]) continue;  // empty
+    if (cur_e >= cs[i]) {
+      cur_e = max(cur_e, ce[i]);
+      continue;
+    }
+    if (cur_e > cur_s) {
+      if (count == kWinMax) {
+        out[0] = -1;
+        return;
+      }
+      out[2 + 2 * count] = cur_s;
+      out[3 + 2 * count] = cur_e - cur_s;
+      size += cur_e - cur_s;
+      ++count;
+    }
+    cur_s = cs[i];
+    cur_e = ce[i];
+  }
+  if (cur_e > cur_s) {
+    if (count == kWinMax) {
+      out[0] = -1;
+      return;
+    }
+    out[2 + 2 * count] = cur_s;
+    out[3 + 2 * count] = cur_e - cur_s;
+    size += cur_e - cur_s;
+    ++count;
+  }
+  out[0] = size <= cap ? count : -1;
+  out[1] = size;
+}
+
 inline int blocks_for(size_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -266,6 +396,13 @@ template void launch_neighbors<float>(const NeighborArgs&, const double*, const 
 template void launch_neighbors<double>(const NeighborArgs&, const double*, const int*, const int*,
                                        const int*, const int*, int*, Edge<double>*, int*,
                                        hipStream_t);
+
+void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,
+                    hipStream_t s)
+{
+  if (N <= 0) return;
+  k_windows<<<blocks_for(N, kWinTile), 64, 0, s>>>(N, g, sorted_key, win, cap);
+}
 
 size_t radix_sort_temp_bytes(int N, int bits)
 {

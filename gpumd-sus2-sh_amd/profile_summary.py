@@ -4,6 +4,7 @@
   python gpumd-sus2-sh_amd/profile_summary.py stats  <kernel_stats.csv> [--top 12]
   python gpumd-sus2-sh_amd/profile_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> --kernel <substring> --copy-bytes <bytes>
+         [--atoms N] [--command "..."]
 
 `stats` prints the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (calls,
 average / total duration, share of GPU time).  `traffic` reads one `--pmc FETCH_SIZE` and one
@@ -47,7 +48,7 @@ def find(d, sub):
     return hits[0]
 
 
-def traffic(fetch_csv, write_csv, kernel, copy_bytes):
+def traffic(fetch_csv, write_csv, kernel, copy_bytes, atoms, command):
     fetch = per_kernel(fetch_csv, "FETCH_SIZE")
     write = per_kernel(write_csv, "WRITE_SIZE")
     # the largest copyBuffer dispatch is the calibration copy (bench.py --copy-check)
@@ -67,6 +68,9 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
         "write_scale": w_scale,
     }
     out["hbm_bytes"] = out["fetch_kib"] * 1024.0 * f_scale + out["write_kib"] * 1024.0 * w_scale
+    out["atoms"] = atoms
+    out["hbm_bytes_per_atom"] = out["hbm_bytes"] / atoms if atoms else None
+    out["command"] = command
     print(json.dumps(out, indent=1))
 
 
@@ -81,11 +85,13 @@ def main():
     t.add_argument("write_csv")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
+    t.add_argument("--atoms", type=int, default=0)
+    t.add_argument("--command", default="", help="the profiled command, recorded in the output")
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.csv, a.top)
     else:
-        traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes)
+        traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes, a.atoms, a.command)
 
 
 if __name__ == "__main__":

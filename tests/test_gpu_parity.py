@@ -237,6 +237,36 @@ def test_lane_program_wave_splits(make_model, waves):
         _compare(ref, _run(path, cell, types, pos, options=opts), bool(fp32))
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_force_tiles_with_neighbour_windows(make_model, fp32):
+    """A periodic box of 6x6x6 cells and 2,048 atoms: eight 256-atom force tiles whose
+    neighbours are summed in LDS over index windows (wrapped rows included) -- against the
+    oracle, and against the same evaluation as one structure of a batch (batches keep the
+    per-edge global atomics)."""
+    path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])
+    pos, types, cell = st.fcc_alloy(8, 8, 8, 3.85, [0.5, 0.5], 0.08, 31, 32)
+    ref = Oracle(path).compute(cell, types, pos)
+    opts = ["Cu", "Zr", f"sus2_float={int(fp32)}"]
+    got = _run(path, cell, types, pos, options=opts)
+    _compare(ref, got, fp32)
+    import torch
+    n = len(types)
+    pot = sus2sh.SUS2_SH(path, 2 * n, opts)
+    dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt), device="cuda")
+    e = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3, 2 * n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9, 2 * n, dtype=torch.float64, device="cuda")
+    sus2sh.compute_batch(pot, [cell, cell], np.array([0, n, 2 * n], dtype=np.int32),
+                         dev(np.concatenate([types, types]), np.int32),
+                         dev(np.concatenate([pos, pos], axis=1), np.float64), e, f, v)
+    torch.cuda.synchronize()
+    pot.close()
+    e, f, v = e.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy()
+    for k in range(2):
+        sl = slice(k * n, (k + 1) * n)
+        _compare(ref, (e[sl], f[:, sl], v[:, sl]), fp32)
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("tile", [1, 4, 16])
 def test_graph_kernel_variants(make_model, variant, tile):

@@ -34,35 +34,47 @@ def stats(path, top):
 
 
 def per_kernel(path, counter):
+    """{kernel name: (mean counter value per dispatch, dispatch count, largest dispatch)}"""
     acc = collections.defaultdict(list)
     for r in csv.DictReader(open(path)):
         if r["Counter_Name"] == counter:
             acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+    return {k: (sum(v) / len(v), len(v), max(v)) for k, v in acc.items()}
 
 
 def find(d, sub):
+    """The kernel whose name contains `sub`; among several instantiations the one dispatched
+    most often (the autotuner's choice runs every step, the other candidates only while
+    being timed)."""
     hits = [k for k in d if sub in k]
     if not hits:
         raise SystemExit(f"no kernel matching {sub!r}")
-    return hits[0]
+    return max(hits, key=lambda k: d[k][1])
 
 
 def traffic(fetch_csv, write_csv, kernel, copy_bytes, atoms, command):
     fetch = per_kernel(fetch_csv, "FETCH_SIZE")
     write = per_kernel(write_csv, "WRITE_SIZE")
-    # the largest copyBuffer dispatch is the calibration copy (bench.py --copy-check)
-    copies = [k for k in fetch if "copyBuffer" in k or "elementwise_kernel" in k]
-    ck = max(copies, key=lambda k: fetch[k])
-    cw = max((k for k in write if k in copies), key=lambda k: write[k])
-    f_scale = copy_bytes / (fetch[ck] * 1024.0)
-    w_scale = copy_bytes / (write[cw] * 1024.0)
+    # the largest device-to-device copy dispatch is the calibration copy (bench.py
+    # --copy-check): it reads and writes copy_bytes
+    ck = "__amd_rocclr_copyBuffer"
+    if ck not in fetch or ck not in write:
+        raise SystemExit("no copyBuffer dispatch: run the profiled command with --copy-check")
+    f_scale = copy_bytes / (fetch[ck][2] * 1024.0)
+    w_scale = copy_bytes / (write[ck][2] * 1024.0)
     k = find(fetch, kernel)
+    print(f"{'kernel':64s} {'calls':>5s} {'fetch MB':>10s} {'write MB':>10s}", file=sys.stderr)
+    for name in sorted(fetch, key=lambda n: -fetch[n][0]):
+        w = write.get(name, (0.0, 0))[0]
+        print(f"{name[:64]:64s} {fetch[name][1]:5d} {fetch[name][0] * 1024 * f_scale / 1e6:10.1f} "
+              f"{w * 1024 * w_scale / 1e6:10.1f}", file=sys.stderr)
     out = {
         "kernel": k[:120],
-        "fetch_kib": fetch[k],
-        "write_kib": write[find(write, kernel)],
-        "copy_kernel": ck[:80],
+        "fetch_kib": fetch[k][0],
+        "write_kib": write[k][0],
+        "copy_kernel": ck,
+        "copy_fetch_kib": fetch[ck][2],
+        "copy_write_kib": write[ck][2],
         "copy_bytes": copy_bytes,
         "fetch_scale": f_scale,
         "write_scale": w_scale,

@@ -1,11 +1,15 @@
 #!/usr/bin/env python3
-"""Run the parity/benchmark configurations C1, C2, C4, C5 (C3 is bench.py) on one GPU and
-print one JSON record per configuration.
+"""Run the parity/benchmark configurations C1, C2, C4, C5 (C3 is bench.py) and the
+reference-shaped C3 variant on one GPU and print one JSON record per configuration.
 
-  python gpumd-sus2-sh_amd/run_configs.py [--configs c1 c2 c4 c5] [--quick]
+  python tests/run_configs.py [--configs c1 c2 c4 c5 c3ref] [--quick]
 
 Errors are reported against the engine's FP64 path and, where the FP64 CPU oracle finishes
 in seconds, against the oracle.  --quick shrinks every workload for a smoke run.
+
+c3ref is the C3 structure (80x80x40 fcc Cu-Zr, 1,024,000 atoms) under a model of the shape
+the reference's A100 figures are quoted on (body-5 l3333, rb = 10, rc = 7.5 A,
+docs/gpumd-sus2-sh-interface-plan.md:24-40 and :1409), timed as a fixed-cell force loop.
 """
 import argparse
 import json
@@ -14,9 +18,8 @@ import sys
 import tempfile
 import time
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(HERE)
-sys.path.insert(0, HERE)
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "gpumd-sus2-sh_amd"))
 sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
@@ -211,14 +214,56 @@ def run_c5(quick):
                 fp32_vs_oracle_force=float(errs[:, 2].max()))
 
 
+def run_c3ref(quick):
+    import torch
+    import sus2sh
+    from sus2sh import structures as st
+    spec = sus2sh.synthetic_model(3, 3, 5, body_l_max=(3, 3, 3, 3), species_count=2,
+                                  rb_size=10, max_dist=7.5, seed=3003, scaling=0.5)
+    path = _write(spec, "c3ref_l3333_rb10_rc75")
+    nx, ny, nz = (20, 20, 10) if quick else (80, 80, 40)
+    pos, types, cell = st.fcc_alloy(nx, ny, nz, 3.85, [0.5, 0.5], 0.08, 3001, 3002)
+    n = len(types)
+    out = dict(config="C3ref", atoms=n, model="l3333 body5 rb10 rc7.5 seed3003",
+               reference_a100_atom_steps_per_s=1.19500e7)
+    forces = {}
+    for fp32, steps in ((True, 5 if quick else 20), (False, 2 if quick else 5)):
+        pot = sus2sh.SUS2_SH(path, n, ["Cu", "Zr", f"sus2_float={int(fp32)}"])
+        t, x = _dev(types, np.int32), _dev(pos, np.float64)
+        e = torch.zeros(n, dtype=torch.float64, device="cuda")
+        f = torch.zeros(3, n, dtype=torch.float64, device="cuda")
+        v = torch.zeros(9, n, dtype=torch.float64, device="cuda")
+        box = sus2sh.box_struct(cell)
+        for _ in range(2):
+            e.zero_(), f.zero_(), v.zero_()
+            pot.compute(box, t, x, e, f, v)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.zero_(), f.zero_(), v.zero_()
+            pot.compute(box, t, x, e, f, v)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        key = "fp32" if fp32 else "fp64"
+        out[key + "_ms_per_step"] = 1e3 * dt
+        out[key + "_atom_steps_per_s"] = n / dt
+        out["mean_neighbors"] = pot.mean_neighbors()
+        forces[key] = f.cpu().numpy()
+        pot.close()
+    out["fp32_over_fp64"] = out["fp32_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
+    out["max_force_err_fp32_vs_fp64"] = float(np.abs(forces["fp32"] - forces["fp64"]).max())
+    out["vs_reference_a100"] = out["fp32_atom_steps_per_s"] / 1.19500e7
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--configs", nargs="+", default=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--configs", nargs="+", default=["c1", "c2", "c4", "c5", "c3ref"])
     ap.add_argument("--quick", action="store_true")
     args = ap.parse_args()
     import torch
     assert torch.cuda.is_available(), "run_configs.py needs a HIP device"
-    runners = dict(c1=run_c1, c2=run_c2, c4=run_c4, c5=run_c5)
+    runners = dict(c1=run_c1, c2=run_c2, c4=run_c4, c5=run_c5, c3ref=run_c3ref)
     for c in args.configs:
         log(f"[config] {c} start")
         t0 = time.perf_counter()

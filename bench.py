@@ -107,6 +107,52 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
     return out
 
 
+def run_graph_loop(path, cell, types, pos, precision, steps, warmup):
+    """The same fixed-cell force loop replayed from one captured HIP graph (zero outputs,
+    evaluate): deferred neighbour-capacity check, one sus2sh_check after the loop."""
+    import torch
+    import sus2sh
+    n = len(types)
+    opts = ["Cu", "Zr", "sus2_float=%d" % (1 if precision == "fp32" else 0),
+            "sus2_sh_deferred_check=1"]
+    pot = sus2sh.SUS2_SH(path, n, opts)
+    dev = "cuda"
+    t = torch.as_tensor(types, device=dev)
+    x = torch.as_tensor(pos, device=dev).contiguous()
+    e = torch.zeros(n, dtype=torch.float64, device=dev)
+    f = torch.zeros(3, n, dtype=torch.float64, device=dev)
+    v = torch.zeros(9, n, dtype=torch.float64, device=dev)
+    box = sus2sh.box_struct(cell)
+
+    def step():
+        e.zero_()
+        f.zero_()
+        v.zero_()
+        pot.compute(box, t, x, e, f, v)
+
+    step()  # synchronised sizing call (and the autotuner)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        step()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pot.check()
+    out = dict(atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps, steps=steps,
+               force=f.cpu().numpy())
+    del graph
+    pot.close()
+    return out
+
+
 def cpu_baseline(path, seconds=12.0):
     """FP64 C/OpenMP oracle (a port of the same algorithm) on a bounded sample of the same
     workload: a 20x20x20-cell (32,000 atom) piece with the same lattice, composition and
@@ -138,6 +184,9 @@ def main():
     ap.add_argument("--cells", type=int, nargs=3, default=[80, 80, 40])
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=50,
+                    help="steps of the HIP-graph-replayed FP32 loop reported beside the eager "
+                         "loop (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before the runs, copy MIB MiB device-to-device (untimed): the plain "
@@ -174,6 +223,12 @@ def main():
     log(f"[bench] N={N}; FP32 run: {args.warmup} warmup + {args.steps} timed steps")
     r32 = run_engine(path, cell, types, pos, "fp32", args.steps, args.warmup, profile=True)
     log(f"[bench] fp32 {r32['atom_steps_per_s']:.4e} atom-steps/s, stages {r32.get('stage_ms')}")
+    graph32 = None
+    if args.graph_steps > 0:
+        log(f"[bench] FP32 graph-replayed loop: {args.graph_steps} steps")
+        graph32 = run_graph_loop(path, cell, types, pos, "fp32", args.graph_steps, 2)
+        log(f"[bench] fp32 graph {graph32['atom_steps_per_s']:.4e} atom-steps/s")
+        graph32["max_force_dev_vs_eager"] = float(np.abs(graph32.pop("force") - r32["force"]).max())
     log(f"[bench] FP64 run: 2 warmup + {args.fp64_steps} timed steps")
     r64 = run_engine(path, cell, types, pos, "fp64", args.fp64_steps, 2, profile=True)
     log(f"[bench] fp64 {r64['atom_steps_per_s']:.4e} atom-steps/s, stages {r64.get('stage_ms')}")
@@ -237,6 +292,7 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "flops_per_atom": flops[dominant],
                      "launch_ms": stages[dominant]},
+        "graph_replay_fp32": graph32,
         "cpu_baseline": cpu,
         "vs_baseline_note": "reference A100 figure 5.23633e6 is a whole GPUMD NPT step with an "
                             "rc=7.5 rb=10 l3333 glass model",

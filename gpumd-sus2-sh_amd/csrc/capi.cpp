@@ -39,6 +39,9 @@ int guarded(std::string* err, F&& fn)
   } catch (const sus2sh::DeviceError& e) {
     *err = e.what();
     return SUS2SH_ERR_DEVICE;
+  } catch (const sus2sh::CapacityError& e) {
+    *err = e.what();
+    return SUS2SH_ERR_CAPACITY;
   } catch (const std::bad_alloc&) {
     *err = "host allocation failed";
     return SUS2SH_ERR_DEVICE;
@@ -128,6 +131,13 @@ int sus2sh_compute_batch(sus2sh_handle* handle, int num_structures, const sus2sh
                                   d_potential, d_force, d_virial,
                                   static_cast<hipStream_t>(stream), d_param_grad);
   });
+}
+
+int sus2sh_check(sus2sh_handle* handle, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&handle->error,
+                 [&] { handle->engine->check(static_cast<hipStream_t>(stream)); });
 }
 
 int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,

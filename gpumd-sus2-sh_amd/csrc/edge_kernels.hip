@@ -412,9 +412,7 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
 #define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV)                                           \
   {                                                                                            \
     auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV>;                                       \
-    if (lds > 65536)                                                                           \
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                           \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)); \
+    if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV>>();               \
     kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
   }
 #define SUS2SH_FORCES_CASE(LL, KK)                                                           \

@@ -56,6 +56,10 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       if (w != 1 && w != 2 && w != 4 && w != 8)
         throw InputError("sus2_sh_lane_waves must be 1, 2, 4 or 8: " + val);
       o.lane_waves = w;  // fixes the lane program shape (4 accumulators per wave)
+    } else if (key == "sus2_sh_deferred_check") {
+      o.deferred_check = parse_bool(val, key);
+    } else if (key == "sus2_sh_md_graph") {
+      o.md_graph = parse_bool(val, key);
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -159,14 +163,19 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     upload_tables<float>();
   else
     upload_tables<double>();
-  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_max_), 4 * sizeof(int), 0),
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_max_), 8 * sizeof(int), 0),
             "hipHostMalloc");
+  latch_.resize(8 * sizeof(int));
+  launch_fill_int(latch_.as<int>(), 0, 8, nullptr);
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   (void)num_atoms;
 }
 
 Engine::~Engine()
 {
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (md_event_) (void)hipEventDestroy(md_event_);
+  if (md_stream_) (void)hipStreamDestroy(md_stream_);
   if (host_max_) (void)hipHostFree(host_max_);
 }
 
@@ -332,7 +341,8 @@ void Engine::ensure_atoms(int N, size_t edge_bytes)
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
   if (N > atoms_alloc_) {
     key_.resize(n * sizeof(int));
-    val_.resize(n * sizeof(int));
+    slot_.resize(n * sizeof(int));
+    arrival_.resize(n * sizeof(int));
     skey_.resize(n * sizeof(int));
     satom_.resize(n * sizeof(int));
     wrapped_.resize(3 * n * sizeof(double));
@@ -465,6 +475,10 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (N >= kMaxEdgeAtoms)
     throw ArgumentError("SUS2_SH: at most 134217727 atoms per call (neighbour index width)");
   const bool batch = nstruct > 1;
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
+  const bool capturing = capture != hipStreamCaptureStatusNone;
+  if (capturing && batch) throw ArgumentError("SUS2_SH: batched evaluations cannot be captured");
   std::vector<CellGrid> grids(nstruct);
   std::vector<int> cell_off(nstruct + 1, 0);
   for (int st = 0; st < nstruct; ++st) {
@@ -482,35 +496,62 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     sid_.resize(static_cast<size_t>(N) * sizeof(int));
     sid_s_.resize(static_cast<size_t>(N) * sizeof(int));
   }
+  // deferred: no host synchronisation; the neighbour capacity was validated by an earlier
+  // synchronised call with the same atom count, and overflows are latched for check()
+  const bool deferred = (opt_.deferred_check || md_capture_) && cap_known_ && N == cap_atoms_ &&
+                        !batch && !param_grad;
+  if (capturing && !deferred)
+    throw ArgumentError(
+      "SUS2_SH: a captured evaluation needs sus2_sh_deferred_check=1, one structure, no "
+      "parameter gradient, and a preceding uncaptured call with the same atom count");
+  const bool profile = opt_.profile && !capturing;
   if (cap_ == 0) cap_ = 64;
+  const int n_totals = static_cast<int>(cell_sort_totals(ncell));
+  if (capturing && (N > atoms_alloc_ || nl_.bytes() < static_cast<size_t>(cap_) * N * sizeof(Edge<Real>) ||
+                    cell_begin_.bytes() < (static_cast<size_t>(ncell) + 1) * sizeof(int) ||
+                    scan_totals_.bytes() < static_cast<size_t>(n_totals) * sizeof(int)))
+    throw ArgumentError("SUS2_SH: a captured evaluation cannot resize work buffers (the atom "
+                        "count or the cell grid grew since the last uncaptured call)");
   ensure_atoms(N, sizeof(Edge<Real>));
-  cell_begin_.resize(static_cast<size_t>(ncell) * sizeof(int));
-  cell_end_.resize(static_cast<size_t>(ncell) * sizeof(int));
-  int bits = 1;
-  while ((1L << bits) < ncell) ++bits;
-  const size_t temp = radix_sort_temp_bytes(N, bits);
-  sort_temp_.resize(std::max<size_t>(temp, 16));
+  cell_count_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
+  cell_begin_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
+  scan_totals_.resize(static_cast<size_t>(n_totals) * sizeof(int));
 
-  if (opt_.profile && events_.empty()) {
+  if (profile && events_.empty()) {
     events_.resize(6);
     for (hipEvent_t& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
   }
-  if (opt_.profile) hip_check(hipEventRecord(events_[0], s), "hipEventRecord");
+  if (profile) hip_check(hipEventRecord(events_[0], s), "hipEventRecord");
 
-  launch_bin(N, pos, grid, batch ? batch_grids_.as<CellGrid>() : nullptr,
-             batch ? batch_cell_off_.as<int>() : nullptr,
-             batch ? batch_atom_off_.as<int>() : nullptr, nstruct, key_.as<int>(), val_.as<int>(),
-             wrapped_.as<double>(), batch ? sid_.as<int>() : nullptr, s);
-  hip_check(radix_sort_pairs(sort_temp_.as<void>(), temp, key_.as<int>(), skey_.as<int>(),
-                             val_.as<int>(), satom_.as<int>(), N, bits, s),
-            "radix sort");
-  launch_fill_int(cell_begin_.as<int>(), 0, ncell, s);
-  launch_fill_int(cell_end_.as<int>(), 0, ncell, s);
   launch_fill_int(max_count_.as<int>(), 0, 4, s);
-  launch_gather(N, model_.species_count, skey_.as<int>(), satom_.as<int>(), wrapped_.as<double>(),
-                type, pos_s_.as<double>(), type_s_.as<int>(), cell_begin_.as<int>(),
-                cell_end_.as<int>(), max_count_.as<int>() + 1, batch ? sid_.as<int>() : nullptr,
-                batch ? sid_s_.as<int>() : nullptr, s);
+  CellSortArgs cs{};
+  cs.N = N;
+  cs.S = model_.species_count;
+  cs.ncell = ncell;
+  cs.pos = pos;
+  cs.type = type;
+  cs.grid = grid;
+  cs.grids = batch ? batch_grids_.as<CellGrid>() : nullptr;
+  cs.cell_off = batch ? batch_cell_off_.as<int>() : nullptr;
+  cs.atom_off = batch ? batch_atom_off_.as<int>() : nullptr;
+  cs.nstruct = nstruct;
+  cs.key = key_.as<int>();
+  cs.slot = slot_.as<int>();
+  cs.arrival = arrival_.as<int>();
+  cs.cell_count = cell_count_.as<int>();
+  cs.cell_begin = cell_begin_.as<int>();
+  cs.scan_totals = scan_totals_.as<int>();
+  cs.wrapped = wrapped_.as<double>();
+  cs.sid = batch ? sid_.as<int>() : nullptr;
+  cs.satom = satom_.as<int>();
+  cs.skey = skey_.as<int>();
+  cs.pos_s = pos_s_.as<double>();
+  cs.type_s = type_s_.as<int>();
+  cs.bad_type = max_count_.as<int>() + 1;
+  cs.sid_s = batch ? sid_s_.as<int>() : nullptr;
+  launch_cell_sort(cs, s);
+  const int* cell_begin = cell_begin_.as<int>();
+  const int* cell_end = cell_begin + 1;
   NeighborArgs na{};
   na.N = N;
   na.grid = grid;
@@ -518,29 +559,41 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   na.cell_off = batch ? batch_cell_off_.as<int>() : nullptr;
   na.sid_s = batch ? sid_s_.as<int>() : nullptr;
   na.rc2 = model_.max_dist * model_.max_dist;
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  if (deferred) {
     na.cap = cap_;
-    if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
-    launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
     launch_neighbors<Real>(na, pos_s_.as<double>(), type_s_.as<int>(), skey_.as<int>(),
-                           cell_begin_.as<int>(), cell_end_.as<int>(), count_.as<int>(),
-                           nl_.as<Edge<Real>>(), max_count_.as<int>(), s);
-    hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),
-                             hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync D2H");
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    const int maxc = host_max_[0];
-    const unsigned long long total =
-      static_cast<unsigned long long>(static_cast<unsigned>(host_max_[2])) |
-      (static_cast<unsigned long long>(static_cast<unsigned>(host_max_[3])) << 32);
-    mean_neighbors_ = static_cast<double>(total) / N;
-    if (host_max_[1] != 0)
-      throw ArgumentError("atom type outside [0, species_count) passed to compute");
-    if (maxc <= cap_) break;
-    cap_ = ((maxc + 8 + 7) / 8) * 8;
-    ensure_atoms(N, sizeof(Edge<Real>));
+                           cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
+                           max_count_.as<int>(), s);
+    launch_latch(max_count_.as<int>(), latch_.as<int>(), s);
+  } else {
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      na.cap = cap_;
+      if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
+      launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
+      launch_neighbors<Real>(na, pos_s_.as<double>(), type_s_.as<int>(), skey_.as<int>(),
+                             cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
+                             max_count_.as<int>(), s);
+      hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),
+                               hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync D2H");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+      const int maxc = host_max_[0];
+      const unsigned long long total =
+        static_cast<unsigned long long>(static_cast<unsigned>(host_max_[2])) |
+        (static_cast<unsigned long long>(static_cast<unsigned>(host_max_[3])) << 32);
+      mean_neighbors_ = static_cast<double>(total) / N;
+      if (host_max_[1] != 0)
+        throw ArgumentError("atom type outside [0, species_count) passed to compute");
+      if (maxc <= cap_) break;
+      cap_ = ((maxc + 8 + 7) / 8) * 8;
+      ensure_atoms(N, sizeof(Edge<Real>));
+    }
+    if (!batch) {
+      cap_known_ = true;
+      cap_atoms_ = N;
+    }
   }
-  if (opt_.profile) hip_check(hipEventRecord(events_[1], s), "hipEventRecord");
+  if (profile) hip_check(hipEventRecord(events_[1], s), "hipEventRecord");
 
   EdgeLaunch<Real> el{};
   el.N = N;
@@ -560,7 +613,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.fmap = fmap_.as<int>();
   if (!launch_basics<Real>(el, basics_.as<Real>(), s))
     throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
-  if (opt_.profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
+  if (profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
 
   GraphTable<Real> gt{};
   gt.fwd_rows = fwd_rows_.as<int4>();
@@ -599,7 +652,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     pgo.Q = Q;
     pgo.S = S;
   }
-  if (opt_.autotune && !tuned_ && !param_grad && N >= 4096) {
+  if (opt_.autotune && !tuned_ && !param_grad && !capturing && N >= 4096) {
     // Autotuner: time every feasible candidate -- (variant, tile) of the workgroup kernel
     // and the lane program -- and keep the fastest (HIP events on the engine's stream; the
     // graph kernels are idempotent, so the trial launches only rewrite e_s and grads).
@@ -667,7 +720,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   } else if (!launch_graph_variant<Real>(variant_, tile_, N, gt, s)) {
     throw UnsupportedError("SUS2_SH: graph kernel variant does not fit in LDS.");
   }
-  if (opt_.profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
+  if (profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
   // neighbour windows of the force tiles (single structure): LDS accumulation of the
   // neighbour forces, budget 48 KiB (FP32) / 64 KiB (FP64) of accumulators per tile
@@ -684,13 +737,13 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(),
                            param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
-  if (opt_.profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
+  if (profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
   launch_scatter(N, satom_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
                  pot, force, virial, s);
   hip_check(hipGetLastError(), "kernel launch");
   if (param_grad) finish_param_grad(param_grad, s);
-  if (opt_.profile) {
+  if (profile) {
     hip_check(hipEventRecord(events_[5], s), "hipEventRecord");
     hip_check(hipEventSynchronize(events_[5]), "hipEventSynchronize");
     stage_ms_.assign(5, 0.0f);
@@ -736,6 +789,38 @@ void Engine::set_params(const double* theta, int n)
     upload_tables<double>();
 }
 
+void Engine::reset_latch(hipStream_t s) { launch_fill_int(latch_.as<int>(), 0, 8, s); }
+
+void Engine::read_latch(hipStream_t s, int* out)
+{
+  hip_check(hipMemcpyAsync(host_max_, latch_.as<int>(), 8 * sizeof(int), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync D2H");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  std::memcpy(out, host_max_, 8 * sizeof(int));
+}
+
+void Engine::check(hipStream_t s)
+{
+  int l[8];
+  read_latch(s, l);
+  reset_latch(s);
+  if (l[4] == 0) return;
+  const unsigned long long total = static_cast<unsigned long long>(static_cast<unsigned>(l[2])) |
+                                   (static_cast<unsigned long long>(static_cast<unsigned>(l[3])) << 32);
+  if (cap_atoms_ > 0) mean_neighbors_ = static_cast<double>(total) / cap_atoms_;
+  if (l[1] != 0) throw ArgumentError("atom type outside [0, species_count) passed to compute");
+  if (l[0] > cap_) {
+    std::ostringstream msg;
+    msg << "SUS2_SH: neighbour capacity exceeded in a deferred-check evaluation (" << l[0]
+        << " neighbours, capacity " << cap_ << "): the outputs of the " << l[4]
+        << " evaluation(s) since the previous check miss neighbours; the capacity is raised "
+           "for the next (synchronised) call";
+    cap_ = ((l[0] + 8 + 7) / 8) * 8;
+    cap_known_ = false;
+    throw CapacityError(msg.str());
+  }
+}
+
 void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double* masses,
                     double* pos, double* vel, double dt_fs, int nsteps, double* pot, double* force,
                     double* virial, double* energy_trace, hipStream_t s)
@@ -745,26 +830,120 @@ void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double*
   const int S = model_.species_count;
   for (int t = 0; t < S; ++t)
     if (!(masses[t] > 0.0)) throw ArgumentError("md_nve: masses must be positive");
+  // graph mode runs on the engine's own stream (the legacy default stream cannot be
+  // captured), ordered after the caller's stream and back
+  const hipStream_t caller = s;
+  if (opt_.md_graph) {
+    if (!md_stream_) hip_check(hipStreamCreateWithFlags(&md_stream_, hipStreamNonBlocking),
+                               "hipStreamCreateWithFlags");
+    if (!md_event_) hip_check(hipEventCreateWithFlags(&md_event_, hipEventDisableTiming),
+                              "hipEventCreate");
+    hip_check(hipEventRecord(md_event_, caller), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(md_stream_, md_event_, 0), "hipStreamWaitEvent");
+    s = md_stream_;
+  }
   md_mass_.upload(masses, S * sizeof(double));
   md_energy_.resize(2 * static_cast<size_t>(nsteps + 1) * sizeof(double));
+  md_step_.resize(sizeof(int));
   double* en = md_energy_.as<double>();
+  const double* mass = md_mass_.as<double>();
+  int* step = md_step_.as<int>();
   launch_fill_double(en, 0.0, 2 * static_cast<size_t>(nsteps + 1), s);
   const size_t n = static_cast<size_t>(N);
-  auto evaluate = [&](int k) {
+  auto evaluate = [&]() {
     launch_fill_double(pot, 0.0, n, s);
     launch_fill_double(force, 0.0, 3 * n, s);
     launch_fill_double(virial, 0.0, 9 * n, s);
     compute(box, N, type, pos, pot, force, virial, s, nullptr);
-    (void)k;
   };
-  evaluate(0);
-  launch_energies(N, type, md_mass_.as<double>(), pot, vel, en, s);
-  for (int k = 1; k <= nsteps; ++k) {
-    launch_half_kick(N, 0.5 * dt_fs, type, md_mass_.as<double>(), force, vel, s);
+  // one velocity-Verlet step; the energies go to en[2 * step] and the counter advances
+  auto md_step = [&]() {
+    launch_half_kick(N, 0.5 * dt_fs, type, mass, force, vel, s);
     launch_drift(N, dt_fs, vel, pos, s);
-    evaluate(k);
-    launch_half_kick(N, 0.5 * dt_fs, type, md_mass_.as<double>(), force, vel, s);
-    launch_energies(N, type, md_mass_.as<double>(), pot, vel, en + 2 * k, s);
+    evaluate();
+    launch_half_kick(N, 0.5 * dt_fs, type, mass, force, vel, s);
+    launch_energies(N, type, mass, pot, vel, en, step, s);
+    launch_step_advance(step, s);
+  };
+  evaluate();  // synchronised: sizes the neighbour capacity
+  launch_fill_int(step, 0, 1, s);
+  launch_energies(N, type, mass, pot, vel, en, step, s);
+  launch_step_advance(step, s);
+  if (!opt_.md_graph) {
+    for (int k = 1; k <= nsteps; ++k) md_step();
+  } else {
+    // Steady steps replay one captured step (no host synchronisation inside).  Chunks of
+    // kChunk steps start from a snapshot of positions and velocities; a chunk whose
+    // neighbour counts overflowed the capacity (latched on the device) is rolled back and
+    // run again with a larger capacity.
+    constexpr int kChunk = 50;
+    md_x0_.resize(3 * n * sizeof(double));
+    md_v0_.resize(3 * n * sizeof(double));
+    hipGraphExec_t exec = nullptr;
+    auto drop = [&]() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+    };
+    try {
+      int done = 0, retries = 0;
+      while (done < nsteps) {
+        const int chunk = std::min(kChunk, nsteps - done);
+        hip_check(hipMemcpyAsync(md_x0_.as<double>(), pos, 3 * n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s), "hipMemcpyAsync D2D");
+        hip_check(hipMemcpyAsync(md_v0_.as<double>(), vel, 3 * n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s), "hipMemcpyAsync D2D");
+        reset_latch(s);
+        if (!exec) {
+          hipGraph_t graph = nullptr;
+          hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
+                    "hipStreamBeginCapture");
+          md_capture_ = true;
+          try {
+            md_step();
+          } catch (...) {
+            md_capture_ = false;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          md_capture_ = false;
+          hip_check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+          const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          hip_check(e, "hipGraphInstantiate");
+        }
+        for (int k = 0; k < chunk; ++k) hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch");
+        int l[8];
+        read_latch(s, l);
+        if (l[1] != 0) throw ArgumentError("atom type outside [0, species_count) passed to compute");
+        if (l[0] > cap_) {
+          if (++retries > 8) throw DeviceError("md_nve: neighbour capacity did not settle");
+          // roll back to the chunk start and size the capacity with a synchronised call
+          cap_ = ((l[0] + 8 + 7) / 8) * 8;
+          cap_known_ = false;
+          drop();
+          hip_check(hipMemcpyAsync(pos, md_x0_.as<double>(), 3 * n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s), "hipMemcpyAsync D2D");
+          hip_check(hipMemcpyAsync(vel, md_v0_.as<double>(), 3 * n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s), "hipMemcpyAsync D2D");
+          launch_fill_int(step, done + 1, 1, s);
+          launch_fill_double(en + 2 * static_cast<size_t>(done + 1), 0.0, 2 * static_cast<size_t>(chunk), s);
+          evaluate();
+          continue;
+        }
+        const unsigned long long total =
+          static_cast<unsigned long long>(static_cast<unsigned>(l[2])) |
+          (static_cast<unsigned long long>(static_cast<unsigned>(l[3])) << 32);
+        mean_neighbors_ = static_cast<double>(total) / N;
+        done += chunk;
+      }
+    } catch (...) {
+      md_capture_ = false;
+      drop();
+      throw;
+    }
+    drop();
   }
   hip_check(hipGetLastError(), "md kernel launch");
   if (energy_trace) {
@@ -772,6 +951,10 @@ void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double*
                              hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+  if (s != caller) {
+    hip_check(hipEventRecord(md_event_, s), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(caller, md_event_, 0), "hipStreamWaitEvent");
   }
 }
 

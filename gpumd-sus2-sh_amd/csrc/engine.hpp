@@ -1,7 +1,7 @@
 // Host engine behind the C ABI: owns the model, its device tables and the per-atom work
 // buffers, and runs one evaluation as a fixed kernel sequence on the caller's stream:
-//   bin -> radix sort -> cell ranges -> neighbours -> basic moments -> CG graph
-//   -> forces/virial -> scatter to caller order.
+//   cell sort -> neighbours -> basic moments -> CG graph -> forces/virial -> scatter to
+//   caller order.
 // Mirrors SUS2_SH (src/force/sus2_sh.cuh:7-203): constructor = model load + table upload
 // (src/force/sus2_sh.cu:7782-9467), compute = src/force/sus2_sh.cu:9620-10675.
 #pragma once
@@ -62,6 +62,14 @@ struct EngineOptions {
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
   int lane_waves = 0;    // lane kernel: waves per 64-atom tile (0: autotuned shape)
+  // no host synchronisation in steady calls (capturable); the neighbour-capacity and type
+  // checks are latched on the device and reported by check()
+  bool deferred_check = false;
+  bool md_graph = true;  // md_nve replays steady steps from a captured HIP graph
+};
+
+struct CapacityError : public std::runtime_error {
+  explicit CapacityError(const std::string& msg) : std::runtime_error(msg) {}
 };
 
 class Engine {
@@ -82,6 +90,11 @@ class Engine {
   void md_nve(const sus2sh_box& box, int N, const int* type, const double* masses, double* pos,
               double* vel, double dt_fs, int nsteps, double* pot, double* force, double* virial,
               double* energy_trace, hipStream_t stream);
+
+  // Synchronise `stream` and report the checks latched by deferred-check evaluations since
+  // the previous check (neighbour capacity: CapacityError, with the capacity raised for the
+  // next call; atom type out of range: ArgumentError).
+  void check(hipStream_t stream);
 
   // Replace the model coefficients (dE/dtheta order) and re-upload the device tables.
   void set_params(const double* theta, int n);
@@ -119,6 +132,8 @@ class Engine {
            const double* pos, double* pot, double* force, double* virial, hipStream_t stream,
            double* param_grad);
   void ensure_atoms(int N, size_t edge_bytes);
+  void reset_latch(hipStream_t s);
+  void read_latch(hipStream_t s, int* out);
   void finish_param_grad(double* d_param_grad, hipStream_t s);
   CellGrid make_grid(const sus2sh_box& box, int N, int* ncell) const;
 
@@ -133,6 +148,9 @@ class Engine {
   std::vector<float> tune_ms_;
   LaneProgram lane_;
   int cap_ = 0;
+  bool cap_known_ = false;  // cap_ was validated by a synchronised call with cap_atoms_ atoms
+  int cap_atoms_ = -1;
+  bool md_capture_ = false;  // md_nve is capturing a step: evaluations take the deferred path
   size_t max_lds_ = 65536;
   int device_ = 0;
 
@@ -151,12 +169,16 @@ class Engine {
 
   // per-atom work buffers
   int atoms_alloc_ = 0;
-  DeviceBuffer key_, val_, skey_, satom_, wrapped_, pos_s_, type_s_, count_;
-  DeviceBuffer cell_begin_, cell_end_, sort_temp_, max_count_;
+  DeviceBuffer key_, slot_, arrival_, skey_, satom_, wrapped_, pos_s_, type_s_, count_;
+  DeviceBuffer cell_count_, cell_begin_, scan_totals_, max_count_;
+  DeviceBuffer latch_;  // deferred checks: max count, bad type, total edges (2), evaluations
+  DeviceBuffer md_step_, md_x0_, md_v0_;
+  hipStream_t md_stream_ = nullptr;  // md_nve graph mode (default priority)
+  hipEvent_t md_event_ = nullptr;
   DeviceBuffer nl_, basics_, grads_, e_s_, f_s_, v_s_;
   DeviceBuffer pg_rad_, pg_misc_, md_mass_, md_energy_;
   DeviceBuffer batch_grids_, batch_cell_off_, batch_atom_off_, sid_, sid_s_;
-  int* host_max_ = nullptr;  // pinned
+  int* host_max_ = nullptr;  // pinned, 8 ints
 
   std::vector<float> stage_ms_;
   std::vector<hipEvent_t> events_;

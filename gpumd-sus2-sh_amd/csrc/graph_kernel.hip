@@ -284,9 +284,8 @@ void launch_variant(int N, const Real* basics, const int* type_s, const GraphTab
                     size_t max_lds, int blocks)
 {
   auto kern = &k_graph<Real, W, NT, PG, VM>;
-  if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds));
+  if (lds > 65536) allow_large_lds<&k_graph<Real, W, NT, PG, VM>>();
+  (void)max_lds;
   kern<<<blocks, NT, lds, s>>>(N, basics, type_s, gt, e_s, grads, pg);
 }
 

@@ -12,6 +12,21 @@
 
 namespace sus2sh {
 
+// Raise a kernel's dynamic-LDS limit to the gfx950 maximum (160 KiB) once per kernel, on its
+// first (uncaptured) launch, so that later launches -- also those recorded into a HIP
+// graph -- make no attribute call.
+constexpr int kMaxDynamicLds = 160 * 1024;
+template <auto Kernel>
+inline void allow_large_lds()
+{
+  static const bool done = [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(Kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds);
+    return true;
+  }();
+  (void)done;
+}
+
 // Graph program tables on the device.
 template <typename Real>
 struct GraphTable {
@@ -66,18 +81,42 @@ struct NeighborArgs {
 };
 
 // --- neighbour build (neighbor.hip) ---
-// grids == nullptr: one structure with grid g; otherwise atom_off [nstruct+1] (device) maps
-// atoms to structures, and sid receives each atom's structure.
-void launch_bin(int N, const double* pos, const CellGrid& g, const CellGrid* grids,
-                const int* cell_off, const int* atom_off, int nstruct, int* cell_key,
-                int* atom_val, double* wrapped, int* sid, hipStream_t s);
+// Cell sort: wrap into the box, cell key, counting sort by key (order inside a cell by atom
+// index), gather of wrapped positions and types into the sorted order.  grids == nullptr:
+// one structure with grid `grid`; otherwise atom_off [nstruct+1] (device) maps atoms to
+// structures, sid receives each atom's structure and sid_s the same in sorted order.
+struct CellSortArgs {
+  int N;
+  int S;             // species count; bad_type is set to 1 for a type outside [0, S)
+  int ncell;
+  const double* pos;
+  const int* type;
+  CellGrid grid;
+  const CellGrid* grids;
+  const int* cell_off;
+  const int* atom_off;
+  int nstruct;
+  int* key;          // [N] cell of atom i
+  int* slot;         // [N] arrival rank of atom i in its cell
+  int* arrival;      // [N] atoms in cell order, arrival order inside a cell
+  int* cell_count;   // [ncell + 1]
+  int* cell_begin;   // [ncell + 1]: cell k holds sorted atoms [cell_begin[k], cell_begin[k+1])
+  int* scan_totals;  // [cell_sort_totals(ncell)]
+  double* wrapped;   // [3][N]
+  int* sid;
+  int* satom;        // [N] atom at sorted position p
+  int* skey;         // [N] cell of sorted position p
+  double* pos_s;     // [3][N]
+  int* type_s;       // [N]
+  int* bad_type;
+  int* sid_s;
+};
+void launch_cell_sort(const CellSortArgs& a, hipStream_t s);
+size_t cell_sort_totals(int ncell);
+// latch (device int[8]) accumulates max_count[0..3] of a deferred-check call
+void launch_latch(const int* max_count, int* latch, hipStream_t s);
 void launch_fill_int(int* p, int value, size_t n, hipStream_t s);
 void launch_fill_double(double* p, double value, size_t n, hipStream_t s);
-// bad_type: set to 1 when a type is outside [0, S) (clamped to 0 for the computation)
-void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
-                   const double* wrapped, const int* type, double* pos_s, int* type_s,
-                   int* cell_begin, int* cell_end, int* bad_type, const int* sid, int* sid_s,
-                   hipStream_t s);
 template <typename Real>
 void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* type_s,
                       const int* sorted_key, const int* cell_begin, const int* cell_end,
@@ -91,9 +130,6 @@ constexpr int kWinMax = 64;
 constexpr int kWinStride = 2 + 2 * kWinMax;
 void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,
                     hipStream_t s);
-size_t radix_sort_temp_bytes(int N, int bits);
-hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
-                            const int* vals_in, int* vals_out, int N, int bits, hipStream_t s);
 
 // --- edge kernels (edge_kernels.hip) ---
 template <typename Real>
@@ -159,8 +195,11 @@ bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneT
 void launch_half_kick(int N, double half_dt, const int* type, const double* mass, const double* f,
                       double* v, hipStream_t s);
 void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s);
+// step == nullptr: out[0..1] += (potential, kinetic); otherwise out[2 * *step ..] +=, and
+// launch_step_advance(step) increments the device step counter
+void launch_step_advance(int* step, hipStream_t s);
 void launch_energies(int N, const int* type, const double* mass, const double* pot,
-                     const double* v, double* out, hipStream_t s);
+                     const double* v, double* out, const int* step, hipStream_t s);
 
 // --- scatter of per-sorted-atom results to the caller's arrays ---
 void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,

@@ -185,9 +185,7 @@ void launch_lane_shape(int N, const Real* basics, const int* type_s, const LaneT
                        double* e_s, Real* grads, hipStream_t s, size_t lds)
 {
   auto kern = &k_lane_graph<Real, NS, WPT>;
-  if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (lds > 65536) allow_large_lds<&k_lane_graph<Real, NS, WPT>>();
   const unsigned blocks = (static_cast<unsigned>(N) + kLaneWave - 1) / kLaneWave;
   kern<<<blocks, kLaneWave * WPT, lds, s>>>(N, basics, type_s, lt, e_s, grads);
 }

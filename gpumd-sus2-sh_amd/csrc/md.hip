@@ -32,10 +32,11 @@ __global__ void k_drift(int N, double dt, const double* __restrict__ v, double* 
   x[2 * N + i] += dt * v[2 * N + i];
 }
 
-// out[0] += sum potential, out[1] += sum kinetic (one atomic pair per workgroup)
+// out[0] += sum potential, out[1] += sum kinetic (one atomic pair per workgroup); with a
+// device step counter the pair is out[2 * step], out[2 * step + 1]
 __global__ void k_energies(int N, const int* __restrict__ type, const double* __restrict__ mass,
                            const double* __restrict__ pot, const double* __restrict__ v,
-                           double* __restrict__ out)
+                           double* __restrict__ out, const int* __restrict__ step)
 {
   __shared__ double sp[4], sk[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,6 +56,7 @@ __global__ void k_energies(int N, const int* __restrict__ type, const double* __
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    if (step) out += 2 * static_cast<size_t>(*step);
     double a = 0.0, b = 0.0;
     for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) {
       a += sp[w];
@@ -63,6 +65,11 @@ __global__ void k_energies(int N, const int* __restrict__ type, const double* __
     unsafeAtomicAdd(out, a);
     unsafeAtomicAdd(out + 1, b);
   }
+}
+
+__global__ void k_step_advance(int* step)
+{
+  if (threadIdx.x == 0) *step += 1;
 }
 
 inline int nblocks(int n) { return (n + 255) / 256; }
@@ -81,9 +88,11 @@ void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s)
 }
 
 void launch_energies(int N, const int* type, const double* mass, const double* pot,
-                     const double* v, double* out, hipStream_t s)
+                     const double* v, double* out, const int* step, hipStream_t s)
 {
-  k_energies<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, out);
+  k_energies<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, out, step);
 }
+
+void launch_step_advance(int* step, hipStream_t s) { k_step_advance<<<1, 64, 0, s>>>(step); }
 
 }  // namespace sus2sh

@@ -6,10 +6,11 @@
 // t_d, every image within rc lies within R_d = ceil(rc n_d / t_d) cells of the home cell,
 // and a small cell (n_d < 2 R_d + 1) is simply visited once per image shift.
 //
-// Atoms are sorted by cell (hipCUB radix sort, deterministic), so every later kernel runs
-// in cell order: neighbours of a wavefront's atoms sit in the same few cache lines.
-#include <hipcub/hipcub.hpp>
-
+// Atoms are sorted by cell, so every later kernel runs in cell order: neighbours of a
+// wavefront's atoms sit in the same few cache lines.  The sort is a counting sort over cell
+// keys (count, scan, place) whose order inside a cell is by atom index -- the permutation a
+// stable sort of (key, index) gives, independent of atomic ordering -- and it uses kernels
+// only (no memset or library call), so an evaluation can be captured in a HIP graph.
 #include "kernels.hpp"
 
 namespace sus2sh {
@@ -36,7 +37,8 @@ template <bool BATCH>
 __global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g1,
                       const CellGrid* __restrict__ grids, const int* __restrict__ cell_off,
                       const int* __restrict__ atom_off, int nstruct, int* __restrict__ key,
-                      int* __restrict__ val, double* __restrict__ wrapped, int* __restrict__ sid)
+                      int* __restrict__ slot, int* __restrict__ cell_count,
+                      double* __restrict__ wrapped, int* __restrict__ sid)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -63,8 +65,95 @@ __global__ void k_bin(int N, const double* __restrict__ pos, CellGrid g1,
   wrapped[i] = g.h[0] * s[0] + g.h[1] * s[1] + g.h[2] * s[2];
   wrapped[N + i] = g.h[3] * s[0] + g.h[4] * s[1] + g.h[5] * s[2];
   wrapped[2 * N + i] = g.h[6] * s[0] + g.h[7] * s[1] + g.h[8] * s[2];
-  key[i] = (BATCH ? cell_off[st] : 0) + (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
-  val[i] = i;
+  const int k = (BATCH ? cell_off[st] : 0) + (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+  key[i] = k;
+  slot[i] = atomicAdd(cell_count + k, 1);  // arrival order; made deterministic by k_rank
+}
+
+// Exclusive scan of cell counts: per 2048-entry tile (256 threads x 8), tile totals scanned
+// by one workgroup, totals added back.
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int wave_inclusive_scan(int x)
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(x, off, 64);
+    if (lane >= off) x += t;
+  }
+  return x;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int n, const int* __restrict__ in,
+                                                           int* __restrict__ out,
+                                                           int* __restrict__ totals)
+{
+  __shared__ int v[kScanTile];
+  __shared__ int wsum[kScanBlock / 64];
+  const int base = blockIdx.x * kScanTile;
+  for (int k = threadIdx.x; k < kScanTile; k += kScanBlock)
+    v[k] = base + k < n ? in[base + k] : 0;
+  __syncthreads();
+  int loc[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int u = 0; u < kScanItems; ++u) {
+    loc[u] = sum;
+    sum += v[threadIdx.x * kScanItems + u];
+  }
+  const int incl = wave_inclusive_scan(sum);
+  if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int pre = incl - sum;
+  for (int w = 0; w < static_cast<int>(threadIdx.x >> 6); ++w) pre += wsum[w];
+#pragma unroll
+  for (int u = 0; u < kScanItems; ++u) v[threadIdx.x * kScanItems + u] = pre + loc[u];
+  __syncthreads();
+  for (int k = threadIdx.x; k < kScanTile; k += kScanBlock)
+    if (base + k < n) out[base + k] = v[k];
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) t += wsum[w];
+    totals[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_totals(int n, int* __restrict__ totals)
+{
+  __shared__ int wsum[16];
+  int carry = 0;
+  for (int base = 0; base < n; base += 1024) {
+    const int k = base + static_cast<int>(threadIdx.x);
+    const int x = k < n ? totals[k] : 0;
+    const int incl = wave_inclusive_scan(x);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int pre = incl - x, all = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < static_cast<int>(threadIdx.x >> 6)) pre += wsum[w];
+      all += wsum[w];
+    }
+    if (k < n) totals[k] = carry + pre;
+    carry += all;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scan_add(int n, int* __restrict__ out, const int* __restrict__ totals)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] += totals[k / kScanTile];
+}
+
+__global__ void k_place(int N, const int* __restrict__ key, const int* __restrict__ slot,
+                        const int* __restrict__ cbeg, int* __restrict__ arrival)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  arrival[cbeg[key[i]] + slot[i]] = i;
 }
 
 __global__ void k_fill_int(int* __restrict__ p, int v, size_t n)
@@ -79,15 +168,25 @@ __global__ void k_fill_double(double* __restrict__ p, double v, size_t n)
   if (i < n) p[i] = v;
 }
 
-__global__ void k_gather(int N, int S, const int* __restrict__ skey, const int* __restrict__ satom,
-                         const double* __restrict__ wrapped, const int* __restrict__ type,
-                         double* __restrict__ pos_s, int* __restrict__ type_s,
-                         int* __restrict__ cbeg, int* __restrict__ cend, int* __restrict__ bad_type,
-                         const int* __restrict__ sid, int* __restrict__ sid_s)
+// Final position of each atom: its cell's start plus the number of atoms of the same cell
+// with a smaller index; then the gather of wrapped positions and types into that order.
+__global__ void k_rank(int N, int S, const int* __restrict__ key, const int* __restrict__ arrival,
+                       const int* __restrict__ cbeg, const double* __restrict__ wrapped,
+                       const int* __restrict__ type, int* __restrict__ satom,
+                       int* __restrict__ skey, double* __restrict__ pos_s,
+                       int* __restrict__ type_s, int* __restrict__ bad_type,
+                       const int* __restrict__ sid, int* __restrict__ sid_s)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const int i = satom[p];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  const int i = arrival[q];
+  const int k = key[i];
+  const int b = cbeg[k], e = cbeg[k + 1];
+  int r = 0;
+  for (int u = b; u < e; ++u) r += arrival[u] < i;
+  const int p = b + r;
+  satom[p] = i;
+  skey[p] = k;
   pos_s[p] = wrapped[i];
   pos_s[N + p] = wrapped[N + i];
   pos_s[2 * N + p] = wrapped[2 * N + i];
@@ -98,9 +197,6 @@ __global__ void k_gather(int N, int S, const int* __restrict__ skey, const int* 
   }
   type_s[p] = t;
   if (sid_s) sid_s[p] = sid[i];
-  const int k = skey[p];
-  if (p == 0 || skey[p - 1] != k) cbeg[k] = p;
-  if (p == N - 1 || skey[p + 1] != k) cend[k] = p + 1;
 }
 
 __device__ __forceinline__ int wrap_cell(int c, int n, int& shift)
@@ -208,6 +304,18 @@ __global__ void k_scatter(int N, const int* __restrict__ satom, const double* __
   for (int c = 0; c < 3; ++c) force[static_cast<size_t>(c) * N + i] += f_s[static_cast<size_t>(c) * N + p];
 #pragma unroll
   for (int c = 0; c < 9; ++c) vir[static_cast<size_t>(c) * N + i] += v_s[static_cast<size_t>(c) * N + p];
+}
+
+// Deferred checks: latch[0] = max neighbour count, latch[1] |= bad type, latch[2..3] = the
+// last call's total edge count, latch[4] = number of latched calls.
+__global__ void k_latch(const int* __restrict__ mc, int* __restrict__ latch)
+{
+  if (threadIdx.x != 0) return;
+  latch[0] = max(latch[0], mc[0]);
+  latch[1] |= mc[1];
+  latch[2] = mc[2];
+  latch[3] = mc[3];
+  latch[4] += 1;
 }
 
 // First sorted atom whose cell key is >= k.
@@ -344,17 +452,37 @@ inline int blocks_for(size_t n, int bs) { return static_cast<int>((n + bs - 1) /
 
 }  // namespace
 
-void launch_bin(int N, const double* pos, const CellGrid& g, const CellGrid* grids,
-                const int* cell_off, const int* atom_off, int nstruct, int* cell_key,
-                int* atom_val, double* wrapped, int* sid, hipStream_t s)
+void launch_cell_sort(const CellSortArgs& a, hipStream_t s)
 {
-  if (grids)
-    k_bin<true><<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, grids, cell_off, atom_off, nstruct,
-                                                   cell_key, atom_val, wrapped, sid);
+  const int N = a.N;
+  const int n = a.ncell + 1;  // counts of every cell plus one zero: cbeg[ncell] = N
+  const int tiles = blocks_for(n, kScanTile);
+  launch_fill_int(a.cell_count, 0, n, s);
+  if (a.grids)
+    k_bin<true><<<blocks_for(N, 256), 256, 0, s>>>(N, a.pos, a.grid, a.grids, a.cell_off,
+                                                   a.atom_off, a.nstruct, a.key, a.slot,
+                                                   a.cell_count, a.wrapped, a.sid);
   else
-    k_bin<false><<<blocks_for(N, 256), 256, 0, s>>>(N, pos, g, grids, cell_off, atom_off, nstruct,
-                                                    cell_key, atom_val, wrapped, sid);
+    k_bin<false><<<blocks_for(N, 256), 256, 0, s>>>(N, a.pos, a.grid, nullptr, nullptr, nullptr,
+                                                    1, a.key, a.slot, a.cell_count, a.wrapped,
+                                                    nullptr);
+  k_scan_tiles<<<tiles, kScanBlock, 0, s>>>(n, a.cell_count, a.cell_begin, a.scan_totals);
+  if (tiles > 1) {
+    k_scan_totals<<<1, 1024, 0, s>>>(tiles, a.scan_totals);
+    k_scan_add<<<blocks_for(n, 256), 256, 0, s>>>(n, a.cell_begin, a.scan_totals);
+  }
+  k_place<<<blocks_for(N, 256), 256, 0, s>>>(N, a.key, a.slot, a.cell_begin, a.arrival);
+  k_rank<<<blocks_for(N, 256), 256, 0, s>>>(N, a.S, a.key, a.arrival, a.cell_begin, a.wrapped,
+                                             a.type, a.satom, a.skey, a.pos_s, a.type_s,
+                                             a.bad_type, a.sid, a.sid_s);
 }
+
+void launch_latch(const int* max_count, int* latch, hipStream_t s)
+{
+  k_latch<<<1, 64, 0, s>>>(max_count, latch);
+}
+
+size_t cell_sort_totals(int ncell) { return static_cast<size_t>(blocks_for(ncell + 1, kScanTile)); }
 
 void launch_fill_int(int* p, int value, size_t n, hipStream_t s)
 {
@@ -366,15 +494,6 @@ void launch_fill_double(double* p, double value, size_t n, hipStream_t s)
 {
   if (n == 0) return;
   k_fill_double<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
-}
-
-void launch_gather(int N, int S, const int* sorted_key, const int* sorted_atom,
-                   const double* wrapped, const int* type, double* pos_s, int* type_s,
-                   int* cell_begin, int* cell_end, int* bad_type, const int* sid, int* sid_s,
-                   hipStream_t s)
-{
-  k_gather<<<blocks_for(N, 256), 256, 0, s>>>(N, S, sorted_key, sorted_atom, wrapped, type, pos_s,
-                                               type_s, cell_begin, cell_end, bad_type, sid, sid_s);
 }
 
 template <typename Real>
@@ -402,22 +521,6 @@ void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, i
 {
   if (N <= 0) return;
   k_windows<<<blocks_for(N, kWinTile), 64, 0, s>>>(N, g, sorted_key, win, cap);
-}
-
-size_t radix_sort_temp_bytes(int N, int bits)
-{
-  size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const int*>(nullptr),
-                                     static_cast<int*>(nullptr), static_cast<const int*>(nullptr),
-                                     static_cast<int*>(nullptr), N, 0, bits);
-  return bytes;
-}
-
-hipError_t radix_sort_pairs(void* temp, size_t temp_bytes, const int* keys_in, int* keys_out,
-                            const int* vals_in, int* vals_out, int N, int bits, hipStream_t s)
-{
-  return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, N, 0,
-                                     bits, s);
 }
 
 void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,

@@ -14,6 +14,7 @@ SUS2SH_ERR_INPUT = 1
 SUS2SH_ERR_ARGUMENT = 2
 SUS2SH_ERR_DEVICE = 3
 SUS2SH_ERR_UNSUPPORTED = 4
+SUS2SH_ERR_CAPACITY = 5
 
 # Every symbol declared in include/sus2sh.h.
 EXPORTED_SYMBOLS = (
@@ -24,6 +25,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_compute_param_grad",
     "sus2sh_compute_batch",
     "sus2sh_md_nve",
+    "sus2sh_check",
     "sus2sh_set_params",
     "sus2sh_get_params",
     "sus2sh_last_error",
@@ -132,6 +134,8 @@ def lib():
                                 c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p,
                                 P(c_double), c_void_p]
     L.sus2sh_md_nve.restype = c_int
+    L.sus2sh_check.argtypes = [c_void_p, c_void_p]
+    L.sus2sh_check.restype = c_int
     L.sus2sh_set_params.argtypes = [c_void_p, P(c_double), c_int]
     L.sus2sh_set_params.restype = c_int
     L.sus2sh_get_params.argtypes = [c_void_p, P(c_double), c_int]

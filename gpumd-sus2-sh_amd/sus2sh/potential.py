@@ -102,6 +102,14 @@ class SUS2_SH:
     def mean_neighbors(self):
         return lib().sus2sh_mean_neighbors(self._h)
 
+    def check(self, stream=None):
+        """sus2sh_check: synchronise and raise Sus2shError (status 5 = neighbour capacity
+        exceeded, 2 = bad type) for what deferred-check evaluations latched since the last
+        check (option sus2_sh_deferred_check=1)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().sus2sh_check(self._h, ctypes.c_void_p(s)), self._h)
+
     def get_params(self):
         n = self.info()["param_count"]
         out = np.zeros(n)

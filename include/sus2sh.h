@@ -19,6 +19,7 @@ extern "C" {
 #define SUS2SH_ERR_ARGUMENT 2    /* invalid argument to an entry point */
 #define SUS2SH_ERR_DEVICE 3      /* HIP runtime or kernel launch failure */
 #define SUS2SH_ERR_UNSUPPORTED 4 /* valid model that this engine does not evaluate */
+#define SUS2SH_ERR_CAPACITY 5    /* deferred-check evaluations ran out of neighbour capacity */
 
 typedef struct sus2sh_handle sus2sh_handle;
 
@@ -73,8 +74,11 @@ int sus2sh_probe(const char* path);
  * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
  * Honoured options: sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
  * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8>,
- * sus2_sh_autotune=<0|1>,
- * sus2_sh_profile=<0|1>; other sus2_* tokens are accepted and ignored.  Uses the current HIP device. */
+ * sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
+ * sus2_sh_deferred_check=<0|1> (steady calls make no host synchronisation and can be
+ * captured in a HIP graph; see sus2sh_check), sus2_sh_md_graph=<0|1> (sus2sh_md_nve replays
+ * captured steps, default 1); other sus2_* tokens are accepted and ignored.  Uses the
+ * current HIP device. */
 int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
                   sus2sh_handle** out);
 
@@ -94,6 +98,18 @@ void sus2sh_destroy(sus2sh_handle* handle);
 int sus2sh_compute(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                    const int* d_type, const double* d_position, double* d_potential,
                    double* d_force, double* d_virial, void* stream);
+
+/* Deferred checks.  With sus2_sh_deferred_check=1, a sus2sh_compute call that follows a
+ * synchronised call with the same num_atoms enqueues its kernels without any host
+ * synchronisation or allocation (so it can be captured in a HIP graph on a non-default
+ * stream), and latches on the device whether some atom had more neighbours than the list
+ * capacity or a type out of range.  sus2sh_check synchronises `stream` and reports what
+ * was latched since the previous check: SUS2SH_ERR_CAPACITY when neighbours were dropped
+ * (the outputs of those calls are incomplete; the capacity is raised and the next
+ * uncaptured call sizes the list again), SUS2SH_ERR_ARGUMENT for a bad type, else
+ * SUS2SH_OK.  The reference synchronises on every call instead (thrust max_element over the
+ * neighbour counts, src/force/sus2_sh.cu:9596-9598); no reference counterpart. */
+int sus2sh_check(sus2sh_handle* handle, void* stream);
 
 /* Model-parameter gradient of the total energy, dE/dtheta, accumulated (+=) into
  * d_param_grad (double[param_count]).  Parameter order:

@@ -64,8 +64,8 @@ __device__ __forceinline__ void read_rec(const LaneRec<Real, NS, BINARY>& q, con
   }
 }
 
-// Accumulate a group and end its rows.  Store and energy segments are branch-free: every group
-// writes acc to its target word (the trash slot unless the row ends) and scales acc by keep.
+// Accumulate a group and end its rows.  Store segments are branch-free: every group writes
+// acc to its target word (the row's own target, or the trash slot) and scales acc by keep.
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q,
                                            const Real (&xv)[NS], const Real (&yv)[NS],
@@ -80,11 +80,7 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
       acc[j] *= q.keep[j];
     }
   } else if (KIND == 2) {
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      e += acc[j] - acc[j] * q.keep[j];
-      acc[j] *= q.keep[j];
-    }
+    // energy rows: E' is the sum of every term, summed once at the end of the segment
   } else if (q.mask != 0u) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
@@ -96,29 +92,56 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
   }
 }
 
-// One segment of one wave: `groups` records from R.  Two record sets alternate, so the
-// record of group g + 1 is fetched while group g's LDS reads are in flight and nothing is
-// copied between iterations.
+// One segment of one wave: `groups` records from R.  Records are fetched SG at a time
+// (one batch of scalar loads at immediate offsets from one pointer, then one wait), so the
+// scalar unit spends a pointer increment and a loop test per SG groups; inside a batch the
+// LDS reads of group g + 1 are issued before group g is finished.
+template <int NS, typename Real>
+constexpr int lane_batch()
+{
+  return NS * static_cast<int>(sizeof(Real)) <= 8 ? 4 : NS * static_cast<int>(sizeof(Real)) <= 16 ? 2 : 1;
+}
+
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
 {
   constexpr bool BIN = KIND != 1;
   constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
   constexpr int RW = HW + 2 * NS * static_cast<int>(sizeof(Real)) / 4;
+  constexpr int SG = lane_batch<NS, Real>();
   Real acc[NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = Real(0);
-  LaneRec<Real, NS, BIN> a, b;
-  Real xv[NS], yv[NS];
-  fetch_rec<Real, NS, BIN>(R, a);
-  for (int g = 0; g < groups; g += 2) {
-    read_rec<Real, NS, BIN>(a, X, xv, yv);
-    if (g + 1 < groups) fetch_rec<Real, NS, BIN>(R + (g + 1) * RW, b);
-    finish_rec<Real, NS, KIND>(a, xv, yv, acc, X, e);
-    if (g + 1 >= groups) break;
-    read_rec<Real, NS, BIN>(b, X, xv, yv);
-    if (g + 2 < groups) fetch_rec<Real, NS, BIN>(R + (g + 2) * RW, a);
-    finish_rec<Real, NS, KIND>(b, xv, yv, acc, X, e);
+  Real xa[NS], ya[NS], xb[NS], yb[NS];
+  int g = 0;
+  for (; g + SG <= groups; g += SG) {
+    LaneRec<Real, NS, BIN> r[SG];
+#pragma unroll
+    for (int k = 0; k < SG; ++k) fetch_rec<Real, NS, BIN>(R + k * RW, r[k]);
+    read_rec<Real, NS, BIN>(r[0], X, xa, ya);
+#pragma unroll
+    for (int k = 0; k < SG; ++k) {
+      const int next = k + 1 < SG ? k + 1 : k;  // in range also where it is not used
+      if (k % 2 == 0) {
+        if (k + 1 < SG) read_rec<Real, NS, BIN>(r[next], X, xb, yb);
+        finish_rec<Real, NS, KIND>(r[k], xa, ya, acc, X, e);
+      } else {
+        if (k + 1 < SG) read_rec<Real, NS, BIN>(r[next], X, xa, ya);
+        finish_rec<Real, NS, KIND>(r[k], xb, yb, acc, X, e);
+      }
+    }
+    R += SG * RW;
+  }
+  for (; g < groups; ++g) {
+    LaneRec<Real, NS, BIN> r;
+    fetch_rec<Real, NS, BIN>(R, r);
+    read_rec<Real, NS, BIN>(r, X, xa, ya);
+    finish_rec<Real, NS, KIND>(r, xa, ya, acc, X, e);
+    R += RW;
+  }
+  if (KIND == 2) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) e += acc[j];
   }
 }
 

@@ -548,14 +548,15 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
             X[t] = acc[j];
             acc[j] *= keep;
           } else if (kind == kLaneBinaryEnergy) {
-            e += acc[j] - acc[j] * keep;
-            acc[j] *= keep;
+            (void)keep;  // energy terms are summed at the end of the segment
           } else if (mask & (1u << j)) {
             X[t] += acc[j];
             acc[j] = 0.0;
           }
         }
       }
+      if (kind == kLaneBinaryEnergy)
+        for (int j = 0; j < ns; ++j) e += acc[j];
     }
   }
   for (int i = 0; i < B; ++i) grad_b[i] = X[B + i];

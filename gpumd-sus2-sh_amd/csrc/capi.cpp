@@ -7,6 +7,7 @@
 #include "engine.hpp"
 #include "lane_program.hpp"
 #include "model.hpp"
+#include "zbl.hpp"
 
 struct sus2sh_handle {
   std::unique_ptr<sus2sh::Engine> engine;
@@ -338,6 +339,21 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
     for (int i = 0; i < n; ++i)
       e_prime[i] = sus2sh::run_lane_program_host(p, basics + static_cast<size_t>(i) * B,
                                                  grad + static_cast<size_t>(i) * B);
+  });
+}
+
+int sus2sh_zbl_tables(const char* mtp_path, int nsym, const char** symbols, int* atomic_numbers,
+                      double* inner, double* outer)
+{
+  if (!mtp_path || nsym < 0 || (nsym > 0 && !symbols)) return SUS2SH_ERR_ARGUMENT;
+  return guarded(&g_create_error, [&] {
+    const sus2sh::Model m = sus2sh::load_model(mtp_path);
+    if (!m.zbl_enabled) throw sus2sh::ArgumentError("model has no ZBL term (zbl_enabled)");
+    const sus2sh::ZblTables t = sus2sh::build_zbl_tables(m, nsym, symbols);
+    const int S = m.species_count;
+    if (atomic_numbers) std::memcpy(atomic_numbers, t.atomic_number.data(), S * sizeof(int));
+    if (inner) std::memcpy(inner, t.inner.data(), static_cast<size_t>(S) * S * sizeof(double));
+    if (outer) std::memcpy(outer, t.outer.data(), static_cast<size_t>(S) * S * sizeof(double));
   });
 }
 

@@ -99,8 +99,6 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   model_ = load_model(path);
   if (model_.radial_kind == RadialKind::LaguerreLog1p)
     throw UnsupportedError("SUS2_SH: RBLaguerre_log1p radial basis is not supported by this engine.");
-  if (model_.zbl_enabled)
-    throw UnsupportedError("SUS2_SH: ZBL-enabled models are not supported by this engine.");
   if (model_.l_max > kMaxL)
     throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 4 only.");
   if (model_.k_max > kMaxK) throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 6.");
@@ -108,6 +106,14 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     throw UnsupportedError("SUS2_SH: this engine supports at most 16 species.");
   opt_ = parse_options(model_, nopts, opts);
   prog_ = compile_graph(model_);
+  if (model_.zbl_enabled) {
+    // atomic numbers and pair cutoffs (src/force/sus2_sh.cu:7824-7860); the neighbour
+    // cutoff of the reference is max(max_dist, largest ZBL outer cutoff)
+    zbl_ = build_zbl_tables(model_, nopts, opts);
+    if (zbl_.outer_max > model_.max_dist)
+      throw UnsupportedError("SUS2_SH: ZBL outer cutoff beyond max_dist is not supported by "
+                             "this engine.");
+  }
 
   hip_check(hipGetDevice(&device_), "hipGetDevice");
   int lds = 0;
@@ -163,6 +169,11 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     upload_tables<float>();
   else
     upload_tables<double>();
+  if (model_.zbl_enabled) {
+    zbl_z_.upload(zbl_.atomic_number.data(), zbl_.atomic_number.size() * sizeof(int));
+    zbl_inner_.upload(zbl_.inner.data(), zbl_.inner.size() * sizeof(double));
+    zbl_outer_.upload(zbl_.outer.data(), zbl_.outer.size() * sizeof(double));
+  }
   hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_max_), 8 * sizeof(int), 0),
             "hipHostMalloc");
   latch_.resize(8 * sizeof(int));
@@ -737,6 +748,12 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(),
                            param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+  if (model_.zbl_enabled) {
+    const ZblTable zt{zbl_z_.as<int>(), zbl_inner_.as<double>(), zbl_outer_.as<double>(),
+                      model_.species_count};
+    launch_zbl<Real>(N, count_.as<int>(), nl_.as<Edge<Real>>(), type_s_.as<int>(), zt,
+                     e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(), s);
+  }
   if (profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
   launch_scatter(N, satom_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),

@@ -18,6 +18,7 @@
 #include "model.hpp"
 #include "lane_program.hpp"
 #include "program.hpp"
+#include "zbl.hpp"
 
 namespace sus2sh {
 
@@ -103,6 +104,7 @@ class Engine {
   const Model& model() const { return model_; }
   const GraphProgram& program() const { return prog_; }
   const LaneProgram& lane_program() const { return lane_; }
+  const ZblTables& zbl() const { return zbl_; }
   const EngineOptions& options() const { return opt_; }
   int graph_tile() const { return tile_; }
   int graph_variant() const { return variant_; }
@@ -164,6 +166,8 @@ class Engine {
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
   DeviceBuffer lane_segs_, lane_ins_;
   DeviceBuffer win_;  // neighbour windows of the force tiles
+  ZblTables zbl_;     // model.zbl_enabled: atomic numbers and pair cutoffs
+  DeviceBuffer zbl_z_, zbl_inner_, zbl_outer_;
   int cu_count_ = 256;
   bool radial_uniform_ = false;
 

@@ -131,6 +131,17 @@ constexpr int kWinStride = 2 + 2 * kWinMax;
 void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,
                     hipStream_t s);
 
+// --- ZBL pair term (zbl.hip) ---
+struct ZblTable {
+  const int* z;          // [S] atomic numbers
+  const double* inner;   // [S * S]
+  const double* outer;   // [S * S]
+  int S;
+};
+template <typename Real>
+void launch_zbl(int N, const int* count, const Edge<Real>* nl, const int* type_s,
+                const ZblTable& t, double* e_s, double* f_s, double* v_s, hipStream_t s);
+
 // --- edge kernels (edge_kernels.hip) ---
 template <typename Real>
 struct EdgeLaunch {

@@ -711,6 +711,20 @@ Model parse_model_text(const std::string& text)
   m.radial_kind = radial_kind_of(m.radial_basis_type);
   m.max_dist = double_after(text, "max_dist =");
   m.zbl_enabled = optional_bool_after(text, "zbl_enabled =", false);
+  if (m.zbl_enabled) {
+    const auto optional_double = [&](const char* token, double dflt) {
+      return has_token(text, token) ? double_after(text, token) : dflt;
+    };
+    m.zbl_inner = optional_double("zbl_inner =", 0.7);
+    m.zbl_outer = optional_double("zbl_outer =", 1.4);
+    // type-wise cutoffs: explicit flag, else implied by the presence of a factor
+    m.zbl_typewise = has_token(text, "zbl_typewise_cutoff_enabled =")
+                       ? optional_bool_after(text, "zbl_typewise_cutoff_enabled =", false)
+                       : has_token(text, "zbl_typewise_cutoff_factor =");
+    m.zbl_typewise_factor = optional_double("zbl_typewise_cutoff_factor =", 0.7);
+    if (has_token(text, "zbl_atomic_numbers ="))
+      m.zbl_atomic_numbers = numbers<int>(braced_after(text, "zbl_atomic_numbers ="));
+  }
   m.rb_size = int_after(text, "radial_basis_size =");
   m.radial_funcs_count = int_after(text, "radial_funcs_count =");
   m.moments_count = int_after(text, "alpha_moments_count =");

@@ -87,8 +87,14 @@ struct Model {
   std::vector<Product> products;
   std::vector<int> scalar_body_orders;  // optional sh_scalar_info
 
-  // ZBL header fields (parsed, not evaluated by this engine)
+  // ZBL pair term (sus2_sh.cu:2132-2147); cutoff tables are built at potential creation,
+  // when the run.in symbols are known (zbl.hpp)
   bool zbl_enabled = false;
+  double zbl_inner = 0.7;
+  double zbl_outer = 1.4;
+  bool zbl_typewise = false;
+  double zbl_typewise_factor = 0.7;
+  std::vector<int> zbl_atomic_numbers;  // optional metadata, one per species
 
   // explicit metadata (sus2_sh.cu:1913-2093)
   bool standard_graph_matched = false;

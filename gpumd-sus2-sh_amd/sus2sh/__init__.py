@@ -61,3 +61,22 @@ def lane_check(path, basics=None, max_slots=160, waves=1):
     if not d["ok"]:
         e = g = None
     return d, e, g
+
+
+def zbl_tables(path, symbols=()):
+    """Host-only ZBL tables of a zbl_enabled model (sus2sh_zbl_tables): (atomic numbers [S],
+    inner [S, S], outer [S, S]); `symbols` are the element symbols that follow the model file
+    in run.in (used when the file has no zbl_atomic_numbers)."""
+    import ctypes
+    import numpy as np
+    from ._capi import check
+    dims = model_check(path)
+    S = dims["species_count"]
+    z = (ctypes.c_int * S)()
+    a = np.zeros((S, S))
+    b = np.zeros((S, S))
+    arr = (ctypes.c_char_p * max(1, len(symbols)))(*[s.encode() for s in symbols])
+    P = ctypes.POINTER(ctypes.c_double)
+    check(lib().sus2sh_zbl_tables(str(path).encode(), len(symbols), arr, z, a.ctypes.data_as(P),
+                                  b.ctypes.data_as(P)))
+    return np.array(list(z)), a, b

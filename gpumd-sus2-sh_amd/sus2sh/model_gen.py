@@ -173,6 +173,18 @@ def write_mtp(spec: ModelSpec, path):
         f"radial_basis_type = {spec.radial_basis_type}",
         f"\tmin_dist = {_fmt(spec.min_dist)}",
         f"\tmax_dist = {_fmt(spec.max_dist)}",
+    ]
+    # optional header keys, e.g. the ZBL pair term: zbl_enabled, zbl_inner, zbl_outer,
+    # zbl_typewise_cutoff_factor, zbl_atomic_numbers
+    for key, val in spec.extra.items():
+        if isinstance(val, bool):
+            val = "true" if val else "false"
+        elif isinstance(val, (list, tuple, np.ndarray)):
+            val = _braced(list(val), str)
+        elif isinstance(val, float):
+            val = _fmt(val)
+        lines.append(f"{key} = {val}")
+    lines += [
         f"\tradial_basis_size = {spec.rb_size}",
         f"\tradial_funcs_count = {spec.radial_funcs_count}",
         "\tscal_coeffs = " + _braced(spec.scal_coeffs),

@@ -198,6 +198,14 @@ int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_
                         double* product_coeff, int* scalars, int* scalar_body);
 void sus2sh_graph_free(sus2sh_graph* graph);
 
+/* Host only: the ZBL pair tables of a model with zbl_enabled -- atomic number per species
+ * (file metadata zbl_atomic_numbers, else the element symbols), and [S*S] inner / outer
+ * cutoffs (type-wise when zbl_typewise_cutoff_enabled or a factor is given).  Replaces the
+ * table set-up of src/force/sus2_sh.cu:7824-7860 with src/force/sus2_zbl_common.cuh:77-168;
+ * errors carry the reference's messages (sus2sh_last_error(NULL)). */
+int sus2sh_zbl_tables(const char* mtp_path, int nsym, const char** symbols, int* atomic_numbers,
+                      double* inner, double* outer);
+
 /* Parse and validate a model file on the host only (the reference loader,
  * src/force/sus2_sh.cu:2105-2482).  dims[0..16) = species_count, l_max, k_max,
  * body_order, radial_basis_size, radial_funcs_count, moments_count, basic_count,

@@ -53,6 +53,13 @@ def lib():
     L.oracle_num_threads.restype = c_int
     L.oracle_set_num_threads.argtypes = [c_int]
     L.oracle_set_num_threads.restype = None
+    L.oracle_zbl_pair.argtypes = [c_int, c_int, c_double, c_double, c_double, P(c_double),
+                                  P(c_double)]
+    L.oracle_zbl_pair.restype = None
+    L.oracle_set_zbl.argtypes = [c_void_p, P(c_int), P(c_double), P(c_double)]
+    L.oracle_set_zbl.restype = c_int
+    L.oracle_zbl_enabled.argtypes = [c_void_p]
+    L.oracle_zbl_enabled.restype = c_int
     _lib = L
     return L
 
@@ -85,6 +92,14 @@ def neighbor_counts(cell, positions, rc, pbc=(1, 1, 1), brute=True):
     return out
 
 
+def zbl_pair(zi, zj, r, inner, outer):
+    """(phi, dphi/dr) of one ZBL pair (eV, eV/A)."""
+    e, de = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_zbl_pair(int(zi), int(zj), float(r), float(inner), float(outer),
+                          ctypes.byref(e), ctypes.byref(de))
+    return e.value, de.value
+
+
 def _box(cell):
     H = np.asarray(cell, dtype=np.float64).T
     return np.ascontiguousarray(np.concatenate([H.reshape(-1), np.linalg.inv(H).reshape(-1)]))
@@ -108,6 +123,19 @@ class Oracle:
         if getattr(self, "_m", None):
             lib().oracle_free(self._m)
             self._m = None
+
+    @property
+    def zbl_enabled(self):
+        return bool(lib().oracle_zbl_enabled(self._m))
+
+    def set_zbl(self, atomic_numbers, inner, outer):
+        """ZBL model: atomic number per species and [S, S] pair cutoffs (A)."""
+        S = self.dims["S"]
+        z = np.ascontiguousarray(atomic_numbers, dtype=np.int32)
+        a = np.ascontiguousarray(np.broadcast_to(inner, (S, S)), dtype=np.float64)
+        b = np.ascontiguousarray(np.broadcast_to(outer, (S, S)), dtype=np.float64)
+        if z.size != S or lib().oracle_set_zbl(self._m, _ip(z), _dp(a), _dp(b)) != 0:
+            raise ValueError("set_zbl: one atomic number per species (at most 16 species)")
 
     def radial(self, ti, tj, r):
         Mr = self.dims["Mr"]
@@ -137,7 +165,9 @@ class Oracle:
         pg = np.zeros(self.param_count) if param_grad else None
         pb = np.asarray(pbc, dtype=np.int32)
         box = _box(cell)
-        lib().oracle_compute(self._m, n, _ip(t), _dp(pos), _dp(box), _ip(pb), 1 if brute else 0,
-                             _dp(e), _dp(f), _dp(v), _dp(pg) if pg is not None else None)
+        if lib().oracle_compute(self._m, n, _ip(t), _dp(pos), _dp(box), _ip(pb),
+                                1 if brute else 0, _dp(e), _dp(f), _dp(v),
+                                _dp(pg) if pg is not None else None) != 0:
+            raise ValueError("oracle_compute: ZBL model without set_zbl()")
         out = (e, f.reshape(3, n), v.reshape(9, n))
         return out + (pg,) if param_grad else out

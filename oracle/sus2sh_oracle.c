@@ -14,6 +14,10 @@
  *   edge derivative, forces, virial  src/force/sus2_sh.cu:5996-6035, 6347-6446
  *   multi-image neighbours           src/force/sus2_sh.cu:7657-7767, 9547-9618
  *   parameter gradient dE/dtheta     no reference counterpart (adjoint of the above)
+ *   ZBL pair term                    src/force/sus2_zbl_common.cuh:171-225 applied per edge as
+ *                                    in src/force/sus2_sh.cu:7522-7627; the universal screened
+ *                                    Coulomb form of Ziegler, Biersack and Littmark (1985)
+ *                                    with the reference's screening constant 2.134563
  * Everything is double precision end to end (the reference keeps float force/virial
  * buffers even in its double mode, src/force/sus2_sh.cuh:200-202).
  *
@@ -40,6 +44,10 @@ typedef struct {
   int *mapping, *basic_k, *basic_l, *basic_m;
   int *p_left, *p_right, *p_target;
   double *p_coeff;
+  /* ZBL: zbl_enabled in the file; per-species Z and per-pair cutoffs set by the caller */
+  int zbl, zbl_ready;
+  int zbl_z[16];
+  double zbl_inner[256], zbl_outer[256];
 } oracle_model;
 
 /* ------------------------------------------------------------------------ */
@@ -209,6 +217,14 @@ oracle_model* oracle_load(const char* path)
   m->B = or_int(text, "alpha_index_basic_count =", &ok);
   m->Q = or_int(text, "alpha_scalar_moments =", &ok);
   m->T = or_int(text, "sh_product_count =", &ok);
+  {
+    char zw[32];
+    int zok = 1;
+    zw[0] = 0;
+    if (strstr(text, "zbl_enabled =")) or_word(text, "zbl_enabled =", zw, sizeof zw, &zok);
+    m->zbl = zok && (strcmp(zw, "1") == 0 || strcmp(zw, "true") == 0 || strcmp(zw, "TRUE") == 0 ||
+                     strcmp(zw, "yes") == 0 || strcmp(zw, "on") == 0);
+  }
   if (!ok || m->S <= 0 || m->L < 0 || m->L > OR_MAX_L || m->K <= 0 || m->Mr != m->K * (m->L + 1) ||
       m->B <= 0 || m->A < m->B || m->T < 0 || m->Q <= 0 || m->rb <= 0) {
     free(text);
@@ -765,10 +781,55 @@ int oracle_graph(const oracle_model* m, int n, const double* basics, double* e_p
   return 0;
 }
 
+/* ZBL pair energy phi(r) and d phi / dr: screened Coulomb 14.3996454784255 Zi Zj / r times
+ * sum_k a_k exp(-b_k r / a), 1 / a = 2.134563 (Zi^0.23 + Zj^0.23), times the cosine switch
+ * (1 + cos(pi (r - inner) / (outer - inner))) / 2 above inner; zero from outer on. */
+void oracle_zbl_pair(int zi, int zj, double r, double inner, double outer, double* e, double* de)
+{
+  static const double a[4] = {0.18175, 0.50986, 0.28022, 0.02817};
+  static const double b[4] = {3.1998, 0.94229, 0.4029, 0.20162};
+  *e = 0.0;
+  *de = 0.0;
+  if (zi <= 0 || zj <= 0 || r <= 0.0 || r >= outer) return;
+  const double ainv = 2.134563 * (pow((double)zi, 0.23) + pow((double)zj, 0.23));
+  double phi = 0.0, dphi = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    const double t = a[k] * exp(-b[k] * ainv * r);
+    phi += t;
+    dphi -= b[k] * ainv * t;
+  }
+  const double q = 14.3996454784255 * (double)zi * (double)zj;
+  double sw = 1.0, dsw = 0.0;
+  if (r > inner) {
+    const double w = 3.14159265358979323846 / (outer - inner);
+    sw = 0.5 * (1.0 + cos(w * (r - inner)));
+    dsw = -0.5 * w * sin(w * (r - inner));
+  }
+  const double u = q * phi / r, du = q * (dphi / r - phi / (r * r));
+  *e = sw * u;
+  *de = sw * du + dsw * u;
+}
+
+/* Per-species atomic numbers and per-pair [S*S] cutoffs of a ZBL model. */
+int oracle_set_zbl(oracle_model* m, const int* z, const double* inner, const double* outer)
+{
+  if (m->S > 16) return -1;
+  for (int t = 0; t < m->S; ++t) m->zbl_z[t] = z[t];
+  for (int k = 0; k < m->S * m->S; ++k) {
+    m->zbl_inner[k] = inner[k];
+    m->zbl_outer[k] = outer[k];
+  }
+  m->zbl_ready = 1;
+  return 0;
+}
+
+int oracle_zbl_enabled(const oracle_model* m) { return m->zbl; }
+
 int oracle_compute(const oracle_model* m, int N, const int* type, const double* pos,
                    const double* h, const int* pbc, int mode, double* energy, double* force,
                    double* virial, double* dparam)
 {
+  if (m->zbl && !m->zbl_ready) return -1;
   or_init_ylm();
   or_nlist nl;
   if (mode & 1)
@@ -862,6 +923,34 @@ int oracle_compute(const oracle_model* m, int N, const int* type, const double* 
             pg[off_scal + so + S * S] += adj * dRds * Y[y];
           }
         }
+        fs[0] += g[0];
+        fs[1] += g[1];
+        fs[2] += g[2];
+        f[j] -= g[0];
+        f[N + j] -= g[1];
+        f[2 * N + j] -= g[2];
+        vir[0] -= dx * g[0];
+        vir[1] -= dy * g[1];
+        vir[2] -= dz * g[2];
+        vir[3] -= dx * g[1];
+        vir[4] -= dx * g[2];
+        vir[5] -= dy * g[2];
+        vir[6] -= dy * g[0];
+        vir[7] -= dz * g[0];
+        vir[8] -= dz * g[1];
+      }
+      /* ZBL: E_i += phi / 2 per edge inside the pair cutoff, g = (phi' / 2) d / r */
+      for (int e = nl.start[i]; m->zbl && e < nl.start[i + 1]; ++e) {
+        const int j = nl.j[e];
+        const int pair = ti * S + type[j];
+        const double dx = nl.d[3 * e], dy = nl.d[3 * e + 1], dz = nl.d[3 * e + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double ez, dez;
+        oracle_zbl_pair(m->zbl_z[ti], m->zbl_z[type[j]], r, m->zbl_inner[pair], m->zbl_outer[pair],
+                        &ez, &dez);
+        if (ez == 0.0 && dez == 0.0) continue;
+        energy[i] += 0.5 * ez;
+        const double g[3] = {0.5 * dez * dx / r, 0.5 * dez * dy / r, 0.5 * dez * dz / r};
         fs[0] += g[0];
         fs[1] += g[1];
         fs[2] += g[2];

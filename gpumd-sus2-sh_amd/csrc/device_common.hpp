@@ -7,6 +7,18 @@
 
 namespace sus2sh {
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (workgroup b runs on XCD
+// b mod 8), and each XCD has its own L2.  xcd_block() renumbers the workgroups so that XCD x
+// runs one contiguous range of logical blocks: atoms are in cell order, so the neighbours a
+// workgroup reads (positions, adjoints, force accumulators) then stay in the L2 of the XCD
+// that works on their region.  A bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block()
+{
+  const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // One neighbour edge, stored k-major (edge (k, p) at k * N + p) so that a wavefront whose
 // lanes are 64 consecutive atoms reads 64 consecutive edges.  j packs the neighbour's
 // cell-sorted index (low 27 bits) and its species (bits 27-30), so one 16-byte load gives

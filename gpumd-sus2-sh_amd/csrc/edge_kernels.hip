@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   stage_tables(a.rt, MR, row, s_coeff, s_scal);
 
   const int N = a.N;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= N) return;
   Real acc[FG];
 #pragma unroll
@@ -195,8 +195,9 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   int* w_off = w_first + kWinMax;
   Real* acc_w = reinterpret_cast<Real*>(smem_raw + pg_off + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16));
   int w_count = -1, w_total = 0;
+  const int block = xcd_block();  // the force tile: kWinTile atoms of the cell order
   if (WIN) {
-    const int* tw = win + static_cast<size_t>(blockIdx.x) * kWinStride;
+    const int* tw = win + static_cast<size_t>(block) * kWinStride;
     w_count = tw[0];
     w_total = tw[1];
     if (w_count >= 0) {
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   stage_tables(a.rt, MR, row, s_coeff, s_scal);  // ends with a barrier
 
   const int N = a.N;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = block * blockDim.x + threadIdx.x;
   const bool valid = p < N;
   if (!PG && !WIN && !valid) return;
   Real G[FG];

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
                                                    int* __restrict__ max_count)
 {
   const int N = a.N;
-  const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p0 = xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = p0 < N;
   const int p = valid ? p0 : N - 1;  // inactive lanes repeat the last atom, store nothing
   const int st = BATCH ? a.sid_s[p] : 0;

@@ -152,12 +152,65 @@ def run_c4(quick):
                fp32_vs_fp64=_err(r32, (r64["e"], r64["f"], r64["v"])),
                param_grad_fp32_vs_fp64_rel=float(np.abs(r32["g"] - r64["g"]).max() /
                                                  max(1.0, np.abs(r64["g"]).max())))
+    out.update(_c4_central_differences(path, cell, types, pos, r32, r64))
     if quick or n <= 64000:
         ref = Oracle(path).compute(cell, types, pos, param_grad=True)
         out["fp64_vs_oracle"] = _err(r64, ref)
         out["param_grad_fp64_vs_oracle_rel"] = float(np.abs(r64["g"] - ref[3]).max() /
                                                      max(1.0, np.abs(ref[3]).max()))
     return out
+
+
+def _c4_central_differences(path, cell, types, pos, r32, r64, h=1e-4, samples=6):
+    """Central differences (h = 1e-4) of the FP64 engine's total energy on a subsample of
+    atom coordinates and of model parameters, against the analytic FP64 and FP32 forces and
+    dE/dtheta."""
+    import torch
+    import sus2sh
+    n = len(types)
+    rng = np.random.default_rng(4004)
+    pot = sus2sh.SUS2_SH(path, n, ["A", "B", "C", "sus2_float=0"])
+    t, x = _dev(types, np.int32), _dev(pos, np.float64)
+    e = torch.zeros(n, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3, n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9, n, dtype=torch.float64, device="cuda")
+    box = sus2sh.box_struct(cell)
+
+    def energy():
+        e.zero_(), f.zero_(), v.zero_()
+        pot.compute(box, t, x, e, f, v)
+        return float(e.sum().item())
+
+    fd_f = []
+    for a in rng.choice(n, samples, replace=False):
+        c = int(rng.integers(0, 3))
+        x[c, a] += h
+        ep = energy()
+        x[c, a] -= 2 * h
+        em = energy()
+        x[c, a] += h
+        fd = -(ep - em) / (2 * h)
+        fd_f.append((abs(r64["f"][c, a] - fd), abs(r32["f"][c, a] - fd)))
+    theta = pot.get_params()
+    fd_g = []
+    for k in rng.choice(len(theta), samples, replace=False):
+        th = theta.copy()
+        th[k] += h
+        pot.set_params(th)
+        ep = energy()
+        th[k] -= 2 * h
+        pot.set_params(th)
+        em = energy()
+        fd = (ep - em) / (2 * h)
+        scale = max(1.0, abs(fd))
+        fd_g.append((abs(r64["g"][k] - fd) / scale, abs(r32["g"][k] - fd) / scale))
+    pot.set_params(theta)
+    pot.close()
+    fd_f, fd_g = np.array(fd_f), np.array(fd_g)
+    return dict(fd_samples=samples, fd_step=h,
+                fd_force_err_fp64=float(fd_f[:, 0].max()), fd_force_err_fp32=float(fd_f[:, 1].max()),
+                fd_param_grad_rel_err_fp64=float(fd_g[:, 0].max()),
+                fd_param_grad_rel_err_fp32=float(fd_g[:, 1].max()))
 
 
 def run_c5(quick):

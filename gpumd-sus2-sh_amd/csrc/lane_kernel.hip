@@ -52,6 +52,21 @@ __device__ __forceinline__ void fetch_rec(cu32* r, LaneRec<Real, NS, BINARY>& q)
   q.mask = r[3 * NS];
 }
 
+// Make every field of a fetched record live in scalar registers at this point (an empty asm
+// that takes them as SGPR operands), so that a batch of records costs one wait at its start
+// instead of loads the compiler would sink into the row-end branches, each followed by a
+// wait that also drains the LDS reads in flight.
+template <typename Real, int NS, bool BINARY>
+__device__ __forceinline__ void pin_rec(const LaneRec<Real, NS, BINARY>& q)
+{
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    asm volatile("" : : "s"(q.xo[j]), "s"(q.to[j]), "s"(q.c[j]));
+    if (BINARY) asm volatile("" : : "s"(q.yo[j]));
+  }
+  asm volatile("" : : "s"(q.mask));
+}
+
 // Issue the LDS reads of a group.
 template <typename Real, int NS, bool BINARY>
 __device__ __forceinline__ void read_rec(const LaneRec<Real, NS, BINARY>& q, const Real* X,
@@ -64,8 +79,10 @@ __device__ __forceinline__ void read_rec(const LaneRec<Real, NS, BINARY>& q, con
   }
 }
 
-// Accumulate a group and end its rows.  Store segments are branch-free: every group writes
-// acc to its target word (the row's own target, or the trash slot) and scales acc by keep.
+// Accumulate a group and end its rows: a row ends on a uniform bit of the record's mask
+// (scalar test and branch; most groups end no row), where the accumulator is stored (kinds
+// 0, 1) or added (kind 3) to the row's target and restarted.  Energy rows (kind 2) are summed
+// once at the end of the segment.
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q,
                                            const Real (&xv)[NS], const Real (&yv)[NS],
@@ -73,19 +90,15 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
 {
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] += KIND != 1 ? (q.c[j] * xv[j]) * yv[j] : q.c[j] * xv[j];
-  if (KIND == 0 || KIND == 1) {
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      X[q.to[j]] = acc[j];
-      acc[j] *= q.keep[j];
-    }
-  } else if (KIND == 2) {
-    // energy rows: E' is the sum of every term, summed once at the end of the segment
-  } else if (q.mask != 0u) {
+  (void)e;
+  if (KIND != 2 && q.mask != 0u) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       if (q.mask & (1u << j)) {
-        X[q.to[j]] += acc[j];
+        if (KIND == 3)
+          X[q.to[j]] += acc[j];
+        else
+          X[q.to[j]] = acc[j];
         acc[j] = Real(0);
       }
     }
@@ -118,6 +131,8 @@ __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& 
     LaneRec<Real, NS, BIN> r[SG];
 #pragma unroll
     for (int k = 0; k < SG; ++k) fetch_rec<Real, NS, BIN>(R + k * RW, r[k]);
+#pragma unroll
+    for (int k = 0; k < SG; ++k) pin_rec<Real, NS, BIN>(r[k]);
     read_rec<Real, NS, BIN>(r[0], X, xa, ya);
 #pragma unroll
     for (int k = 0; k < SG; ++k) {

@@ -545,8 +545,11 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
           const double keep = get_real(R + hw + j * rs);
           const uint32_t t = R[2 * ns + j] / col;
           if (kind == kLaneBinaryStore || kind == kLaneUnaryStore) {
-            X[t] = acc[j];
-            acc[j] *= keep;
+            (void)keep;  // rows end on their mask bit
+            if (mask & (1u << j)) {
+              X[t] = acc[j];
+              acc[j] = 0.0;
+            }
           } else if (kind == kLaneBinaryEnergy) {
             (void)keep;  // energy terms are summed at the end of the segment
           } else if (mask & (1u << j)) {

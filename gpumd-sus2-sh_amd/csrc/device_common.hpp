@@ -120,6 +120,33 @@ __device__ __forceinline__ void radial_map(int kind, Real a, Real s, Real r, Rea
   }
 }
 
+// NB values from a 16-byte aligned LDS row, as NB / (16 / sizeof(Real)) vector loads.
+template <typename Real, int NB>
+__device__ __forceinline__ void load_row16(const Real* __restrict__ row, Real (&out)[NB])
+{
+  if constexpr (sizeof(Real) == 4) {
+    static_assert(NB % 4 == 0, "rows of 4-byte values come in 16-byte vectors");
+    const float4* v = reinterpret_cast<const float4*>(row);
+#pragma unroll
+    for (int k = 0; k < NB / 4; ++k) {
+      const float4 q = v[k];
+      out[4 * k] = q.x;
+      out[4 * k + 1] = q.y;
+      out[4 * k + 2] = q.z;
+      out[4 * k + 3] = q.w;
+    }
+  } else {
+    static_assert(NB % 2 == 0, "rows of 8-byte values come in 16-byte vectors");
+    const double2* v = reinterpret_cast<const double2*>(row);
+#pragma unroll
+    for (int k = 0; k < NB / 2; ++k) {
+      const double2 q = v[k];
+      out[2 * k] = q.x;
+      out[2 * k + 1] = q.y;
+    }
+  }
+}
+
 // R_mu(r) (and R'_mu(r)) for all MR radial functions of one pair: Chebyshev recurrence on
 // xi with the (r - rc)^2 envelope (src/force/sus2_sh.cu:3171-3220).  `coeff` and `scal`
 // point at the pair's block of the tables (usually in LDS).
@@ -149,7 +176,9 @@ __device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
     }
 #pragma unroll
     for (int mu = 0; mu < MR; ++mu) {
-      const Real* c = coeff + mu * NB;
+      // the row is 16-byte aligned (see the table stride): read it as 16-byte vectors
+      Real c[NB];
+      load_row16<Real, NB>(coeff + mu * NB, c);
       Real acc = 0, accd = 0;
 #pragma unroll
       for (int n = 0; n < NB; ++n) {

@@ -21,13 +21,26 @@ __host__ __device__ constexpr int padded_stride(int n) { return n + ((1 - n % 32
 // Coefficient row length in LDS: NB (zero-padded) for the uniform kernels, rb otherwise.
 __host__ __device__ constexpr int coeff_row(int NB, int rb) { return NB > 0 ? NB : rb; }
 
+// Pair stride (in Reals) of the coefficient table.  The uniform kernels (NB > 0) read their
+// 8- or 16-coefficient rows as 16-byte vectors, so each pair table starts 16-byte aligned, 4
+// words past a multiple of 32 banks: the pair tables that the lanes of a wave read at once
+// then sit in disjoint bank quads.  Scalar rows (NB = 0) keep the 1 mod 32 padding.
 template <typename Real>
-__device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, int row,
+__host__ __device__ constexpr int coeff_stride(int NB, int n)
+{
+  if (NB == 0) return padded_stride(n);
+  constexpr int wpr = static_cast<int>(sizeof(Real)) / 4;  // words per Real
+  const int w = n * wpr;
+  return (w + ((4 - w % 32) + 32) % 32) / wpr;
+}
+
+template <typename Real>
+__device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, int row, int NB,
                                              Real* s_coeff, Real* s_scal)
 {
   const int pairs = rt.S * rt.S;
   const int nc = Mr * row, ns = Mr * 2;
-  const int cs = padded_stride(nc), ss = padded_stride(ns);
+  const int cs = coeff_stride<Real>(NB, nc), ss = padded_stride(ns);
   for (int i = threadIdx.x; i < pairs * nc; i += blockDim.x) {
     const int p = i / nc, k = i - p * nc, mu = k / row, n = k - mu * row;
     s_coeff[p * cs + k] = n < rt.rb ? rt.coeff[(p * Mr + mu) * rt.rb + n] : Real(0);
@@ -48,9 +61,9 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
   const int row = coeff_row(NB, a.rt.rb);
-  const int cs = padded_stride(MR * row), ss = padded_stride(MR * 2);
+  const int cs = coeff_stride<Real>(NB, MR * row), ss = padded_stride(MR * 2);
   Real* s_scal = s_coeff + a.rt.S * a.rt.S * cs;
-  stage_tables(a.rt, MR, row, s_coeff, s_scal);
+  stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);
 
   const int N = a.N;
   const int p = xcd_block() * blockDim.x + threadIdx.x;
@@ -181,7 +194,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
   const int rb = a.rt.rb;
   Real* s_coeff = reinterpret_cast<Real*>(smem_raw);
   const int row = coeff_row(NB, rb);
-  const int cs = padded_stride(MR * row), ss = padded_stride(MR * 2);
+  const int cs = coeff_stride<Real>(NB, MR * row), ss = padded_stride(MR * 2);
   Real* s_scal = s_coeff + S * S * cs;
   // PG: per-pair adjoint accumulators [S*S][MR*rb + 2*MR] (double), 16-byte aligned
   const size_t pg_off = ((static_cast<size_t>(S) * S * (cs + ss) * sizeof(Real)) + 15) & ~size_t(15);
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* 
       for (int i = threadIdx.x; i < 3 * w_total; i += blockDim.x) acc_w[i] = Real(0);
     }
   }
-  stage_tables(a.rt, MR, row, s_coeff, s_scal);  // ends with a barrier
+  stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);  // ends with a barrier
 
   const int N = a.N;
   const int p = block * blockDim.x + threadIdx.x;
@@ -350,7 +363,7 @@ size_t table_bytes(const EdgeLaunch<Real>& a, int NB)
 {
   const int Mr = a.K * (a.L + 1);
   return static_cast<size_t>(a.rt.S) * a.rt.S *
-         (padded_stride(Mr * coeff_row(NB, a.rt.rb)) + padded_stride(Mr * 2)) * sizeof(Real);
+         (coeff_stride<Real>(NB, Mr * coeff_row(NB, a.rt.rb)) + padded_stride(Mr * 2)) * sizeof(Real);
 }
 
 // NB of the uniform kernels: coefficient rows padded to 8 or 16; 0 selects the per-mu path.

@@ -124,9 +124,9 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
             "hipDeviceGetAttribute");
 
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
-  // LDS radial tables of the edge kernels: per pair, coefficients and (a, s), each padded to
-  // 1 mod 32 words (edge_kernels.hip)
-  const auto padded = [](size_t n) { return n + ((1 + 32 - n % 32) % 32); };
+  // LDS radial tables of the edge kernels: per pair, coefficients and (a, s), each padded by
+  // at most 31 words (1 mod 32, or 4 mod 32 for the 16-byte rows; edge_kernels.hip)
+  const auto padded = [](size_t n) { return n + 31; };
   // (rows of the uniform kernels are zero-padded to 8 or 16 coefficients)
   const size_t row = model_.rb_size <= 8 ? 8 : 16;
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *

@@ -191,6 +191,38 @@ __device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
     }
     return;
   }
+  if (kind == 2) {
+    // RBLaguerre_log1p (src/force/sus2_sh.cu:3299-3354): phi_n = (r - rc)^2 e^(-u/2) L_n(u),
+    // u = a log(1 + r / rho) with (a, rho) = the pair's (scal, shift) and rho >= 1e-8;
+    // Laguerre recurrence (n + 1) L_{n+1} = (2n + 1 - u) L_n - n L_{n-1}, L'_{n+1} = L'_n - L_n
+#pragma unroll
+    for (int mu = 0; mu < MR; ++mu) {
+      const Real a = scal[2 * mu];
+      const Real rho = scal[2 * mu + 1] > Real(1e-8) ? scal[2 * mu + 1] : Real(1e-8);
+      const Real u = a * log1p(r / rho);
+      const Real du = a / (rho + r);
+      const Real ex = exp(Real(-0.5) * u);
+      const Real* c = coeff + mu * rb;
+      Real l0 = Real(1), l1 = Real(1) - u, d1 = Real(-1);
+      Real acc = c[0], accd = Real(0);
+      if (rb > 1) {
+        acc += c[1] * l1;
+        accd += c[1] * d1;
+      }
+      for (int n = 1; n + 1 < rb; ++n) {
+        const Real l2 = ((Real(2 * n + 1) - u) * l1 - Real(n) * l0) / Real(n + 1);
+        const Real d2 = d1 - l1;
+        acc += c[n + 1] * l2;
+        if (DERIV) accd += c[n + 1] * d2;
+        l0 = l1;
+        l1 = l2;
+        d1 = d2;
+      }
+      R[mu] = env * ex * acc;
+      if (DERIV) dR[mu] = ex * (denv * acc + env * du * (accd - Real(0.5) * acc));
+    }
+    return;
+  }
 #pragma unroll
   for (int mu = 0; mu < MR; ++mu) {
     radial_map<Real>(kind, scal[2 * mu], scal[2 * mu + 1], r, xi, dxi);

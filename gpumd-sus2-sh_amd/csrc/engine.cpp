@@ -97,8 +97,6 @@ void DeviceBuffer::upload(const void* host, size_t bytes)
 Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** opts)
 {
   model_ = load_model(path);
-  if (model_.radial_kind == RadialKind::LaguerreLog1p)
-    throw UnsupportedError("SUS2_SH: RBLaguerre_log1p radial basis is not supported by this engine.");
   if (model_.l_max > kMaxL)
     throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 4 only.");
   if (model_.k_max > kMaxK) throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 6.");
@@ -259,7 +257,8 @@ void Engine::upload_tables()
   std::vector<Real> cR(coeff.begin(), coeff.end()), sR(scal.begin(), scal.end());
   radial_coeff_.upload(cR.data(), cR.size() * sizeof(Real));
   radial_scal_.upload(sR.data(), sR.size() * sizeof(Real));
-  radial_uniform_ = true;
+  // uniform kernels (one map for every mu, Chebyshev rows) are for the Chebyshev bases
+  radial_uniform_ = m.radial_kind != RadialKind::LaguerreLog1p;
   const int pairs = m.species_count * m.species_count;
   for (int p = 0; p < pairs && radial_uniform_; ++p)
     for (int mu = 1; mu < m.radial_funcs_count; ++mu)
@@ -485,6 +484,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
 {
   if (N >= kMaxEdgeAtoms)
     throw ArgumentError("SUS2_SH: at most 134217727 atoms per call (neighbour index width)");
+  if (param_grad && model_.radial_kind == RadialKind::LaguerreLog1p)
+    throw UnsupportedError("SUS2_SH: dE/dtheta is not implemented for RBLaguerre_log1p models.");
   const bool batch = nstruct > 1;
   hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
   hip_check(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
@@ -619,7 +620,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.rt.S = model_.species_count;
   el.rt.rb = model_.rb_size;
   el.rt.rc = static_cast<Real>(model_.max_dist);
-  el.rt.kind = model_.radial_kind == RadialKind::ChebyshevSSSRational ? 1 : 0;
+  el.rt.kind = static_cast<int>(model_.radial_kind);  // 0 tanh, 1 rational, 2 Laguerre
   el.rt.uniform = radial_uniform_ ? 1 : 0;
   el.fmap = fmap_.as<int>();
   if (!launch_basics<Real>(el, basics_.as<Real>(), s))

@@ -35,7 +35,7 @@ constexpr int kMaxSpecies = 16;
 enum class RadialKind : int {
   ChebyshevSSS = 0,          // xi = tanh(a (r - s) / 2)
   ChebyshevSSSRational = 1,  // xi = z / sqrt(1 + z^2), z = a (r - s) / 2 (README.md:55-56)
-  LaguerreLog1p = 2,         // accepted by the reference loader; not evaluated here
+  LaguerreLog1p = 2,         // phi_n = (r - rc)^2 e^(-u/2) L_n(u), u = a log(1 + r / rho)
 };
 
 struct Product {

@@ -199,7 +199,9 @@ oracle_model* oracle_load(const char* path)
   or_word(text, "potential_tag =", word, sizeof word, &ok);
   if (strcmp(word, "SUS2-SH") != 0) ok = 0;
   or_word(text, "radial_basis_type =", word, sizeof word, &ok);
-  if (strncmp(word, "RBChebyshev_sss_rational", 24) == 0)
+  if (strncmp(word, "RBLaguerre_log1p", 16) == 0)
+    m->kind = 2;
+  else if (strncmp(word, "RBChebyshev_sss_rational", 24) == 0)
     m->kind = 1;
   else if (strncmp(word, "RBChebyshev_sss", 15) == 0)
     m->kind = 0;
@@ -460,6 +462,44 @@ static void or_radial_full(const oracle_model* m, int ti, int tj, double r, doub
   const double env = dr * dr, denv = 2.0 * dr;
   const double tscale = m->scaling * m->type_coeffs[ti] * m->type_coeffs[tj];
   double T[64], dT[64];
+  if (m->kind == 2) {
+    /* RBLaguerre_log1p (sus2_sh.cu:3299-3354): phi_n = env e^(-u/2) L_n(u),
+     * u = a log(1 + r / rho), rho = max(shift, 1e-8); L_n by the three-term recurrence and
+     * L_n' from L_{n+1}' = L_n' - L_n (dE/dtheta of (a, rho) is not provided) */
+    for (int mu = 0; mu < m->Mr; ++mu) {
+      const int so = 2 * mu * S * S + S * ti + tj;
+      const double a = m->scal[so];
+      const double rho = m->scal[so + S * S] > 1e-8 ? m->scal[so + S * S] : 1e-8;
+      const double u = a * log1p(r / rho), du = a / (rho + r), ex = exp(-0.5 * u);
+      double Lg[64], dL[64];
+      Lg[0] = 1.0;
+      dL[0] = 0.0;
+      if (m->rb > 1) {
+        Lg[1] = 1.0 - u;
+        dL[1] = -1.0;
+      }
+      for (int n = 1; n + 1 < m->rb; ++n) {
+        Lg[n + 1] = ((2.0 * n + 1.0 - u) * Lg[n] - n * Lg[n - 1]) / (n + 1.0);
+        dL[n + 1] = dL[n] - Lg[n];
+      }
+      double acc = 0.0, accd = 0.0;
+      for (int n = 0; n < m->rb; ++n) {
+        const double c = m->radial[mu * m->rb + n] * tscale;
+        const double phi = env * ex * Lg[n];
+        const double dphi = ex * (denv * Lg[n] + env * du * (dL[n] - 0.5 * Lg[n]));
+        acc += c * phi;
+        accd += c * dphi;
+        if (Tn) Tn[mu * m->rb + n] = phi;
+        if (dTdxi) dTdxi[mu * m->rb + n] = 0.0;
+      }
+      R[mu] = acc;
+      if (dR) dR[mu] = accd;
+      if (xi_out) xi_out[mu] = 0.0;
+      if (dxi_da) dxi_da[mu] = 0.0;
+      if (dxi_ds) dxi_ds[mu] = 0.0;
+    }
+    return;
+  }
   for (int mu = 0; mu < m->Mr; ++mu) {
     const int so = 2 * mu * S * S + S * ti + tj;
     const double a = m->scal[so];

@@ -119,6 +119,38 @@ def rational(make_model):
                       seed=103, radial_basis_type="RBChebyshev_sss_rational", scal_mode="random")
 
 
+@pytest.fixture(scope="module")
+def laguerre(make_model):
+    return make_model("oracle_lag", l_max=2, k_max=2, body_order=5, rb_size=7, max_dist=5.0,
+                      seed=104, radial_basis_type="RBLaguerre_log1p", scal_mode="random")
+
+
+def test_laguerre_radial_closed_form(laguerre):
+    """R_mu(r) = (r-rc)^2 exp(-u/2) sum_n c_n L_n(u), u = a log(1 + r/rho), with L_n the
+    Laguerre polynomials of scipy.special.eval_laguerre."""
+    import re
+    from scipy.special import eval_laguerre
+    o = Oracle(laguerre)
+    text = open(laguerre).read()
+    spec_scal = np.array([float(v) for v in re.search(r"scal_coeffs = \{([^}]*)\}", text).group(1).split(",")])
+    rows = re.findall(r"\t\t\{([^}]*)\}", text)
+    S, Mr, rb, rc = 2, 6, 7, 5.0
+    scaling = float(re.search(r"scaling = (\S+)", text).group(1))
+    rad = np.array([[float(v) for v in r.split(",")] for r in rows[:Mr]])
+    w = rad[0, rb:]
+    for ti in range(S):
+        for tj in range(S):
+            for r in (0.9, 2.1, 3.4, 4.7):
+                R, _ = o.radial(ti, tj, r)
+                for mu in range(Mr):
+                    a = spec_scal[2 * mu * S * S + S * ti + tj]
+                    rho = spec_scal[2 * mu * S * S + S * S + S * ti + tj]
+                    u = a * math.log1p(r / rho)
+                    val = sum(rad[mu, n] * eval_laguerre(n, u) for n in range(rb))
+                    ref = (r - rc) ** 2 * math.exp(-0.5 * u) * scaling * w[ti] * w[tj] * val
+                    assert R[mu] == pytest.approx(ref, rel=1e-11, abs=1e-13)
+
+
 def test_radial_closed_form(l3k2_random):
     """R_mu(r) = (r-rc)^2 sum_n c_n cos(n acos xi), xi = tanh(a (r - s)/2)."""
     import re
@@ -144,7 +176,7 @@ def test_radial_closed_form(l3k2_random):
                     assert R[mu] == pytest.approx(ref, rel=1e-11, abs=1e-13)
 
 
-@pytest.mark.parametrize("fixture", ["l3k2_random", "rational"])
+@pytest.mark.parametrize("fixture", ["l3k2_random", "rational", "laguerre"])
 def test_radial_derivative_and_cutoff(request, fixture):
     o = Oracle(request.getfixturevalue(fixture))
     rc = o.rc
@@ -163,7 +195,7 @@ def _struct(n=3, a=3.8, sigma=0.06, S=2, seed=1):
     return st.fcc_alloy(n, n, n, a, [1.0 / S] * S, sigma, seed, seed + 1)
 
 
-@pytest.mark.parametrize("fixture", ["l2k2", "l3k2_random", "rational"])
+@pytest.mark.parametrize("fixture", ["l2k2", "l3k2_random", "rational", "laguerre"])
 def test_forces_are_energy_gradient(request, fixture):
     o = Oracle(request.getfixturevalue(fixture))
     pos, types, cell = _struct(S=o.dims["S"])

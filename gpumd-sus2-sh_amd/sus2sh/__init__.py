@@ -8,6 +8,7 @@ from .model_gen import (ModelSpec, StandardGraph, param_vector, standard_graph, 
 from .potential import (SUS2_SH, box_struct, compute_batch, evaluate,  # noqa: F401
                         maxwell_velocities, md_nve)
 from . import structures  # noqa: F401
+from . import gpumd_io  # noqa: F401
 
 
 def probe(path):

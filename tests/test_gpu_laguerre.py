@@ -50,3 +50,22 @@ def test_laguerre_param_grad_is_refused(make_model):
     pot.compute(sus2sh.box_struct(cell), t, x, e, f, v)  # the handle stays usable
     torch.cuda.synchronize()
     pot.close()
+
+
+def test_reference_smoke_inputs(model_dir):
+    """The reference's GPUMD smoke inputs (2-atom H-C cell, eight-symbol potential line with
+    sus2_float=1) through the run.in / XYZ readers and the engine, against the oracle; the
+    model file the line names is replaced by a synthetic eight-species Laguerre model."""
+    import os
+    from sus2sh import gpumd_io
+    gold = os.path.join(os.path.dirname(__file__), "golden", "gpumd_smoke")
+    spec = sus2sh.synthetic_model(2, 2, 5, species_count=8, rb_size=6, max_dist=5.0, seed=81,
+                                  radial_basis_type="RBLaguerre_log1p", scaling=0.5)
+    mtp = sus2sh.write_mtp(spec, os.path.join(str(model_dir), "smoke_lag8_gpu.mtp"))
+    path, opts, cell, pbc, types, pos = gpumd_io.gpumd_inputs(
+        os.path.join(gold, "run.in"), os.path.join(gold, "model.xyz"), mtp)
+    print(f"[case] gpumd smoke inputs options={opts}", flush=True)
+    ref = Oracle(path).compute(cell, types, pos, pbc=pbc)
+    got = sus2sh.evaluate(path, cell, types, pos, options=opts, pbc=pbc)
+    assert np.abs(ref[1]).max() > 1e-6  # the H-C pair interacts
+    _compare(ref, got, True)

@@ -357,6 +357,7 @@ void Engine::ensure_atoms(int N, size_t edge_bytes)
     satom_.resize(n * sizeof(int));
     wrapped_.resize(3 * n * sizeof(double));
     pos_s_.resize(3 * n * sizeof(double));
+    packed_.resize(n * sizeof(PackedAtom));
     type_s_.resize(n * sizeof(int));
     count_.resize(n * sizeof(int));
     basics_.resize(static_cast<size_t>(model_.basic_count) * n * rsz);
@@ -558,6 +559,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   cs.satom = satom_.as<int>();
   cs.skey = skey_.as<int>();
   cs.pos_s = pos_s_.as<double>();
+  cs.packed = packed_.as<PackedAtom>();
   cs.type_s = type_s_.as<int>();
   cs.bad_type = max_count_.as<int>() + 1;
   cs.sid_s = batch ? sid_s_.as<int>() : nullptr;
@@ -573,7 +575,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   na.rc2 = model_.max_dist * model_.max_dist;
   if (deferred) {
     na.cap = cap_;
-    launch_neighbors<Real>(na, pos_s_.as<double>(), type_s_.as<int>(), skey_.as<int>(),
+    launch_neighbors<Real>(na, pos_s_.as<double>(), packed_.as<PackedAtom>(), skey_.as<int>(),
                            cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
                            max_count_.as<int>(), s);
     launch_latch(max_count_.as<int>(), latch_.as<int>(), s);
@@ -582,7 +584,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       na.cap = cap_;
       if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
       launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
-      launch_neighbors<Real>(na, pos_s_.as<double>(), type_s_.as<int>(), skey_.as<int>(),
+      launch_neighbors<Real>(na, pos_s_.as<double>(), packed_.as<PackedAtom>(), skey_.as<int>(),
                              cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
                              max_count_.as<int>(), s);
       hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),

@@ -70,6 +70,12 @@ struct ParamGradOut {
   int S;
 };
 
+// One cell-sorted atom for the neighbour search: wrapped position and species (as a
+// double), 32 bytes, so a candidate is one cache-line-local pair of 16-byte loads.
+struct alignas(32) PackedAtom {
+  double x, y, z, t;
+};
+
 struct NeighborArgs {
   int N;
   CellGrid grid;              // single structure
@@ -107,6 +113,7 @@ struct CellSortArgs {
   int* satom;        // [N] atom at sorted position p
   int* skey;         // [N] cell of sorted position p
   double* pos_s;     // [3][N]
+  PackedAtom* packed;  // [N] position and species of sorted position p
   int* type_s;       // [N]
   int* bad_type;
   int* sid_s;
@@ -118,7 +125,7 @@ void launch_latch(const int* max_count, int* latch, hipStream_t s);
 void launch_fill_int(int* p, int value, size_t n, hipStream_t s);
 void launch_fill_double(double* p, double value, size_t n, hipStream_t s);
 template <typename Real>
-void launch_neighbors(const NeighborArgs& a, const double* pos_s, const int* type_s,
+void launch_neighbors(const NeighborArgs& a, const double* pos_s, const PackedAtom* packed,
                       const int* sorted_key, const int* cell_begin, const int* cell_end,
                       int* count, Edge<Real>* nl, int* max_count, hipStream_t s);
 // Neighbour index windows of each force tile of kWinTile consecutive cell-sorted atoms: the

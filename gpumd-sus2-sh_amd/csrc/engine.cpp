@@ -355,6 +355,7 @@ void Engine::ensure_atoms(int N, size_t edge_bytes)
     arrival_.resize(n * sizeof(int));
     skey_.resize(n * sizeof(int));
     satom_.resize(n * sizeof(int));
+    rank_.resize(n * sizeof(int));
     wrapped_.resize(3 * n * sizeof(double));
     pos_s_.resize(3 * n * sizeof(double));
     packed_.resize(n * sizeof(PackedAtom));
@@ -557,6 +558,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   cs.wrapped = wrapped_.as<double>();
   cs.sid = batch ? sid_.as<int>() : nullptr;
   cs.satom = satom_.as<int>();
+  cs.rank = rank_.as<int>();
   cs.skey = skey_.as<int>();
   cs.pos_s = pos_s_.as<double>();
   cs.packed = packed_.as<PackedAtom>();
@@ -759,7 +761,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
-  launch_scatter(N, satom_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
+  launch_scatter(N, rank_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
                  pot, force, virial, s);
   hip_check(hipGetLastError(), "kernel launch");
   if (param_grad) finish_param_grad(param_grad, s);

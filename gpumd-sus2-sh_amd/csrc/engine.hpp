@@ -173,7 +173,8 @@ class Engine {
 
   // per-atom work buffers
   int atoms_alloc_ = 0;
-  DeviceBuffer key_, slot_, arrival_, skey_, satom_, wrapped_, pos_s_, packed_, type_s_, count_;
+  DeviceBuffer key_, slot_, arrival_, skey_, satom_, rank_, wrapped_, pos_s_, packed_, type_s_;
+  DeviceBuffer count_;
   DeviceBuffer cell_count_, cell_begin_, scan_totals_, max_count_;
   DeviceBuffer latch_;  // deferred checks: max count, bad type, total edges (2), evaluations
   DeviceBuffer md_step_, md_x0_, md_v0_;

@@ -111,6 +111,7 @@ struct CellSortArgs {
   double* wrapped;   // [3][N]
   int* sid;
   int* satom;        // [N] atom at sorted position p
+  int* rank;         // [N] sorted position of atom i (inverse of satom)
   int* skey;         // [N] cell of sorted position p
   double* pos_s;     // [3][N]
   PackedAtom* packed;  // [N] position and species of sorted position p
@@ -220,7 +221,7 @@ void launch_energies(int N, const int* type, const double* mass, const double* p
                      const double* v, double* out, const int* step, hipStream_t s);
 
 // --- scatter of per-sorted-atom results to the caller's arrays ---
-void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,
+void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,
                     const double* v_s, double* potential, double* force, double* virial,
                     hipStream_t s);
 

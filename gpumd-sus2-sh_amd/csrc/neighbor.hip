@@ -176,7 +176,7 @@ __global__ void k_rank(int N, int S, const int* __restrict__ key, const int* __r
                        int* __restrict__ skey, double* __restrict__ pos_s,
                        PackedAtom* __restrict__ packed, int* __restrict__ type_s,
                        int* __restrict__ bad_type, const int* __restrict__ sid,
-                       int* __restrict__ sid_s)
+                       int* __restrict__ sid_s, int* __restrict__ rank)
 {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= N) return;
@@ -187,6 +187,7 @@ __global__ void k_rank(int N, int S, const int* __restrict__ key, const int* __r
   for (int u = b; u < e; ++u) r += arrival[u] < i;
   const int p = b + r;
   satom[p] = i;
+  rank[i] = p;
   skey[p] = k;
   pos_s[p] = wrapped[i];
   pos_s[N + p] = wrapped[N + i];
@@ -297,14 +298,16 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
   }
 }
 
-__global__ void k_scatter(int N, const int* __restrict__ satom, const double* __restrict__ e_s,
+// Caller order: lane i adds the results of its sorted position p = rank[i] -- the caller's
+// arrays are read-modify-written with coalesced accesses, the sorted arrays are gathered.
+__global__ void k_scatter(int N, const int* __restrict__ rank, const double* __restrict__ e_s,
                           const double* __restrict__ f_s, const double* __restrict__ v_s,
                           double* __restrict__ pot, double* __restrict__ force,
                           double* __restrict__ vir)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const int i = satom[p];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int p = rank[i];
   pot[i] += e_s[p];
 #pragma unroll
   for (int c = 0; c < 3; ++c) force[static_cast<size_t>(c) * N + i] += f_s[static_cast<size_t>(c) * N + p];
@@ -480,7 +483,7 @@ void launch_cell_sort(const CellSortArgs& a, hipStream_t s)
   k_place<<<blocks_for(N, 256), 256, 0, s>>>(N, a.key, a.slot, a.cell_begin, a.arrival);
   k_rank<<<blocks_for(N, 256), 256, 0, s>>>(N, a.S, a.key, a.arrival, a.cell_begin, a.wrapped,
                                              a.type, a.satom, a.skey, a.pos_s, a.packed,
-                                             a.type_s, a.bad_type, a.sid, a.sid_s);
+                                             a.type_s, a.bad_type, a.sid, a.sid_s, a.rank);
 }
 
 void launch_latch(const int* max_count, int* latch, hipStream_t s)
@@ -529,12 +532,11 @@ void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, i
   k_windows<<<blocks_for(N, kWinTile), 64, 0, s>>>(N, g, sorted_key, win, cap);
 }
 
-void launch_scatter(int N, const int* sorted_atom, const double* e_s, const double* f_s,
+void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,
                     const double* v_s, double* potential, double* force, double* virial,
                     hipStream_t s)
 {
-  k_scatter<<<blocks_for(N, 256), 256, 0, s>>>(N, sorted_atom, e_s, f_s, v_s, potential, force,
-                                                virial);
+  k_scatter<<<blocks_for(N, 256), 256, 0, s>>>(N, rank, e_s, f_s, v_s, potential, force, virial);
 }
 
 }  // namespace sus2sh

@@ -8,7 +8,7 @@
 // splits every segment's independent rows between them, so a wave runs one WPT-th of the
 // instructions and the CU holds WPT times more waves for the same LDS.  A barrier separates
 // segments (a segment reads what earlier segments wrote).  A group of NS instructions is one
-// pre-decoded record (LDS word offsets, row-end mask, keep factors, coefficients) that is
+// pre-decoded record (LDS word offsets, row-end mask, coefficients) that is
 // uniform across the wave and fetched with scalar loads, so a group costs no scalar decoding;
 // its NS accumulators are independent, so the group issues its NS (or 2 NS) LDS reads before
 // the first FMA that needs them, and the next record is fetched while they are in flight.
@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "lane_program.hpp"
 
 namespace sus2sh {
 
@@ -32,13 +33,13 @@ using cu32 = const __attribute__((address_space(4))) uint32_t;
 template <typename Real, int NS, bool BINARY>
 struct LaneRec {
   uint32_t xo[NS], yo[NS], to[NS], mask;
-  Real keep[NS], c[NS];
+  Real c[NS];
 };
 
 template <typename Real, int NS, bool BINARY>
 __device__ __forceinline__ void fetch_rec(cu32* r, LaneRec<Real, NS, BINARY>& q)
 {
-  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
+  constexpr int HW = lane_header_words(NS, static_cast<int>(sizeof(Real)));
   using creal = const __attribute__((address_space(4))) Real;
   creal* f = reinterpret_cast<creal*>(r + HW);
 #pragma unroll
@@ -46,8 +47,7 @@ __device__ __forceinline__ void fetch_rec(cu32* r, LaneRec<Real, NS, BINARY>& q)
     q.xo[j] = r[j];
     if (BINARY) q.yo[j] = r[NS + j];
     q.to[j] = r[2 * NS + j];
-    q.keep[j] = f[j];
-    q.c[j] = f[NS + j];
+    q.c[j] = f[j];
   }
   q.mask = r[3 * NS];
 }
@@ -119,8 +119,7 @@ template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
 {
   constexpr bool BIN = KIND != 1;
-  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
-  constexpr int RW = HW + 2 * NS * static_cast<int>(sizeof(Real)) / 4;
+  constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
   constexpr int SG = lane_batch<NS, Real>();
   Real acc[NS];
 #pragma unroll
@@ -167,8 +166,7 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
                                                                double* __restrict__ e_out,
                                                                Real* __restrict__ grad_out)
 {
-  constexpr int HW = (3 * NS + 1 + 3) / 4 * 4;
-  constexpr int RW = HW + 2 * NS * static_cast<int>(sizeof(Real)) / 4;
+  constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const unsigned lane = threadIdx.x & (kLaneWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLaneWave));

@@ -496,8 +496,7 @@ LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
         R[ns + j] = ((w >> 10) & 1023u) * col;
         R[2 * ns + j] = static_cast<uint32_t>(p.write_slot[base + j]) * col;
         if (end != kLaneEndNone) mask |= 1u << j;
-        put_real(R + hw + j * rs, p.keep[base + j] ? 1.0 : 0.0);
-        put_real(R + hw + (ns + j) * rs, p.coeff[base + j]);           // coefficient
+        put_real(R + hw + j * rs, p.coeff[base + j]);
       }
       R[3 * ns] = mask;
     }
@@ -538,20 +537,18 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
         for (int j = 0; j < ns; ++j) {
           xv[j] = X0[R[j] / col];
           yv[j] = bin ? X0[R[ns + j] / col] : 1.0;
-          acc[j] += get_real(R + hw + (ns + j) * rs) * xv[j] * yv[j];
+          acc[j] += get_real(R + hw + j * rs) * xv[j] * yv[j];
         }
         const uint32_t mask = R[3 * ns];
         for (int j = 0; j < ns; ++j) {
-          const double keep = get_real(R + hw + j * rs);
           const uint32_t t = R[2 * ns + j] / col;
           if (kind == kLaneBinaryStore || kind == kLaneUnaryStore) {
-            (void)keep;  // rows end on their mask bit
             if (mask & (1u << j)) {
               X[t] = acc[j];
               acc[j] = 0.0;
             }
           } else if (kind == kLaneBinaryEnergy) {
-            (void)keep;  // energy terms are summed at the end of the segment
+            // energy terms are summed at the end of the segment
           } else if (mask & (1u << j)) {
             X[t] += acc[j];
             acc[j] = 0.0;

@@ -87,10 +87,23 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
 //   word  3 nslot             row-end mask: bit j = instruction j ends its row
 //   then nslot keep factors (Real: 0 at a row end, else 1; acc *= keep after each group)
 //   and nslot coefficients (Real), starting at word header_words()
+// Device record of one group of ns instructions, every word used so that a batch of
+// records is a few wide scalar loads: ns x word offsets, ns y word offsets, ns target word
+// offsets, the row-end mask, then ns coefficients (8-byte coefficients start on an even
+// word, with one padding word when 3 ns + 1 is odd).
+constexpr int lane_header_words(int ns, int real_bytes)
+{
+  return real_bytes == 8 ? (3 * ns + 1 + 1) / 2 * 2 : 3 * ns + 1;
+}
+constexpr int lane_record_words(int ns, int real_bytes)
+{
+  return lane_header_words(ns, real_bytes) + ns * real_bytes / 4;
+}
+
 struct LaneRecords {
   int nslot = 0, real_bytes = 0;
-  int header_words() const { return (3 * nslot + 1 + 3) / 4 * 4; }
-  int rec_words() const { return header_words() + 2 * nslot * real_bytes / 4; }
+  int header_words() const { return lane_header_words(nslot, real_bytes); }
+  int rec_words() const { return lane_record_words(nslot, real_bytes); }
   std::vector<uint32_t> words;
   std::vector<int> seg_first;  // first record of each segment
 };

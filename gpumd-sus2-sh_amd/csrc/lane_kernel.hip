@@ -112,7 +112,9 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
 template <int NS, typename Real>
 constexpr int lane_batch()
 {
-  return NS * static_cast<int>(sizeof(Real)) <= 8 ? 4 : NS * static_cast<int>(sizeof(Real)) <= 16 ? 2 : 1;
+  // as many records as fit about 48 scalar registers, at most four
+  constexpr int rw = lane_record_words(NS, static_cast<int>(sizeof(Real)));
+  return 48 / rw < 1 ? 1 : (48 / rw > 4 ? 4 : 48 / rw);
 }
 
 template <typename Real, int NS, int KIND>

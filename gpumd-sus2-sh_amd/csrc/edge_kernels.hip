@@ -180,7 +180,8 @@ __device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
 // neighbour windows (launch_windows) and flushed with one global atomic per neighbour and
 // component; a tile without windows (count < 0) falls back to per-edge global atomics.
 template <typename Real, int L, int K, bool PG, int NB, bool WIN>
-__global__ __launch_bounds__(256) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8) ? 3 : 1))) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
                                                 double* __restrict__ v_s,
                                                 double* __restrict__ pg_rad,

@@ -305,6 +305,9 @@ def run_c3ref(quick):
         pot.close()
     out["fp32_over_fp64"] = out["fp32_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
     out["max_force_err_fp32_vs_fp64"] = float(np.abs(forces["fp32"] - forces["fp64"]).max())
+    out["rms_force_err_fp32_vs_fp64"] = float(np.sqrt(((forces["fp32"] - forces["fp64"]) ** 2).mean()))
+    out["max_abs_force_fp64"] = float(np.abs(forces["fp64"]).max())
+    out["rms_force_fp64"] = float(np.sqrt((forces["fp64"] ** 2).mean()))
     out["vs_reference_a100"] = out["fp32_atom_steps_per_s"] / 1.19500e7
     return out
 

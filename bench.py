@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the SUS2-SH potential path on one MI355X.
 
-Workload (BASELINE metric): Cu-Zr l3k3 at 1,024,000 atoms -- 80x80x40 fcc cells, a = 3.85 A,
+Workload: Cu-Zr l3k3 at 1,024,000 atoms -- 80x80x40 fcc cells, a = 3.85 A,
 50/50 species (seed 3001), Gaussian jitter sigma = 0.08 A (seed 3002); SUS2-SH model
 sh_l_max = 3, sh_k_max = 3, body order 5 with sh_body_l_max = {3,3,3,3} (the reference's
 l3333 graph: 1349 moments, 5475 products, 572 scalars), 8 RBChebyshev_sss functions per
@@ -45,8 +45,9 @@ def build_workload(nx, ny, nz, seed_model=3003):
 
 
 def kernel_flops_per_atom(info, nbar):
-    """Algorithmic FP32 flops per atom per kernel (FMA = 2; transcendentals not counted),
-    the per-atom formula of the survey's roofline accounting applied kernel by kernel."""
+    """Algorithmic FP32 flops per atom per kernel (FMA = 2; transcendentals not counted):
+    basic n (2B + 4 Mr rb + 4 (L+1)^2), graph 3 T + 3 (2T) + 2Q, force
+    n (8B + 8 Mr rb + 16 (L+1)^2 + 6 Mr + 18), with n the mean neighbour count."""
     L, K = info["l_max"], info["k_max"]
     B, Mr, rb = info["basic_count"], info["radial_funcs_count"], info["radial_basis_size"]
     nc = (L + 1) ** 2

@@ -1,8 +1,8 @@
 // sus2sh.hpp -- header-only C++ mirror of the GPUMD plugin class on top of the C ABI.
 //
 // `sus2sh::SUS2_SH` keeps the reference's constructor and compute() signatures
-// (src/force/sus2_sh.cuh:12-25) so a GPUMD tree can wrap it in its Potential subclass
-// (see INTEGRATION.md).  It is templated on the caller's Box / GPU_Vector types: anything
+// (src/force/sus2_sh.cuh:12-25) so a GPUMD tree can wrap it in its Potential subclass.
+// It is templated on the caller's Box / GPU_Vector types: anything
 // with `cpu_h[18]`, `pbc_x/pbc_y/pbc_z` (Box) and `data()` / `size()` (vectors) works.
 // Errors throw sus2sh::Error carrying the reference's message text; the reference prints
 // it and calls exit(1) (src/force/sus2_sh.cu:269-273).

@@ -8,9 +8,16 @@
 //                   W_i -= r_ij (x) g   (src/force/sus2_sh.cu:5996-6035, 6347-6446)
 // The backward pass recomputes R, R', Y and grad Y per edge instead of storing them.
 // Radial tables are staged in LDS (pair type varies per lane).
+//
+// The kernels are instantiated for 2 precisions x 5 values of l_max; each (precision, l_max)
+// pair is compiled as a unit of its own (SUS2SH_EDGE_REAL, SUS2SH_EDGE_L set by the build) so
+// that the build runs them in parallel.  Compiled without them, this file holds only the
+// dispatch on l_max.
 #include "kernels.hpp"
 
 namespace sus2sh {
+
+#if defined(SUS2SH_EDGE_REAL) && defined(SUS2SH_EDGE_L)
 
 namespace {
 
@@ -375,42 +382,35 @@ int uniform_nb(const EdgeLaunch<Real>& a)
   return a.rt.rb <= 8 ? 8 : 16;
 }
 
-#define SUS2SH_FOR_EACH_LK(X) \
-  X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(0, 5) X(0, 6) \
-  X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) \
-  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6) \
-  X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(3, 5) X(3, 6) \
-  X(4, 1) X(4, 2) X(4, 3) X(4, 4) X(4, 5) X(4, 6)
+#define SUS2SH_FOR_EACH_K(X) X(1) X(2) X(3) X(4) X(5) X(6)
 
 }  // namespace
 
-template <typename Real>
-bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
+template <typename Real, int LL>
+bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
 {
-  if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
   const int nb = uniform_nb(a);
   const size_t lds = table_bytes(a, nb);
-#define SUS2SH_BASICS_CASE(LL, KK)                                                \
-  if (a.L == LL && a.K == KK) {                                                   \
+#define SUS2SH_BASICS_CASE(KK)                                                    \
+  if (a.K == KK) {                                                                \
     if (nb == 8)                                                                  \
-      k_basics<Real, LL, KK, 8><<<blocks, 256, lds, s>>>(a, basics);               \
+      k_basics<Real, LL, KK, 8><<<blocks, 256, lds, s>>>(a, basics);              \
     else if (nb == 16)                                                            \
-      k_basics<Real, LL, KK, 16><<<blocks, 256, lds, s>>>(a, basics);              \
+      k_basics<Real, LL, KK, 16><<<blocks, 256, lds, s>>>(a, basics);             \
     else                                                                          \
-      k_basics<Real, LL, KK, 0><<<blocks, 256, lds, s>>>(a, basics);               \
+      k_basics<Real, LL, KK, 0><<<blocks, 256, lds, s>>>(a, basics);              \
     return true;                                                                  \
   }
-  SUS2SH_FOR_EACH_LK(SUS2SH_BASICS_CASE)
+  SUS2SH_FOR_EACH_K(SUS2SH_BASICS_CASE)
 #undef SUS2SH_BASICS_CASE
   return false;
 }
 
-template <typename Real>
-bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
-                   double* pg_rad, const int* win, int win_cap, hipStream_t s)
+template <typename Real, int LL>
+bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
+                     double* pg_rad, const int* win, int win_cap, hipStream_t s)
 {
-  if (a.N <= 0) return true;
   const int blocks = (a.N + 255) / 256;
   static_assert(kWinTile == 256, "force tiles are the 256-thread workgroups");
   const int nb = pg_rad ? 0 : uniform_nb(a);  // the parameter gradient uses the per-mu path
@@ -430,8 +430,8 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
     if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV>>();               \
     kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
   }
-#define SUS2SH_FORCES_CASE(LL, KK)                                                           \
-  if (a.L == LL && a.K == KK) {                                                              \
+#define SUS2SH_FORCES_CASE(KK)                                                               \
+  if (a.K == KK) {                                                                           \
     if (pg_rad) {                                                                            \
       SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false)                                           \
     } else if (use_win) {                                                                    \
@@ -445,10 +445,50 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
     }                                                                                        \
     return true;                                                                             \
   }
-  SUS2SH_FOR_EACH_LK(SUS2SH_FORCES_CASE)
+  SUS2SH_FOR_EACH_K(SUS2SH_FORCES_CASE)
 #undef SUS2SH_FORCES_CASE
 #undef SUS2SH_FORCES_LAUNCH
   return false;
+}
+
+template bool launch_basics_l<SUS2SH_EDGE_REAL, SUS2SH_EDGE_L>(const EdgeLaunch<SUS2SH_EDGE_REAL>&,
+                                                              SUS2SH_EDGE_REAL*, hipStream_t);
+template bool launch_forces_l<SUS2SH_EDGE_REAL, SUS2SH_EDGE_L>(
+    const EdgeLaunch<SUS2SH_EDGE_REAL>&, const SUS2SH_EDGE_REAL*, double*, double*, double*,
+    const int*, int, hipStream_t);
+
+#else  // dispatch on l_max
+
+template <typename Real>
+bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
+{
+  if (a.N <= 0) return true;
+  switch (a.L) {
+    case 0: return launch_basics_l<Real, 0>(a, basics, s);
+    case 1: return launch_basics_l<Real, 1>(a, basics, s);
+    case 2: return launch_basics_l<Real, 2>(a, basics, s);
+    case 3: return launch_basics_l<Real, 3>(a, basics, s);
+    case 4: return launch_basics_l<Real, 4>(a, basics, s);
+    default: return false;
+  }
+}
+
+template <typename Real>
+bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
+                   double* pg_rad, const int* win, int win_cap, hipStream_t s)
+{
+  if (a.N <= 0) return true;
+#define SUS2SH_FORCES_L(LL) \
+  case LL: return launch_forces_l<Real, LL>(a, grads, f_s, v_s, pg_rad, win, win_cap, s);
+  switch (a.L) {
+    SUS2SH_FORCES_L(0)
+    SUS2SH_FORCES_L(1)
+    SUS2SH_FORCES_L(2)
+    SUS2SH_FORCES_L(3)
+    SUS2SH_FORCES_L(4)
+    default: return false;
+  }
+#undef SUS2SH_FORCES_L
 }
 
 template bool launch_basics<float>(const EdgeLaunch<float>&, float*, hipStream_t);
@@ -457,5 +497,7 @@ template bool launch_forces<float>(const EdgeLaunch<float>&, const float*, doubl
                                    double*, const int*, int, hipStream_t);
 template bool launch_forces<double>(const EdgeLaunch<double>&, const double*, double*, double*,
                                     double*, const int*, int, hipStream_t);
+
+#endif
 
 }  // namespace sus2sh

@@ -175,6 +175,12 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);
 template <typename Real>
 bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
                    double* pg_rad, const int* win, int win_cap, hipStream_t s);
+// The l_max-specific halves of launch_basics / launch_forces (one compiled unit each).
+template <typename Real, int L>
+bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);
+template <typename Real, int L>
+bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
+                     double* pg_rad, const int* win, int win_cap, hipStream_t s);
 
 // --- graph kernel (graph_kernel.hip) ---
 // mode 0: rows / back chunks read per thread; mode 1: interleaved per-thread instruction

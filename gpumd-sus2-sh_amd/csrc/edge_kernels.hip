@@ -9,7 +9,7 @@
 // The backward pass recomputes R, R', Y and grad Y per edge instead of storing them.
 // Radial tables are staged in LDS (pair type varies per lane).
 //
-// The kernels are instantiated for 2 precisions x 5 values of l_max; each (precision, l_max)
+// The kernels are instantiated for 2 precisions x 7 values of l_max; each (precision, l_max)
 // pair is compiled as a unit of its own (SUS2SH_EDGE_REAL, SUS2SH_EDGE_L set by the build) so
 // that the build runs them in parallel.  Compiled without them, this file holds only the
 // dispatch on l_max.
@@ -186,9 +186,11 @@ __device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
 // WIN: neighbour forces of the tile (kWinTile atoms) are summed in LDS over the tile's
 // neighbour windows (launch_windows) and flushed with one global atomic per neighbour and
 // component; a tile without windows (count < 0) falls back to per-edge global atomics.
+// The FP32 windowed kernel with 8-coefficient rows is held to 168 VGPRs (3 waves per SIMD)
+// for l_max <= 4; the l_max = 5, 6 grids would spill far more at that budget.
 template <typename Real, int L, int K, bool PG, int NB, bool WIN>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8) ? 3 : 1))) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
+__attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 4) ? 3 : 1))) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
                                                 double* __restrict__ v_s,
                                                 double* __restrict__ pg_rad,
@@ -469,6 +471,8 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
     case 2: return launch_basics_l<Real, 2>(a, basics, s);
     case 3: return launch_basics_l<Real, 3>(a, basics, s);
     case 4: return launch_basics_l<Real, 4>(a, basics, s);
+    case 5: return launch_basics_l<Real, 5>(a, basics, s);
+    case 6: return launch_basics_l<Real, 6>(a, basics, s);
     default: return false;
   }
 }
@@ -486,6 +490,8 @@ bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, do
     SUS2SH_FORCES_L(2)
     SUS2SH_FORCES_L(3)
     SUS2SH_FORCES_L(4)
+    SUS2SH_FORCES_L(5)
+    SUS2SH_FORCES_L(6)
     default: return false;
   }
 #undef SUS2SH_FORCES_L

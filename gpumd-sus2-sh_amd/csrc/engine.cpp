@@ -98,7 +98,7 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
 {
   model_ = load_model(path);
   if (model_.l_max > kMaxL)
-    throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 4 only.");
+    throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 6 only.");
   if (model_.k_max > kMaxK) throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 6.");
   if (model_.species_count > kMaxSpecies)
     throw UnsupportedError("SUS2_SH: this engine supports at most 16 species.");

@@ -394,7 +394,7 @@ class GraphGenerator {
 StandardGraph build_standard_graph(int l_max, int k_max, int body_order,
                                    const std::vector<int>& body_l_max)
 {
-  if (l_max < 0 || l_max > kMaxLFile || k_max <= 0 || body_order < 2 || body_order > 6 ||
+  if (l_max < 0 || l_max > kMaxL || k_max <= 0 || body_order < 2 || body_order > 6 ||
       static_cast<int>(body_l_max.size()) < 7) {
     throw InputError("SUS2_SH standard graph: invalid generator arguments.");
   }
@@ -734,7 +734,7 @@ Model parse_model_text(const std::string& text)
     fail("SUS2_SH two-layer mu-body gate models are not supported by this engine.");
 
   if (m.scaling_map != "LK") fail("SUS2_SH requires scaling_map = LK.");
-  if (m.l_max < 0 || m.l_max > kMaxLFile) fail("SUS2_SH supports sh_l_max in [0,6].");
+  if (m.l_max < 0 || m.l_max > kMaxL) fail("SUS2_SH supports sh_l_max in [0,6].");
   if (m.body_order < 2 || m.body_order > 6) fail("SUS2_SH supports sh_body_order in [2,6].");
   if (m.k_max <= 0 || m.radial_funcs_count != m.k_max * (m.l_max + 1))
     fail("SUS2_SH inconsistent sh_k_max/radial_funcs_count.");

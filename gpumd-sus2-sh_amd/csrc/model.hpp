@@ -23,9 +23,8 @@ struct InputError : public std::runtime_error {
   explicit InputError(const std::string& msg) : std::runtime_error(msg) {}
 };
 
-// Limits of this engine. The GPU path evaluates real Y_lm for l <= 4 only.
-constexpr int kMaxL = 4;
-constexpr int kMaxLFile = 6;          // loader accepts sh_l_max in [0,6] (sus2_sh.cu:2273)
+// Limits of this engine (the reference's own, src/force/sus2_sh.cu:29-33, 2273).
+constexpr int kMaxL = 6;              // sh_l_max in [0,6]
 constexpr int kMaxK = 6;
 constexpr int kMaxRadialFuncs = 48;   // sus2_sh.cu:31
 constexpr int kMaxRbSize = 16;        // sus2_sh.cu:32

@@ -7,6 +7,9 @@ Pins:
       l3322   : rows 477, row terms 1416
       l4k4_4422: rows 968, row terms 2996
       l4k5_4422: rows 1730, row terms 5410
+      l6 {6,6,1,1}, k_max 1: A=67, B=49, T=67, Q=14, rows=18 -- the reference loader found
+                 this model's saved graph equal to the generated standard graph
+                 (docs/gpumd-sus2-sh-interface-plan.md:2044, 2217-2223)
   * textbook Clebsch-Gordan values and orthogonality;
   * real CG coefficients are real and couple real harmonics correctly (Gaunt check by
     quadrature against the oracle's independent Y_lm).
@@ -25,6 +28,7 @@ import sus2sh
     ("l3322", 3, 3, 5, (3, 3, 2, 2), dict(T=1416, rows=477)),
     ("l4k4_4422", 4, 4, 5, (4, 4, 2, 2), dict(T=2996, rows=968)),
     ("l4k5_4422", 4, 5, 5, (4, 4, 2, 2), dict(T=5410, rows=1730)),
+    ("l6_6611", 6, 1, 5, (6, 6, 1, 1), dict(A=67, B=49, T=67, Q=14, rows=18)),
 ])
 def test_reference_recorded_dimensions(name, L, K, body, blmax, expect):
     g = sus2sh.standard_graph(L, K, body, blmax)
@@ -108,7 +112,7 @@ def test_real_cg_couples_real_harmonics():
     """Gaunt check: the L=0 real coupling of two real harmonics of degree l is diagonal
     and uniform in m, (Y_l (x) Y_l)_0 = c_l sum_m Y_lm Y_lm with |c_l| = 1/sqrt(2l+1), and
     the real CG matrices are orthogonal in the coupled index."""
-    for l in range(0, 5):
+    for l in range(0, 7):
         vals = set()
         for m1, m2 in itertools.product(range(-l, l + 1), repeat=2):
             c = sus2sh.real_cg(l, m1, l, m2, 0, 0)
@@ -118,7 +122,7 @@ def test_real_cg_couples_real_harmonics():
                 vals.add(round(c, 12))
         assert len(vals) == 1
         assert abs(abs(vals.pop()) - 1.0 / math.sqrt(2 * l + 1)) < 1e-12
-    for l1, l2 in [(1, 1), (2, 1), (2, 2), (3, 2)]:
+    for l1, l2 in [(1, 1), (2, 1), (2, 2), (3, 2), (5, 1), (6, 2)]:
         for L in range(abs(l1 - l2), l1 + l2 + 1):
             for Lp in range(abs(l1 - l2), l1 + l2 + 1):
                 for M in range(-L, L + 1):
@@ -133,8 +137,8 @@ def test_real_cg_gaunt_quadrature():
     every u (with an (l1,l2,L) factor), checked with the oracle's closed-form Y_lm."""
     from oracle.binding import ylm
     pts, _ = _lebedev_like_grid(6, 7)
-    for l1, l2 in [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2)]:
-        for L in range(abs(l1 - l2), l1 + l2 + 1):
+    for l1, l2 in [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (4, 2), (5, 1)]:
+        for L in range(abs(l1 - l2), min(l1 + l2, 6) + 1):
             if (l1 + l2 + L) % 2:
                 continue  # odd couplings vanish pointwise for products of the same vector
             ratios = []

@@ -251,9 +251,14 @@ def test_invariances(l3k2_random):
 
 
 def test_rotation_invariance_pins_graph(make_model):
-    """Body-order-5 and -6 scalars are rotation invariant only with correct real CG."""
+    """Body-order-5 and -6 scalars are rotation invariant only with correct real CG (and, at
+    l_max = 5, 6, Y_lm and CG of those degrees in the same basis)."""
     for name, kw in [("rot5", dict(l_max=3, k_max=2, body_order=5, rb_size=5, max_dist=4.5, seed=31)),
-                     ("rot6", dict(l_max=2, k_max=2, body_order=6, rb_size=5, max_dist=4.5, seed=32))]:
+                     ("rot6", dict(l_max=2, k_max=2, body_order=6, rb_size=5, max_dist=4.5, seed=32)),
+                     ("rot_l5", dict(l_max=5, k_max=2, body_order=5, body_l_max=(5, 5, 1, 1),
+                                     rb_size=5, max_dist=4.5, seed=33)),
+                     ("rot_l6", dict(l_max=6, k_max=1, body_order=5, body_l_max=(6, 6, 1, 1),
+                                     rb_size=5, max_dist=4.5, seed=34))]:
         o = Oracle(make_model(name, **kw))
         rng = np.random.default_rng(3)
         pos = rng.uniform(-2.0, 2.0, size=(3, 9))

@@ -75,6 +75,34 @@ def standard_graph(l_max, k_max, body_order, body_l_max=None):
                          prod[:3 * T].reshape(T, 3), coef[:T], sc[:Q], sb[:Q], layers)
 
 
+def subgraph(g: StandardGraph, keep):
+    """Explicit product graph that computes only the scalars selected by the boolean mask
+    `keep` [Q]: the products those scalars depend on, with the non-basic moments renumbered
+    in their original (topological) order.  The result is a valid non-standard graph of the
+    kind the loader accepts through its explicit-graph path (src/force/sus2_sh.cu:1872-2103),
+    e.g. a pruned (q-total) training graph."""
+    keep = np.asarray(keep, dtype=bool)
+    B = g.basic_count
+    needed = np.zeros(g.node_count, dtype=bool)
+    needed[g.scalars[keep]] = True
+    used = np.zeros(g.product_count, dtype=bool)
+    for t in range(g.product_count - 1, -1, -1):
+        a, b, c = g.products[t]
+        if needed[c]:
+            used[t] = True
+            needed[a] = needed[b] = True
+    needed[:B] = True
+    new_id = np.cumsum(needed) - 1
+    prods = new_id[g.products[used]].astype(np.int32)
+    layer = np.zeros(g.node_count, dtype=np.int64)
+    for a, b, c in g.products[used]:
+        layer[c] = max(layer[c], max(layer[a], layer[b]) + 1)
+    return StandardGraph(g.l_max, g.k_max, g.body_order, g.body_l_max, int(needed.sum()),
+                         g.basic_klm.copy(), prods, g.coeffs[used].copy(),
+                         new_id[g.scalars[keep]].astype(np.int32), g.scalar_body[keep].copy(),
+                         int(layer.max()))
+
+
 @dataclass
 class ModelSpec:
     """Everything needed to write a SUS2-SH `.mtp` file."""

@@ -242,7 +242,8 @@ void Engine::upload_lane()
   const size_t nseg = lane_.seg_kind.size();
   std::vector<int4> segs(nseg);
   for (size_t k = 0; k < nseg; ++k)
-    segs[k] = make_int4(lane_.seg_kind[k], rec.seg_first[k], lane_.seg_groups[k], 0);
+    segs[k] = make_int4(lane_.seg_kind[k], rec.seg_first[k], lane_.seg_groups[k],
+                        lane_.seg_sync[k]);
   lane_segs_.upload(segs.data(), segs.size() * sizeof(int4));
   lane_ins_.upload(rec.words.data(), rec.words.size() * sizeof(uint32_t));
 }

@@ -6,12 +6,13 @@
 // lane i sits at word s * 64 + i, so a uniform slot index reads 64 consecutive words (no
 // bank conflicts).  The WPT waves of the workgroup serve the same 64 atoms: the program
 // splits every segment's independent rows between them, so a wave runs one WPT-th of the
-// instructions and the CU holds WPT times more waves for the same LDS.  A barrier separates
-// segments (a segment reads what earlier segments wrote).  A group of NS instructions is one
+// instructions and the CU holds WPT times more waves for the same LDS.  A barrier precedes
+// every segment that reads what earlier segments wrote (the segment table flags the others:
+// a back segment follows its energy segment without one).  A group of NS instructions is one
 // pre-decoded record (LDS word offsets, row-end mask, coefficients) that is
 // uniform across the wave and fetched with scalar loads, so a group costs no scalar decoding;
 // its NS accumulators are independent, so the group issues its NS (or 2 NS) LDS reads before
-// the first FMA that needs them, and the next record is fetched while they are in flight.
+// the first FMA that needs them, and the reads of the next groups are in flight meanwhile.
 // Segment kinds: 0 binary rows stored, 1 unary rows stored, 2 binary rows summed into the
 // site energy, 3 binary rows added to their targets.
 #include <algorithm>
@@ -117,6 +118,20 @@ constexpr int lane_batch()
   return 48 / rw < 1 ? 1 : (48 / rw > 4 ? 4 : 48 / rw);
 }
 
+// Wait until at most `ahead` LDS operations are outstanding (the reads of the next group),
+// so that a group's NS (or 2 NS) operands cost one wait instead of one per consumer.  The
+// compiler still adds any wait it needs; this one only makes its per-use waits redundant.
+// s_waitcnt immediate: vmcnt and expcnt at their maxima (no wait), lgkmcnt = ahead.
+template <int ahead>
+__device__ __forceinline__ void wait_lds_but()
+{
+  if constexpr (ahead <= 15) {
+    __builtin_amdgcn_sched_barrier(0);  // the wait stays between the reads and their uses
+    __builtin_amdgcn_s_waitcnt(0xC07F | (ahead << 8));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
 {
@@ -126,7 +141,13 @@ __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& 
   Real acc[NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = Real(0);
-  Real xa[NS], ya[NS], xb[NS], yb[NS];
+  // operands of DEPTH + 1 groups in flight: group k + DEPTH is read before group k is
+  // finished (a segment never reads what it writes, so reads may pass the row-end stores)
+  constexpr int kReads = BIN ? 2 * NS : NS;  // LDS reads of one group
+  constexpr int DEPTH = (SG > 2 && 2 * kReads <= 15) ? 2 : (SG > 1 ? 1 : 0);
+  Real xv[DEPTH + 1][NS], yv[DEPTH + 1][NS];
+  Real(&xa)[NS] = xv[0];
+  Real(&ya)[NS] = yv[0];
   int g = 0;
   for (; g + SG <= groups; g += SG) {
     LaneRec<Real, NS, BIN> r[SG];
@@ -134,17 +155,21 @@ __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& 
     for (int k = 0; k < SG; ++k) fetch_rec<Real, NS, BIN>(R + k * RW, r[k]);
 #pragma unroll
     for (int k = 0; k < SG; ++k) pin_rec<Real, NS, BIN>(r[k]);
-    read_rec<Real, NS, BIN>(r[0], X, xa, ya);
+#pragma unroll
+    for (int k = 0; k < DEPTH; ++k) read_rec<Real, NS, BIN>(r[k], X, xv[k], yv[k]);
 #pragma unroll
     for (int k = 0; k < SG; ++k) {
-      const int next = k + 1 < SG ? k + 1 : k;  // in range also where it is not used
-      if (k % 2 == 0) {
-        if (k + 1 < SG) read_rec<Real, NS, BIN>(r[next], X, xb, yb);
-        finish_rec<Real, NS, KIND>(r[k], xa, ya, acc, X, e);
-      } else {
-        if (k + 1 < SG) read_rec<Real, NS, BIN>(r[next], X, xa, ya);
-        finish_rec<Real, NS, KIND>(r[k], xb, yb, acc, X, e);
+      if (k + DEPTH < SG) {
+        constexpr int d = DEPTH;
+        read_rec<Real, NS, BIN>(r[k + d < SG ? k + d : 0], X, xv[(k + d) % (d + 1)],
+                                yv[(k + d) % (d + 1)]);
       }
+      // groups still in flight after group k: min(DEPTH, SG - 1 - k)
+      const int inflight = SG - 1 - k < DEPTH ? SG - 1 - k : DEPTH;
+      if (inflight == 2) wait_lds_but<2 * kReads>();
+      else if (inflight == 1) wait_lds_but<kReads>();
+      else wait_lds_but<0>();
+      finish_rec<Real, NS, KIND>(r[k], xv[k % (DEPTH + 1)], yv[k % (DEPTH + 1)], acc, X, e);
     }
     R += SG * RW;
   }
@@ -185,9 +210,9 @@ __global__ __launch_bounds__(kLaneWave * WPT) void k_lane_graph(int N, const Rea
 
   Real e = 0;
   for (int s = 0; s < lt.nseg; ++s) {
-    __syncthreads();  // every wave runs every segment: uniform barrier
-    cu32* sg = (cu32*)lt.segs + 4 * s;  // (kind, first record, groups, 0)
-    const uint32_t kind = sg[0], first = sg[1], groups = sg[2];
+    cu32* sg = (cu32*)lt.segs + 4 * s;  // (kind, first record, groups, barrier before)
+    const uint32_t kind = sg[0], first = sg[1], groups = sg[2], sync = sg[3];
+    if (sync) __syncthreads();  // every wave runs every segment: uniform barrier
     cu32* R = (cu32*)lt.recs + (static_cast<size_t>(first) + static_cast<size_t>(wave) * groups) * RW;
     switch (kind) {
       case 0: run_segment<Real, NS, 0>(R, static_cast<int>(groups), X, e); break;

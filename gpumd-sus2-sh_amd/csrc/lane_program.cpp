@@ -33,7 +33,7 @@ int find_root(std::vector<int>& parent, int x)
 // instructions of wave w form groups of nslot (slot j feeds accumulator j); every wave of a
 // segment has the same number of groups (padding: c = 0 on the constant slot).
 // Layout: instruction j of group g of wave w at begin + (w * groups + g) * nslot + j.
-void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
+void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows, bool sync = true)
 {
   const int ns = p.nslot, bins = p.nslot * p.waves;
   const bool store = kind == kLaneBinaryStore || kind == kLaneUnaryStore;
@@ -89,6 +89,7 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows)
     }
   }
   p.seg_kind.push_back(kind);
+  p.seg_sync.push_back(sync || p.seg_kind.size() == 1 ? 1 : 0);
   p.seg_begin.push_back(static_cast<int>(p.ins.size()));
   p.seg_groups.push_back(static_cast<int>(groups));
   for (int w = 0; w < p.waves; ++w)
@@ -329,7 +330,9 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
       r.terms = std::move(gb[q]);
       eb.push_back(std::move(r));
     }
-    emit_segment(p, kLaneBinaryAdd, eb);
+    // BWD reads what ENG reads (values, adjoints) and adds into the basic adjoints, which
+    // ENG does not read: both run in one epoch
+    emit_segment(p, kLaneBinaryAdd, eb, false);
   }
   if (p.needs_trash && !with_trash) {
     p.reason = "a store segment has an empty accumulator: needs a trash slot";
@@ -359,11 +362,13 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
       }
     }
   }
-  // waves of a tile run a segment concurrently: within a segment every stored / added slot
-  // is the target of one row only and is never read
+  // waves of a tile run an epoch (segments between two barriers) concurrently: within an
+  // epoch every stored / added slot is the target of one row only and is never read
   {
     std::vector<int> mark(p.slots, -1);
+    int epoch = -1;
     for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+      if (p.seg_sync[s]) ++epoch;
       const size_t b = static_cast<size_t>(p.seg_begin[s]);
       const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot * waves;
       for (size_t k = b; k < e; ++k) {
@@ -374,20 +379,40 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
             p.reason = "lane program writes a reserved slot";
             return p;
           }
-          if (mark[t] == static_cast<int>(s)) {
-            p.reason = "lane program writes one slot twice in a segment";
+          if (mark[t] == epoch) {
+            p.reason = "lane program writes one slot twice in an epoch";
             return p;
           }
-          mark[t] = static_cast<int>(s);
+          mark[t] = epoch;
         }
       }
       for (size_t k = b; k < e; ++k) {
         const uint32_t w = p.ins[k];
-        if (mark[w & 1023u] == static_cast<int>(s) ||
-            (p.seg_kind[s] != kLaneUnaryStore && mark[(w >> 10) & 1023u] == static_cast<int>(s))) {
-          p.reason = "lane program reads a slot its segment writes";
+        if (mark[w & 1023u] == epoch ||
+            (p.seg_kind[s] != kLaneUnaryStore && mark[(w >> 10) & 1023u] == epoch)) {
+          p.reason = "lane program reads a slot its epoch writes";
           return p;
         }
+      }
+    }
+    // reads of an earlier segment of the epoch against the writes of a later one
+    std::vector<int> rmark(p.slots, -1);
+    epoch = -1;
+    for (size_t s = 0; s < p.seg_kind.size(); ++s) {
+      if (p.seg_sync[s]) ++epoch;
+      const size_t b = static_cast<size_t>(p.seg_begin[s]);
+      const size_t e = b + static_cast<size_t>(p.seg_groups[s]) * nslot * waves;
+      for (size_t k = b; k < e; ++k) {
+        const uint32_t w = p.ins[k], end = w >> 30;
+        if ((end == kLaneEndStore || end == kLaneEndAdd) && rmark[(w >> 20) & 1023u] == epoch) {
+          p.reason = "lane program writes a slot its epoch reads";
+          return p;
+        }
+      }
+      for (size_t k = b; k < e; ++k) {
+        const uint32_t w = p.ins[k];
+        rmark[w & 1023u] = epoch;
+        if (p.seg_kind[s] != kLaneUnaryStore) rmark[(w >> 10) & 1023u] = epoch;
       }
     }
   }

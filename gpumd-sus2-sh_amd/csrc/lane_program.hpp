@@ -50,6 +50,9 @@ struct LaneProgram {
   long long useful_terms = 0;  // instructions that are not padding
   // segments: {kind (0 binary, 1 unary), first instruction, groups}
   std::vector<int> seg_kind, seg_begin, seg_groups;
+  // 1: the waves of a tile synchronise before the segment; 0: it runs in the same epoch as
+  // the segment before it (no slot written in an epoch is read in it or written twice)
+  std::vector<uint8_t> seg_sync;
   std::vector<uint32_t> ins;
   std::vector<double> coeff;
   // device form of the row ends: slot each instruction's group writes its accumulator to

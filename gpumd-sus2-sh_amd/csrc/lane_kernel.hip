@@ -82,7 +82,8 @@ __device__ __forceinline__ void read_rec(const LaneRec<Real, NS, BINARY>& q, con
 
 // Accumulate a group and end its rows: a row ends on a uniform bit of the record's mask
 // (scalar test and branch; most groups end no row), where the accumulator is stored (kinds
-// 0, 1) or added (kind 3) to the row's target and restarted.  Energy rows (kind 2) are summed
+// 0, 1) or added (kind 3) to the row's target and restarted.  Mask bit NS ends a row split
+// between accumulators 0 and 1: accumulator 1 is added to accumulator 0 first.  Energy rows (kind 2) are summed
 // once at the end of the segment.
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q,
@@ -93,6 +94,12 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
   for (int j = 0; j < NS; ++j) acc[j] += KIND != 1 ? (q.c[j] * xv[j]) * yv[j] : q.c[j] * xv[j];
   (void)e;
   if (KIND != 2 && q.mask != 0u) {
+    if constexpr (NS >= 2) {
+      if (q.mask & (1u << NS)) {  // a split row ends: its odd half joins accumulator 0
+        acc[0] += acc[1];
+        acc[1] = Real(0);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       if (q.mask & (1u << j)) {
@@ -106,16 +113,16 @@ __device__ __forceinline__ void finish_rec(const LaneRec<Real, NS, KIND != 1>& q
   }
 }
 
-// One segment of one wave: `groups` records from R.  Records are fetched SG at a time
-// (one batch of scalar loads at immediate offsets from one pointer, then one wait), so the
-// scalar unit spends a pointer increment and a loop test per SG groups; inside a batch the
-// LDS reads of group g + 1 are issued before group g is finished.
+// Records per batch.  A batch is one set of scalar loads at immediate offsets from one
+// pointer and one wait (the program does not fit the scalar cache: each batch is a round
+// trip to L2), so the scalar unit spends a pointer increment and a loop test per batch.
 template <int NS, typename Real>
 constexpr int lane_batch()
 {
-  // as many records as fit about 48 scalar registers, at most four
+  // as many records as fit about 64 (FP32) / 48 (FP64) scalar registers, at most 6 / 4
   constexpr int rw = lane_record_words(NS, static_cast<int>(sizeof(Real)));
-  return 48 / rw < 1 ? 1 : (48 / rw > 4 ? 4 : 48 / rw);
+  constexpr int budget = sizeof(Real) == 4 ? 64 : 48, cap = sizeof(Real) == 4 ? 6 : 4;
+  return budget / rw < 1 ? 1 : (budget / rw > cap ? cap : budget / rw);
 }
 
 // Wait until at most `ahead` LDS operations are outstanding (the reads of the next group),
@@ -132,54 +139,68 @@ __device__ __forceinline__ void wait_lds_but()
   }
 }
 
+// CNT records fetched together (one batch of scalar loads, one wait), then run with the
+// operands of DEPTH + 1 groups in flight: group k + DEPTH is read before group k is finished
+// (a segment never reads what it writes, so reads may pass the row-end stores).
+template <typename Real, int NS, int KIND, int CNT>
+__device__ __forceinline__ void run_batch(cu32* R, Real* X, Real& e, Real (&acc)[NS])
+{
+  constexpr bool BIN = KIND != 1;
+  constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
+  constexpr int kReads = BIN ? 2 * NS : NS;  // LDS reads of one group
+  constexpr int DEPTH = (CNT > 2 && 2 * kReads <= 15) ? 2 : (CNT > 1 ? 1 : 0);
+  Real xv[DEPTH + 1][NS], yv[DEPTH + 1][NS];
+  LaneRec<Real, NS, BIN> r[CNT];
+#pragma unroll
+  for (int k = 0; k < CNT; ++k) fetch_rec<Real, NS, BIN>(R + k * RW, r[k]);
+#pragma unroll
+  for (int k = 0; k < CNT; ++k) pin_rec<Real, NS, BIN>(r[k]);
+#pragma unroll
+  for (int k = 0; k < DEPTH; ++k) read_rec<Real, NS, BIN>(r[k], X, xv[k], yv[k]);
+#pragma unroll
+  for (int k = 0; k < CNT; ++k) {
+    if (k + DEPTH < CNT) {
+      constexpr int d = DEPTH;
+      read_rec<Real, NS, BIN>(r[k + d < CNT ? k + d : 0], X, xv[(k + d) % (d + 1)],
+                              yv[(k + d) % (d + 1)]);
+    }
+    // groups still in flight after group k: min(DEPTH, CNT - 1 - k)
+    const int inflight = CNT - 1 - k < DEPTH ? CNT - 1 - k : DEPTH;
+    if (inflight == 2) wait_lds_but<2 * kReads>();
+    else if (inflight == 1) wait_lds_but<kReads>();
+    else wait_lds_but<0>();
+    finish_rec<Real, NS, KIND>(r[k], xv[k % (DEPTH + 1)], yv[k % (DEPTH + 1)], acc, X, e);
+  }
+}
+
+// The last `rest` (< C + 1) records of a segment as one batch.
+template <typename Real, int NS, int KIND, int C>
+__device__ __forceinline__ void run_tail(cu32* R, int rest, Real* X, Real& e, Real (&acc)[NS])
+{
+  if constexpr (C > 0) {
+    if (rest == C)
+      run_batch<Real, NS, KIND, C>(R, X, e, acc);
+    else
+      run_tail<Real, NS, KIND, C - 1>(R, rest, X, e, acc);
+  }
+}
+
+// One segment of one wave: `groups` records from R, SG at a time; the remainder is one
+// shorter batch, so a segment costs ceil(groups / SG) scalar-load round trips.
 template <typename Real, int NS, int KIND>
 __device__ __forceinline__ void run_segment(cu32* R, int groups, Real* X, Real& e)
 {
-  constexpr bool BIN = KIND != 1;
   constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
   constexpr int SG = lane_batch<NS, Real>();
   Real acc[NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = Real(0);
-  // operands of DEPTH + 1 groups in flight: group k + DEPTH is read before group k is
-  // finished (a segment never reads what it writes, so reads may pass the row-end stores)
-  constexpr int kReads = BIN ? 2 * NS : NS;  // LDS reads of one group
-  constexpr int DEPTH = (SG > 2 && 2 * kReads <= 15) ? 2 : (SG > 1 ? 1 : 0);
-  Real xv[DEPTH + 1][NS], yv[DEPTH + 1][NS];
-  Real(&xa)[NS] = xv[0];
-  Real(&ya)[NS] = yv[0];
   int g = 0;
   for (; g + SG <= groups; g += SG) {
-    LaneRec<Real, NS, BIN> r[SG];
-#pragma unroll
-    for (int k = 0; k < SG; ++k) fetch_rec<Real, NS, BIN>(R + k * RW, r[k]);
-#pragma unroll
-    for (int k = 0; k < SG; ++k) pin_rec<Real, NS, BIN>(r[k]);
-#pragma unroll
-    for (int k = 0; k < DEPTH; ++k) read_rec<Real, NS, BIN>(r[k], X, xv[k], yv[k]);
-#pragma unroll
-    for (int k = 0; k < SG; ++k) {
-      if (k + DEPTH < SG) {
-        constexpr int d = DEPTH;
-        read_rec<Real, NS, BIN>(r[k + d < SG ? k + d : 0], X, xv[(k + d) % (d + 1)],
-                                yv[(k + d) % (d + 1)]);
-      }
-      // groups still in flight after group k: min(DEPTH, SG - 1 - k)
-      const int inflight = SG - 1 - k < DEPTH ? SG - 1 - k : DEPTH;
-      if (inflight == 2) wait_lds_but<2 * kReads>();
-      else if (inflight == 1) wait_lds_but<kReads>();
-      else wait_lds_but<0>();
-      finish_rec<Real, NS, KIND>(r[k], xv[k % (DEPTH + 1)], yv[k % (DEPTH + 1)], acc, X, e);
-    }
+    run_batch<Real, NS, KIND, SG>(R, X, e, acc);
     R += SG * RW;
   }
-  for (; g < groups; ++g) {
-    LaneRec<Real, NS, BIN> r;
-    fetch_rec<Real, NS, BIN>(R, r);
-    read_rec<Real, NS, BIN>(r, X, xa, ya);
-    finish_rec<Real, NS, KIND>(r, xa, ya, acc, X, e);
-    R += RW;
-  }
+  run_tail<Real, NS, KIND, SG - 1>(R, groups - g, X, e, acc);
   if (KIND == 2) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) e += acc[j];

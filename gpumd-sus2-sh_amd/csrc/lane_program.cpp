@@ -44,48 +44,83 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows, bool sync = 
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return rows[a].terms.size() > rows[b].terms.size(); });
-  std::vector<std::vector<int>> per(bins);
+  // rows longer than the mean bin load are split between accumulators 0 and 1 of a wave
+  // (energy rows are already split into parts); split rows go first, one wave after another
+  size_t total = 0;
+  for (const Row& r : rows) total += r.terms.size();
+  const size_t mean = (total + bins - 1) / bins;
+  // per[j]: (row, part) with part 0 the whole row, 1 its even terms, 2 its odd terms
+  std::vector<std::vector<std::pair<int, int>>> per(bins);
   std::vector<size_t> load(bins, 0);
-  for (int r : order) {
+  size_t first = 0;
+  if (ns >= 2 && kind != kLaneBinaryEnergy) {
+    int w = 0;
+    for (; first < order.size(); ++first) {
+      const size_t n = rows[order[first]].terms.size();
+      if (n < 4 || n <= mean) break;
+      per[w * ns].push_back({order[first], 1});
+      per[w * ns + 1].push_back({order[first], 2});
+      load[w * ns] += (n + 1) / 2;
+      load[w * ns + 1] += (n + 1) / 2;
+      w = (w + 1) % p.waves;
+    }
+  }
+  for (size_t i = first; i < order.size(); ++i) {
+    const int r = order[i];
     const int j = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-    per[j].push_back(r);
+    per[j].push_back({r, 0});
     load[j] += rows[r].terms.size();
   }
   const size_t groups = *std::max_element(load.begin(), load.end());
   std::vector<std::vector<uint32_t>> words(bins);
   std::vector<std::vector<double>> cs(bins);
   std::vector<std::vector<uint16_t>> wt(bins);
-  std::vector<std::vector<uint8_t>> kp(bins);
+  std::vector<std::vector<uint8_t>> kp(bins), cb(bins);
+  const uint32_t one = static_cast<uint32_t>(p.ones_slot);
   for (int j = 0; j < bins; ++j) {
-    // store segments write the accumulator after every group: partial sums go to the row's
-    // own target (never read inside the segment), the last row of a bin keeps its sum so
-    // that the padding after it rewrites the same value; a bin with no row writes the trash
-    // slot
+    // the last row of a bin keeps its sum: the padding after it leaves it unchanged; a
+    // store bin with no row writes the trash slot
     uint16_t last_target = static_cast<uint16_t>(p.trash_slot);
     if (store && per[j].empty()) p.needs_trash = true;
     for (size_t ri = 0; ri < per[j].size(); ++ri) {
-      const Row& row = rows[per[j][ri]];
+      const Row& row = rows[per[j][ri].first];
+      const int part = per[j][ri].second;
       const bool last_row = ri + 1 == per[j].size();
-      for (size_t k = 0; k < row.terms.size(); ++k) {
-        const Term& t = row.terms[k];
-        const bool last = k + 1 == row.terms.size();
-        const uint32_t end = last ? row.end : kLaneEndNone;
-        const uint32_t tgt = last && row.end != kLaneEndEnergy ? static_cast<uint32_t>(row.target) : 0u;
-        words[j].push_back(static_cast<uint32_t>(t.x) | (static_cast<uint32_t>(t.y) << 10) |
-                           (tgt << 20) | (end << 30));
-        cs[j].push_back(t.c);
+      std::vector<const Term*> ts;
+      for (size_t k = part == 2 ? 1 : 0; k < row.terms.size(); k += part == 0 ? 1 : 2)
+        ts.push_back(&row.terms[k]);
+      // both halves of a split row are (n + 1) / 2 long: the odd half of an odd row starts
+      // with a zero term
+      const size_t len = part == 0 ? row.terms.size() : (row.terms.size() + 1) / 2;
+      for (size_t k = 0; k < len; ++k) {
+        const size_t pad = len - ts.size();
+        const bool last = k + 1 == len;
+        if (k < pad) {
+          words[j].push_back(one | (one << 10));
+          cs[j].push_back(0.0);
+        } else {
+          const Term& t = *ts[k - pad];
+          // the odd half of a split row never ends: accumulator 0 ends the row
+          const uint32_t end = last && part != 2 ? row.end : kLaneEndNone;
+          const uint32_t tgt = end != kLaneEndNone && row.end != kLaneEndEnergy
+                                 ? static_cast<uint32_t>(row.target) : 0u;
+          words[j].push_back(static_cast<uint32_t>(t.x) | (static_cast<uint32_t>(t.y) << 10) |
+                             (tgt << 20) | (end << 30));
+          cs[j].push_back(t.c);
+          ++p.useful_terms;
+        }
         wt[j].push_back(static_cast<uint16_t>(row.target));
-        kp[j].push_back(last && !(store && last_row) ? 0 : 1);
-        ++p.useful_terms;
+        kp[j].push_back(last && part != 2 && !(store && last_row) ? 0 : 1);
+        cb[j].push_back(last && part == 1 ? 1 : 0);
       }
       last_target = static_cast<uint16_t>(row.target);
     }
-    const uint32_t one = static_cast<uint32_t>(p.ones_slot);
     while (words[j].size() < groups) {
       words[j].push_back(one | (one << 10));
       cs[j].push_back(0.0);
       wt[j].push_back(last_target);
       kp[j].push_back(1);
+      cb[j].push_back(0);
     }
   }
   p.seg_kind.push_back(kind);
@@ -99,6 +134,7 @@ void emit_segment(LaneProgram& p, int kind, std::vector<Row>& rows, bool sync = 
         p.coeff.push_back(cs[w * ns + j][g]);
         p.write_slot.push_back(wt[w * ns + j][g]);
         p.keep.push_back(kp[w * ns + j][g]);
+        p.combine.push_back(cb[w * ns + j][g]);
       }
 }
 
@@ -472,6 +508,10 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
           yv[j] = bin ? X0[(wd >> 10) & 1023u] : 1.0;
         }
         for (int j = 0; j < ns; ++j) acc[j] += p.coeff[base + j] * xv[j] * yv[j];
+        if (p.combine[base]) {
+          acc[0] += acc[1];
+          acc[1] = 0.0;
+        }
         for (int j = 0; j < ns; ++j) {
           const uint32_t wd = p.ins[base + j];
           const uint32_t end = wd >> 30, t = (wd >> 20) & 1023u;
@@ -523,6 +563,7 @@ LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
         if (end != kLaneEndNone) mask |= 1u << j;
         put_real(R + hw + j * rs, p.coeff[base + j]);
       }
+      if (p.combine[base]) mask |= 1u << ns;
       R[3 * ns] = mask;
     }
   }
@@ -565,6 +606,10 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
           acc[j] += get_real(R + hw + j * rs) * xv[j] * yv[j];
         }
         const uint32_t mask = R[3 * ns];
+        if (kind != kLaneBinaryEnergy && (mask & (1u << ns))) {
+          acc[0] += acc[1];
+          acc[1] = 0.0;
+        }
         for (int j = 0; j < ns; ++j) {
           const uint32_t t = R[2 * ns + j] / col;
           if (kind == kLaneBinaryStore || kind == kLaneUnaryStore) {

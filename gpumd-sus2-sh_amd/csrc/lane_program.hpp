@@ -26,7 +26,8 @@
 // [v_base, v_base + cap_v) component values, [g_base, g_base + cap_g) component adjoints.
 // Instruction word: x | y << 10 | t << 20 | end << 30 with end 0 (none), 1 (slot t = acc),
 // 2 (slot t += acc), 3 (site energy += acc).  Binary segments add c X[x] X[y], unary
-// segments add c X[x].
+// segments add c X[x].  A row much longer than a segment's mean bin load is split between
+// accumulators 0 and 1 of one wave (alternate terms) and combined at its end.
 #pragma once
 
 #include <cstdint>
@@ -61,6 +62,9 @@ struct LaneProgram {
   // a store accumulator, whose sum is rewritten unchanged by the padding after it)
   std::vector<uint16_t> write_slot;
   std::vector<uint8_t> keep;
+  // 1 on accumulator 0 of the group that ends a split row: accumulator 1 (the row's other
+  // half, in the same wave) is added to accumulator 0 before the row end, then cleared
+  std::vector<uint8_t> combine;
   bool needs_trash = false;  // a store segment has an accumulator without rows
 };
 

@@ -202,7 +202,8 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
   LaneProgram best;
   best.reason = "no slot budget fits the LDS of a workgroup";
-  for (int max_slots : {160, 192, 256, 320, 384, 512, 640, 1024}) {
+  // 208 FP32 slots: three 64-atom tiles (52 KiB each) still share one CU's 160 KiB
+  for (int max_slots : {160, 192, 208, 256, 320, 384, 512, 640, 1024}) {
     if (max_slots < shape.min_slots) continue;
     if (static_cast<size_t>(max_slots) * 64 * rsz > max_lds_) break;
     LaneProgram p = compile_lane_program(model_, max_slots, shape.nslot, shape.waves);
@@ -218,7 +219,7 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
 const std::vector<Engine::LaneShape>& Engine::lane_shapes()
 {
   static const std::vector<LaneShape> shapes = {
-    {4, 4, 160}, {8, 2, 160}, {4, 2, 160}, {8, 2, 192}, {8, 2, 256}, {2, 4, 160}};
+    {4, 4, 160}, {8, 2, 160}, {4, 2, 160}, {8, 2, 192}, {8, 2, 256}, {2, 4, 160}, {8, 2, 208}};
   return shapes;
 }
 

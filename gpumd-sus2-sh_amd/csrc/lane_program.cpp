@@ -234,8 +234,7 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     if (m.products[q].right != m.products[q].left) l2_of[m.products[q].right].push_back(q);
   }
 
-  // region sizes: the free slots are split between values and adjoints in proportion to
-  // the needs of the largest component
+  // region sizes: the free slots are split between values and adjoints
   auto v_need = [&](const std::vector<int>& c) {
     int k = 0;
     for (int n : c) k += n >= B ? 1 : 0;
@@ -252,7 +251,16 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     p.reason = "a graph component exceeds the per-lane slot budget";
     return p;
   }
-  p.cap_v = ng_max > 0 ? std::max(nv_max, free_slots * nv_max / (nv_max + ng_max)) : 0;
+  // split in proportion to the total needs (the batches are then filled evenly), but never
+  // below the largest component's need
+  long long nv_all = 0, ng_all = 0;
+  for (const std::vector<int>& c : comps) {
+    nv_all += v_need(c);
+    ng_all += static_cast<long long>(c.size());
+  }
+  p.cap_v = ng_max > 0 ? std::max(nv_max, static_cast<int>(free_slots * nv_all / (nv_all + ng_all)))
+                       : 0;
+  if (free_slots - p.cap_v < ng_max) p.cap_v = free_slots - ng_max;
   p.cap_g = free_slots - p.cap_v;
   p.g_base = p.v_base + p.cap_v;
   p.slots = p.g_base + p.cap_g;

@@ -29,7 +29,7 @@ __host__ __device__ constexpr int padded_stride(int n) { return n + ((1 - n % 32
 __host__ __device__ constexpr int coeff_row(int NB, int rb) { return NB > 0 ? NB : rb; }
 
 // Pair stride (in Reals) of the coefficient table.  The uniform kernels (NB > 0) read their
-// 8- or 16-coefficient rows as 16-byte vectors, so each pair table starts 16-byte aligned, 4
+// 8-, 12- or 16-coefficient rows as 16-byte vectors, so each pair table starts 16-byte aligned, 4
 // words past a multiple of 32 banks: the pair tables that the lanes of a wave read at once
 // then sit in disjoint bank quads.  Scalar rows (NB = 0) keep the 1 mod 32 padding.
 template <typename Real>
@@ -376,12 +376,13 @@ size_t table_bytes(const EdgeLaunch<Real>& a, int NB)
          (coeff_stride<Real>(NB, Mr * coeff_row(NB, a.rt.rb)) + padded_stride(Mr * 2)) * sizeof(Real);
 }
 
-// NB of the uniform kernels: coefficient rows padded to 8 or 16; 0 selects the per-mu path.
+// NB of the uniform kernels: coefficient rows padded to 8, 12 or 16 (rb = 10, the common
+// production size, pads to 12); 0 selects the per-mu path.
 template <typename Real>
 int uniform_nb(const EdgeLaunch<Real>& a)
 {
   if (!a.rt.uniform) return 0;
-  return a.rt.rb <= 8 ? 8 : 16;
+  return a.rt.rb <= 8 ? 8 : (a.rt.rb <= 12 ? 12 : 16);
 }
 
 #define SUS2SH_FOR_EACH_K(X) X(1) X(2) X(3) X(4) X(5) X(6)
@@ -398,6 +399,8 @@ bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
   if (a.K == KK) {                                                                \
     if (nb == 8)                                                                  \
       k_basics<Real, LL, KK, 8><<<blocks, 256, lds, s>>>(a, basics);              \
+    else if (nb == 12)                                                            \
+      k_basics<Real, LL, KK, 12><<<blocks, 256, lds, s>>>(a, basics);             \
     else if (nb == 16)                                                            \
       k_basics<Real, LL, KK, 16><<<blocks, 256, lds, s>>>(a, basics);             \
     else                                                                          \
@@ -438,10 +441,12 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
       SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false)                                           \
     } else if (use_win) {                                                                    \
       if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true)                              \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, true)                       \
       else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true)                       \
       else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true)                                      \
     } else {                                                                                 \
       if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false)                             \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, false)                      \
       else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false)                      \
       else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false)                                     \
     }                                                                                        \

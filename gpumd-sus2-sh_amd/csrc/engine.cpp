@@ -125,8 +125,8 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   // LDS radial tables of the edge kernels: per pair, coefficients and (a, s), each padded by
   // at most 31 words (1 mod 32, or 4 mod 32 for the 16-byte rows; edge_kernels.hip)
   const auto padded = [](size_t n) { return n + 31; };
-  // (rows of the uniform kernels are zero-padded to 8 or 16 coefficients)
-  const size_t row = model_.rb_size <= 8 ? 8 : 16;
+  // (rows of the uniform kernels are zero-padded to 8, 12 or 16 coefficients)
+  const size_t row = model_.rb_size <= 8 ? 8 : (model_.rb_size <= 12 ? 12 : 16);
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
                        (padded(static_cast<size_t>(model_.radial_funcs_count) * row) +
                         padded(static_cast<size_t>(model_.radial_funcs_count) * 2)) * rsz;

@@ -148,7 +148,7 @@ __device__ __forceinline__ void run_batch(cu32* R, Real* X, Real& e, Real (&acc)
   constexpr bool BIN = KIND != 1;
   constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
   constexpr int kReads = BIN ? 2 * NS : NS;  // LDS reads of one group
-  constexpr int DEPTH = (CNT > 2 && 2 * kReads <= 15) ? 2 : (CNT > 1 ? 1 : 0);
+  constexpr int DEPTH = (CNT > 3 && 3 * kReads <= 15) ? 3 : (CNT > 2 && 2 * kReads <= 15) ? 2 : (CNT > 1 ? 1 : 0);
   Real xv[DEPTH + 1][NS], yv[DEPTH + 1][NS];
   LaneRec<Real, NS, BIN> r[CNT];
 #pragma unroll
@@ -166,7 +166,8 @@ __device__ __forceinline__ void run_batch(cu32* R, Real* X, Real& e, Real (&acc)
     }
     // groups still in flight after group k: min(DEPTH, CNT - 1 - k)
     const int inflight = CNT - 1 - k < DEPTH ? CNT - 1 - k : DEPTH;
-    if (inflight == 2) wait_lds_but<2 * kReads>();
+    if (inflight == 3) wait_lds_but<3 * kReads>();
+    else if (inflight == 2) wait_lds_but<2 * kReads>();
     else if (inflight == 1) wait_lds_but<kReads>();
     else wait_lds_but<0>();
     finish_rec<Real, NS, KIND>(r[k], xv[k % (DEPTH + 1)], yv[k % (DEPTH + 1)], acc, X, e);

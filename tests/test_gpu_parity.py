@@ -57,6 +57,10 @@ CASES = {
     "l2k3_body6": dict(model=dict(l_max=2, k_max=3, body_order=6, rb_size=5, max_dist=4.2,
                                   seed=15, scaling=0.6),
                        struct=("fcc", 4, 3.6, 0.05)),
+    # rb 14: the 16-coefficient uniform radial rows (rb <= 8 pads to 8, rb <= 12 to 12)
+    "l2k2_rb14": dict(model=dict(l_max=2, k_max=2, body_order=5, rb_size=14, max_dist=5.0,
+                                 seed=18, scaling=0.5),
+                      struct=("fcc", 4, 3.85, 0.08)),
     "l5k2_5511": dict(model=dict(l_max=5, k_max=2, body_order=5, body_l_max=(5, 5, 1, 1),
                                  rb_size=10, max_dist=5.0, seed=16, scaling=0.5),
                       struct=("fcc", 4, 3.85, 0.08)),

@@ -125,8 +125,10 @@ constexpr int lane_batch()
   return budget / rw < 1 ? 1 : (budget / rw > cap ? cap : budget / rw);
 }
 
-// Wait until at most `ahead` LDS operations are outstanding (the reads of the next group),
-// so that a group's NS (or 2 NS) operands cost one wait instead of one per consumer.  The
+// Wait until at most `ahead` LDS operations are outstanding (the reads of the groups after
+// the current one), so that a group's NS (or 2 NS) operands cost one wait instead of one per
+// consumer.  LDS operations of a wave complete in order, so a group's reads, followed by
+// `ahead` later ones, are then complete.  The
 // compiler still adds any wait it needs; this one only makes its per-use waits redundant.
 // s_waitcnt immediate: vmcnt and expcnt at their maxima (no wait), lgkmcnt = ahead.
 template <int ahead>
@@ -148,7 +150,10 @@ __device__ __forceinline__ void run_batch(cu32* R, Real* X, Real& e, Real (&acc)
   constexpr bool BIN = KIND != 1;
   constexpr int RW = lane_record_words(NS, static_cast<int>(sizeof(Real)));
   constexpr int kReads = BIN ? 2 * NS : NS;  // LDS reads of one group
-  constexpr int DEPTH = (CNT > 3 && 3 * kReads <= 15) ? 3 : (CNT > 2 && 2 * kReads <= 15) ? 2 : (CNT > 1 ? 1 : 0);
+  // up to three groups ahead, as far as the 4-bit LDS counter can express the wait
+  constexpr int DEPTH = (CNT > 3 && 3 * kReads <= 15)   ? 3
+                        : (CNT > 2 && 2 * kReads <= 15) ? 2
+                        : (CNT > 1 ? 1 : 0);
   Real xv[DEPTH + 1][NS], yv[DEPTH + 1][NS];
   LaneRec<Real, NS, BIN> r[CNT];
 #pragma unroll

@@ -25,7 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "gpumd-sus2-sh_amd"))
 sys.path.insert(0, ROOT)
 
 REFERENCE_A100 = 5.23633e6  # l3k3 1.024M compact serial product, README.md:152 of the reference
-PEAK_FP32_TFLOPS = 157.3
+# latest l3333 (rb 10, rc 7.5) 1.024M figure, docs/gpumd-sus2-sh-interface-plan.md:1409
+REFERENCE_A100_L3333 = 1.19500e7
+PEAK_FP32_TFLOPS = 157.3  # published FP32 vector peak (counts packed FP32 instructions)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # 256 CU x 4 SIMD x 16 lanes x 2 x 2.4 GHz, one FMA per lane
 PEAK_HBM_GBS = 8000.0
 
 
@@ -100,7 +103,8 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
               for (v, w), ms in pot.tuning().items()}
     out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
                info=info, nbar=nbar, energy=float(e.sum().item()), tuning=tuning,
-               force=f.cpu().numpy(), stream=stream.cuda_stream)
+               force=f.cpu().numpy(), potential=e.cpu().numpy(), virial=v.cpu().numpy(),
+               stream=stream.cuda_stream)
     if stage_sum is not None:
         out["stage_ms"] = dict(zip(["neighbor", "basic", "graph", "force", "scatter"],
                                    [s / steps for s in stage_sum]))
@@ -154,27 +158,88 @@ def run_graph_loop(path, cell, types, pos, precision, steps, warmup):
     return out
 
 
-def cpu_baseline(path, seconds=12.0):
-    """FP64 C/OpenMP oracle (a port of the same algorithm) on a bounded sample of the same
-    workload: a 20x20x20-cell (32,000 atom) piece with the same lattice, composition and
-    jitter, evaluated repeatedly for about `seconds` of wall time."""
-    from oracle.binding import Oracle, lib
-    import sus2sh.structures as st
-    pos, types, cell = st.fcc_alloy(20, 20, 20, 3.85, [0.5, 0.5], 0.08, 3001, 3002)
-    o = Oracle(path)
-    threads = lib().oracle_num_threads()
-    o.compute(cell, types, pos)
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(path, cell, types, pos, gpu_runs, seconds=12.0, max_evals=3):
+    """FP64 C/OpenMP oracle (a port of the same algorithm), compiled -O3 -march=native on this
+    host, on the benchmark structure itself (all 1,024,000 atoms): the first evaluation is
+    also the FP64 reference for the GPU results (per-atom energy, force and virial errors of
+    the FP32 and FP64 engine paths); evaluations repeat until `seconds` of wall time or
+    max_evals.  Runs after the timed GPU loops."""
+    import numpy as np
+    from oracle import binding
+    tmp = tempfile.mkdtemp(prefix="sus2sh_oracle_")
+    native = os.path.join(tmp, "liboracle_sus2sh_native.so")
+    try:
+        binding.build_native(native)
+        binding.use_library(native)
+        build = "gcc -O3 -march=native -fopenmp (built on this host)"
+    except Exception as exc:  # fall back to the portable -O3 -march=x86-64-v2 build
+        log(f"[bench] native oracle build failed ({exc}); using the portable build")
+        build = "gcc -O3 -march=x86-64-v2 -fopenmp (portable build)"
+    o = binding.Oracle(path)
+    threads = binding.lib().oracle_num_threads()
     n_eval, t0 = 0, time.perf_counter()
+    ref = None
     while True:
-        o.compute(cell, types, pos)
+        out = o.compute(cell, types, pos)
+        if ref is None:
+            ref = out
         n_eval += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n_eval >= 50:
+        if el >= seconds or n_eval >= max_evals:
             break
-    return {"value": len(types) * n_eval / el, "unit": "atom-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n_eval} FP64 evaluations of a 32,000-atom 20x20x20 fcc Cu-Zr piece "
-                      f"(same model, lattice, composition, jitter) in {el:.1f} s"}
+    errors = {}
+    for key, r in gpu_runs.items():
+        errors[key] = {
+            "max_abs_energy_err_ev_per_atom": float(np.abs(r["potential"] - ref[0]).max()),
+            "max_abs_force_err_ev_per_A": float(np.abs(r["force"] - ref[1]).max()),
+            "rms_force_err_ev_per_A": float(np.sqrt(((r["force"] - ref[1]) ** 2).mean())),
+            "max_abs_virial_err_ev": float(np.abs(r["virial"] - ref[2]).max()),
+        }
+    n = len(types)
+    return {"value": n * n_eval / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(), "build": build,
+            "sample": f"{n_eval} FP64 evaluations of the full {n}-atom benchmark structure "
+                      f"(same model) in {el:.1f} s"}, errors, {
+            "max_abs_force_ev_per_A": float(np.abs(ref[1]).max()),
+            "rms_force_ev_per_A": float(np.sqrt((ref[1] ** 2).mean()))}
+
+
+def c3ref(nx, ny, nz, steps):
+    """Reference-shaped model on the same structure: body-5 l3333, rb 10, rc 7.5 A (the model
+    behind the reference's A100 l3333 figure, docs/gpumd-sus2-sh-interface-plan.md:24-40),
+    scaling 0.12 so that the synthetic forces are ~1 eV/A rms.  FP32 and FP64 fixed-cell
+    force loops; FP32 force error against the FP64 path."""
+    import numpy as np
+    import sus2sh
+    from sus2sh import structures as st
+    spec = sus2sh.synthetic_model(3, 3, 5, body_l_max=(3, 3, 3, 3), species_count=2, rb_size=10,
+                                  max_dist=7.5, seed=3003, scaling=0.12)
+    path = sus2sh.write_mtp(spec, os.path.join(tempfile.mkdtemp(prefix="sus2sh_c3ref_"),
+                                               "cuzr_l3333_rb10_rc75.mtp"))
+    pos, types, cell = st.fcc_alloy(nx, ny, nz, 3.85, [0.5, 0.5], 0.08, 3001, 3002)
+    r32 = run_engine(path, cell, types, pos, "fp32", steps, 2, profile=False)
+    r64 = run_engine(path, cell, types, pos, "fp64", max(2, steps // 4), 1, profile=False)
+    df = r32["force"] - r64["force"]
+    return {"model": "l3333 body5 rb10 rc7.5 seed3003 scaling0.12", "atoms": len(types),
+            "mean_neighbors": r32["nbar"], "fp32_atom_steps_per_s": r32["atom_steps_per_s"],
+            "fp32_ms_per_step": r32["ms_per_step"], "fp64_atom_steps_per_s": r64["atom_steps_per_s"],
+            "fp32_over_fp64": r32["atom_steps_per_s"] / r64["atom_steps_per_s"],
+            "max_force_err_fp32_vs_fp64": float(np.abs(df).max()),
+            "rms_force_fp64": float(np.sqrt((r64["force"] ** 2).mean())),
+            "max_abs_force_fp64": float(np.abs(r64["force"]).max()),
+            "reference_a100_l3333_atom_steps_per_s": REFERENCE_A100_L3333,
+            "fp32_vs_reference_a100_l3333": r32["atom_steps_per_s"] / REFERENCE_A100_L3333}
 
 
 def main():
@@ -189,6 +254,8 @@ def main():
                     help="steps of the HIP-graph-replayed FP32 loop reported beside the eager "
                          "loop (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--c3ref-steps", type=int, default=10,
+                    help="FP32 steps of the reference-shaped model run (rc 7.5, rb 10; 0: skip)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before the runs, copy MIB MiB device-to-device (untimed): the plain "
                          "copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -250,11 +317,23 @@ def main():
             prof = json.load(fh)
         if kernel_name + "<" in prof.get("kernel", "") and prof.get("atoms") == N:
             traffic = prof["hbm_bytes"]
-    cpu = None
+    ref_shape = None
+    if args.c3ref_steps > 0:
+        log(f"[bench] reference-shaped model (rc 7.5, rb 10): {args.c3ref_steps} FP32 steps")
+        ref_shape = c3ref(nx, ny, nz, args.c3ref_steps)
+        log(f"[bench] c3ref {ref_shape}")
+    cpu, oracle_err, oracle_forces = None, None, None
     if not args.no_cpu:
-        log("[bench] CPU baseline (FP64 oracle, bounded sample)")
-        cpu = cpu_baseline(path, args.cpu_seconds)
-        log(f"[bench] cpu {cpu}")
+        log("[bench] CPU baseline: FP64 oracle on the full structure (also the error reference)")
+        cpu, oracle_err, oracle_forces = cpu_baseline(
+            path, cell, types, pos, {"fp32": r32, "fp64": r64}, args.cpu_seconds)
+        log(f"[bench] cpu {cpu}; errors vs oracle {oracle_err}")
+    per_kernel = {}
+    for k in ("basic", "graph", "force"):
+        a = flops[k] * N / (stages[k] * 1e-3) / 1e12
+        per_kernel[k] = {"launch_ms": stages[k], "flops_per_atom": flops[k], "tflops": a,
+                         "frac_of_157_3": a / PEAK_FP32_TFLOPS,
+                         "frac_of_unpacked_78_6": a / PEAK_FP32_UNPACKED_TFLOPS}
 
     result = {
         "metric": "atom-steps/s at Cu-Zr l3k3 1.024M (FP32 vs FP64 path); max force err vs FP64",
@@ -286,13 +365,22 @@ def main():
         "graph_tuning_ms_fp32": r32["tuning"],
         "graph_tuning_ms_fp64": r64["tuning"],
         "step_tflops_fp32": total_flops / (r32["ms_per_step"] * 1e-3) / 1e12,
-        # the dominant kernel is FP32 vector-ALU / LDS work (no matrix-core shape): it is
-        # priced against the published FP32 compute peak; traffic is HBM bytes per launch
-        "roofline": {"bound": "mfma", "kernel": kernel_name, "stage": dominant,
+        # the dominant kernel is FP32 vector-ALU / LDS work: it issues no matrix-core (MFMA)
+        # instruction, so it is bound by VALU issue and priced against the published FP32
+        # vector peak (157.3 TFLOP/s, which counts packed FP32); this build disables packed
+        # FP32, so the unpacked one-FMA-per-lane peak (78.6) is reported beside it.  traffic =
+        # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/)
+        "roofline": {"bound": "valu", "kernel": kernel_name, "stage": dominant,
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "peak_unpacked_fp32": PEAK_FP32_UNPACKED_TFLOPS,
+                     "frac_of_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "flops_per_atom": flops[dominant],
                      "launch_ms": stages[dominant]},
+        "roofline_per_kernel_fp32": per_kernel,
+        "errors_vs_fp64_oracle": oracle_err,
+        "oracle_force_scale": oracle_forces,
+        "reference_shape_c3ref": ref_shape,
         "graph_replay_fp32": graph32,
         "cpu_baseline": cpu,
         "vs_baseline_note": "reference A100 figure 5.23633e6 is a whole GPUMD NPT step with an "

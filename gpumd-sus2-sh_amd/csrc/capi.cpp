@@ -11,7 +11,7 @@
 
 struct sus2sh_handle {
   std::unique_ptr<sus2sh::Engine> engine;
-  std::string error;
+  mutable std::string error;  // also written by the const (read-only) entry points
 };
 
 struct sus2sh_graph {
@@ -167,9 +167,7 @@ int sus2sh_set_params(sus2sh_handle* handle, const double* theta, int n)
 int sus2sh_get_params(const sus2sh_handle* handle, double* theta, int n)
 {
   if (!handle) return SUS2SH_ERR_ARGUMENT;
-  std::string err;
-  const int rc = guarded(&err, [&] { handle->engine->get_params(theta, n); });
-  return rc;
+  return guarded(&handle->error, [&] { handle->engine->get_params(theta, n); });
 }
 
 const char* sus2sh_last_error(const sus2sh_handle* handle)
@@ -215,6 +213,8 @@ int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out)
   out->lane_program = lp.ok ? 1 : 0;
   out->lane_slots = lp.ok ? lp.slots : 0;
   out->lane_instructions = lp.ok ? static_cast<int>(lp.ins.size()) : 0;
+  out->lane_waves = lp.ok ? lp.waves : 0;
+  out->lane_nslot = lp.ok ? lp.nslot : 0;
   return SUS2SH_OK;
 }
 
@@ -316,16 +316,18 @@ int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_
 
 void sus2sh_graph_free(sus2sh_graph* graph) { delete graph; }
 
-int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
-                      const double* basics, double* e_prime, double* grad, int* info)
+int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot, int real_bytes,
+                      int n, const double* basics, double* e_prime, double* grad, int* info)
 {
+  if (nslot == 0) nslot = waves == 8 ? 2 : 4;
   if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
-      (waves != 1 && waves != 2 && waves != 4 && waves != 8))
+      (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
+      (nslot != 1 && nslot != 2 && nslot != 4 && nslot != 8) ||
+      (real_bytes != 0 && real_bytes != 4 && real_bytes != 8))
     return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);
-    const sus2sh::LaneProgram p =
-      sus2sh::compile_lane_program(m, max_slots, waves == 8 ? 2 : 4, waves);
+    const sus2sh::LaneProgram p = sus2sh::compile_lane_program(m, max_slots, nslot, waves);
     if (info) {
       info[0] = p.ok ? 1 : 0;
       info[1] = p.slots;
@@ -336,8 +338,15 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
     }
     if (!p.ok) return;
     const int B = m.basic_count;
+    if (real_bytes == 0) {
+      for (int i = 0; i < n; ++i)
+        e_prime[i] = sus2sh::run_lane_program_host(p, basics + static_cast<size_t>(i) * B,
+                                                   grad + static_cast<size_t>(i) * B);
+      return;
+    }
+    const sus2sh::LaneRecords rec = sus2sh::encode_lane_records(p, real_bytes);
     for (int i = 0; i < n; ++i)
-      e_prime[i] = sus2sh::run_lane_program_host(p, basics + static_cast<size_t>(i) * B,
+      e_prime[i] = sus2sh::run_lane_records_host(p, rec, basics + static_cast<size_t>(i) * B,
                                                  grad + static_cast<size_t>(i) * B);
   });
 }

@@ -29,19 +29,49 @@ bool parse_bool(const std::string& v, const std::string& name)
 
 // Options after the species symbols (src/force/sus2_sh.cu:520-535): sus2_float /
 // sus2_sh_float select the precision (env SUS2_GPUMD_FLOAT); the reference's other
-// sus2_* switches select CUDA micro-variants and are accepted and ignored here.
+// sus2_* switches select CUDA micro-variants and are accepted and ignored here, except the
+// two whose values the reference validates: the factor-pruning mode (legacy | q-total,
+// src/force/sus2_sh.cu:558-587; this engine always evaluates the file's explicit graph, which
+// is what q-total selects) and direct radial evaluation, which must stay on
+// (src/force/sus2_sh.cu:538-556, 7914-7916).  Like the reference, only `key=value` tokens are
+// options: a bare token (no '=') is ignored.
 EngineOptions parse_options(const Model& m, int nopts, const char** opts)
 {
   EngineOptions o;
   if (const char* env = std::getenv("SUS2_GPUMD_FLOAT")) o.use_float = parse_bool(env, "SUS2_GPUMD_FLOAT");
   if (const char* env = std::getenv("SUS2_GPUMD_PROFILE")) o.profile = parse_bool(env, "SUS2_GPUMD_PROFILE");
+  bool radial_direct = true;
+  if (const char* env = std::getenv("SUS2_GPUMD_RADIAL_DIRECT"))
+    radial_direct = parse_bool(env, "SUS2_GPUMD_RADIAL_DIRECT");
+  std::string pruning = "legacy";
+  if (const char* env = std::getenv("SUS2_SH_GPUMD_FACTOR_PRUNING"))
+    pruning = env;
+  else if (const char* env2 = std::getenv("SUS2_GPUMD_FACTOR_PRUNING"))
+    pruning = env2;
   const int begin = std::min(nopts, m.species_count);
   for (int i = begin; i < nopts; ++i) {
     const std::string opt = opts[i] ? opts[i] : "";
     const size_t eq = opt.find('=');
-    const std::string key = eq == std::string::npos ? opt : opt.substr(0, eq);
-    const std::string val = eq == std::string::npos ? "" : opt.substr(eq + 1);
-    if (key == "sus2_float" || key == "sus2_sh_float") {
+    if (eq == std::string::npos) continue;  // not an option token
+    const std::string key = opt.substr(0, eq);
+    const std::string val = opt.substr(eq + 1);
+    if (key == "sus2_radial_direct" || key == "sus2_sh_radial_direct" || key == "radial_direct") {
+      radial_direct = parse_bool(val, key);
+    } else if (key == "sus2_sh_factor_pruning" || key == "sus2_factor_pruning" ||
+               key == "sh_factor_pruning") {
+      pruning = val;
+    } else if (key == "sus2_sh_lane_shape") {
+      // waves x accumulators x minimum slot budget, e.g. 8x2x208
+      int w = 0, a = 0, s = 0;
+      char x1 = 0, x2 = 0;
+      std::istringstream in(val);
+      in >> w >> x1 >> a >> x2 >> s;
+      if (!in || x1 != 'x' || x2 != 'x' || !in.eof())
+        throw InputError("sus2_sh_lane_shape must be <waves>x<accumulators>x<slots>: " + val);
+      o.lane_waves = w;
+      o.lane_nslot = a;
+      o.lane_min_slots = s;
+    } else if (key == "sus2_float" || key == "sus2_sh_float") {
       o.use_float = parse_bool(val, key);
     } else if (key == "sus2_sh_profile") {
       o.profile = parse_bool(val, key);
@@ -55,7 +85,11 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       const int w = std::atoi(val.c_str());
       if (w != 1 && w != 2 && w != 4 && w != 8)
         throw InputError("sus2_sh_lane_waves must be 1, 2, 4 or 8: " + val);
-      o.lane_waves = w;  // fixes the lane program shape (4 accumulators per wave)
+      // fixes the lane program shape: 2 accumulators per wave with 8 waves, else 4; the
+      // smallest slot budget that holds the program, from 160 up
+      o.lane_waves = w;
+      o.lane_nslot = w == 8 ? 2 : 4;
+      o.lane_min_slots = 160;
     } else if (key == "sus2_sh_deferred_check") {
       o.deferred_check = parse_bool(val, key);
     } else if (key == "sus2_sh_md_graph") {
@@ -66,6 +100,18 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
         throw InputError("sus2_sh_graph_tile must be 1, 2, 4, 8 or 16: " + val);
       o.graph_tile = t;
     }
+  }
+  if (!(pruning == "legacy" || pruning == "Legacy" || pruning == "LEGACY" || pruning == "q-total" ||
+        pruning == "q_total" || pruning == "total" || pruning == "QTotal" || pruning == "QTOTAL"))
+    throw InputError("Invalid SUS2_SH factor pruning mode: " + pruning +
+                     " (expected legacy or q-total).");
+  if (!radial_direct)
+    throw InputError("SUS2_SH first backend currently requires direct radial evaluation.");
+  if (o.lane_nslot > 0) {
+    if (!lane_shape_supported(o.lane_waves, o.lane_nslot))
+      throw InputError("sus2_sh_lane_shape: no lane kernel for this waves x accumulators pair");
+    if (o.lane_min_slots < 1 || o.lane_min_slots > kLaneMaxSlot + 1)
+      throw InputError("sus2_sh_lane_shape: slot budget out of range");
   }
   return o;
 }
@@ -225,8 +271,8 @@ const std::vector<Engine::LaneShape>& Engine::lane_shapes()
 
 void Engine::compile_lane()
 {
-  if (opt_.lane_waves > 0) {
-    lane_ = build_lane(LaneShape{opt_.lane_waves, opt_.lane_waves == 8 ? 2 : 4, 160});
+  if (opt_.lane_nslot > 0) {
+    lane_ = build_lane(LaneShape{opt_.lane_waves, opt_.lane_nslot, opt_.lane_min_slots});
     return;
   }
   lane_ = LaneProgram{};
@@ -680,7 +726,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     float best = 1e30f;
     int best_tile = tile_, best_variant = variant_;
     const std::vector<LaneShape>& shapes = lane_shapes();
-    const int n_lane = opt_.lane_waves > 0 ? 1 : static_cast<int>(shapes.size());
+    const int n_lane = opt_.lane_nslot > 0 ? 1 : static_cast<int>(shapes.size());
     tune_ms_.assign(15 + shapes.size(), -1.0f);
     LaneProgram best_lane = lane_;
     auto time_it = [&](int v, int t) {
@@ -699,7 +745,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
         const int t = 1 << li;
         if (v < 3 && graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
         if (v == 3) {
-          if (opt_.lane_waves <= 0) {
+          if (opt_.lane_nslot <= 0) {
             lane_ = build_lane(shapes[li]);
             if (!lane_.ok) continue;
             upload_lane();

@@ -63,6 +63,8 @@ struct EngineOptions {
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
   int lane_waves = 0;    // lane kernel: waves per 64-atom tile (0: autotuned shape)
+  int lane_nslot = 0;    // lane kernel: accumulators per wave (0: 2 with 8 waves, else 4)
+  int lane_min_slots = 160;  // lane kernel: smallest per-lane slot budget tried
   // no host synchronisation in steady calls (capturable); the neighbour-capacity and type
   // checks are latched on the device and reported by check()
   bool deferred_check = false;

@@ -212,6 +212,8 @@ struct LaneTable {
 
 // Lane graph kernel (lane_kernel.hip): one 64-atom tile per workgroup of lt.waves waves;
 // returns false when the tile's LDS would exceed max_lds or the program shape is unknown.
+// Whether a lane kernel is compiled for `waves` waves per tile x `nslot` accumulators.
+bool lane_shape_supported(int waves, int nslot);
 template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);

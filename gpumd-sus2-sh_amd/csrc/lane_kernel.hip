@@ -282,6 +282,12 @@ void launch_lane_shape(int N, const Real* basics, const int* type_s, const LaneT
 
 }  // namespace
 
+bool lane_shape_supported(int waves, int nslot)
+{
+  return (nslot == 2 && (waves == 4 || waves == 8)) || (nslot == 4 && (waves == 1 || waves == 2 || waves == 4)) ||
+         (nslot == 8 && waves == 1);
+}
+
 template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds)

@@ -465,21 +465,26 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     return p;
   }
 
-  // self-check against the flat graph on two seeded basic vectors
+  // self-check against the flat graph on two seeded basic vectors: the program and its FP64
+  // device encoding to 1e-9, the FP32 encoding (coefficients rounded to float) to 1e-5
   uint64_t seed = 0x5eed5eedULL;
-  std::vector<double> b(B), g1(B), g2(B), g3(B);
+  std::vector<double> b(B), g1(B), g2(B), g3(B), g4(B);
+  const LaneRecords rec8 = encode_lane_records(p, 8), rec4 = encode_lane_records(p, 4);
   for (int trial = 0; trial < 2; ++trial) {
     for (int i = 0; i < B; ++i) b[i] = (static_cast<double>(lcg(seed) % 2000001) / 1e6 - 1.0) * 0.5;
     p.ok = true;
     const double e1 = run_lane_program_host(p, b.data(), g1.data());
     const double e2 = run_flat_graph_host(m, b.data(), g2.data());
-    const double e3 = run_lane_records_host(p, encode_lane_records(p, 8), b.data(), g3.data());
+    const double e3 = run_lane_records_host(p, rec8, b.data(), g3.data());
+    const double e4 = run_lane_records_host(p, rec4, b.data(), g4.data());
     double scale = std::abs(e2), err = std::max(std::abs(e1 - e2), std::abs(e3 - e2));
+    double err4 = std::abs(e4 - e2);
     for (int i = 0; i < B; ++i) {
       scale = std::max(scale, std::abs(g2[i]));
       err = std::max(err, std::max(std::abs(g1[i] - g2[i]), std::abs(g3[i] - g2[i])));
+      err4 = std::max(err4, std::abs(g4[i] - g2[i]));
     }
-    if (!(err <= 1e-9 * std::max(1.0, scale))) {
+    if (!(err <= 1e-9 * std::max(1.0, scale)) || !(err4 <= 1e-5 * std::max(1.0, scale))) {
       p.ok = false;
       p.reason = "lane program self-check failed";
       return p;

@@ -84,20 +84,17 @@ LaneProgram compile_lane_program(const Model& model, int max_slots, int nslot = 
 double run_lane_program_host(const LaneProgram& p, const double* basics, double* grad_b);
 
 // Device encoding of a lane program: one record of rec_words() 32-bit words per group of
-// nslot instructions, wave-major inside a segment (record (w * groups + g) of the segment).
+// nslot instructions, wave-major inside a segment (record (w * groups + g) of the segment),
+// every word used so that a batch of records is a few wide scalar loads:
 //   words [0, nslot)          x word offsets in the lane column layout (slot * 64)
 //   words [nslot, 2 nslot)    y word offsets
-//   words [2 nslot, 3 nslot)  target word offsets: store segments write acc to them after
-//                             every group (partial sums land in the row's own target,
-//                             which no instruction of the segment reads), add segments
-//                             add at the row's end
-//   word  3 nslot             row-end mask: bit j = instruction j ends its row
-//   then nslot keep factors (Real: 0 at a row end, else 1; acc *= keep after each group)
-//   and nslot coefficients (Real), starting at word header_words()
-// Device record of one group of ns instructions, every word used so that a batch of
-// records is a few wide scalar loads: ns x word offsets, ns y word offsets, ns target word
-// offsets, the row-end mask, then ns coefficients (8-byte coefficients start on an even
-// word, with one padding word when 3 ns + 1 is odd).
+//   words [2 nslot, 3 nslot)  target word offsets: where an accumulator goes at its row's
+//                             end (stored by store segments, added by add segments; energy
+//                             segments sum their accumulators once, at the segment's end)
+//   word  3 nslot             row-end mask: bit j = instruction j ends its row; bit nslot =
+//                             a row split between accumulators 0 and 1 ends (1 joins 0 first)
+//   then nslot coefficients (Real) from word header_words(): 8-byte coefficients start on
+//   an even word, with one padding word when 3 nslot + 1 is odd.
 constexpr int lane_header_words(int ns, int real_bytes)
 {
   return real_bytes == 8 ? (3 * ns + 1 + 1) / 2 * 2 : 3 * ns + 1;

@@ -60,6 +60,8 @@ typedef struct sus2sh_model_info {
   int lane_program;       /* 1 when the lane program (graph depth <= 2) is available */
   int lane_slots;         /* per-lane on-chip slots of the lane program */
   int lane_instructions;  /* lane-program instructions per atom (including padding) */
+  int lane_waves;         /* lane-program waves per 64-atom tile */
+  int lane_nslot;         /* lane-program accumulators per wave */
 } sus2sh_model_info;
 
 /* Model identity: first token "MTP" and the substring "potential_tag = SUS2-SH".
@@ -72,13 +74,17 @@ int sus2sh_probe(const char* path);
  * num_potential_options, potential_options) (src/force/sus2_sh.cuh:12-16, called from
  * src/force/force.cu:114).  `opts` are the run.in tokens after the model file: the
  * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
- * Honoured options: sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
+ * Honoured options (like the reference, only key=value tokens are options; bare tokens are
+ * ignored): sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
  * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8>,
- * sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
+ * sus2_sh_lane_shape=<waves>x<accumulators>x<min slots> (e.g. 8x2x208: fixes the lane
+ * kernel's full shape), sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
  * sus2_sh_deferred_check=<0|1> (steady calls make no host synchronisation and can be
  * captured in a HIP graph; see sus2sh_check), sus2_sh_md_graph=<0|1> (sus2sh_md_nve replays
- * captured steps, default 1); other sus2_* tokens are accepted and ignored.  Uses the
- * current HIP device. */
+ * captured steps, default 1).  Validated like the reference: sus2_sh_factor_pruning=
+ * <legacy|q-total> (src/force/sus2_sh.cu:558-587; the file's graph is always evaluated as an
+ * explicit graph) and sus2_radial_direct=<1> (0 is rejected, src/force/sus2_sh.cu:7914-7916).
+ * Other sus2_* tokens are accepted and ignored.  Uses the current HIP device. */
 int sus2sh_create(const char* mtp_path, int num_atoms, int nopts, const char** opts,
                   sus2sh_handle** out);
 
@@ -167,9 +173,10 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
 
 /* Autotuner measurements of the graph kernel (ms per launch, -1 when not tried):
  * ms[v * 5 + log2(tile)] for variant v = 0..2 and tile 1..16, and ms[15 + c] for lane-program
- * (variant 3) shape c = 0..5: (waves per 64-atom tile, accumulators per wave, minimum
- * per-lane slots) = (4,4,160), (8,2,160), (4,2,160), (8,2,192), (8,2,256), (2,4,160).
- * Returns the count written. */
+ * (variant 3) shape c = 0..6: (waves per 64-atom tile, accumulators per wave, minimum
+ * per-lane slots) = (4,4,160), (8,2,160), (4,2,160), (8,2,192), (8,2,256), (2,4,160),
+ * (8,2,208).  When the lane shape is fixed (sus2_sh_lane_waves / sus2_sh_lane_shape) only
+ * that shape is timed and its time is in ms[15].  Returns the count written. */
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
 /* Mean number of neighbours per atom found by the last compute. */
@@ -214,15 +221,17 @@ int sus2sh_zbl_tables(const char* mtp_path, int nsym, const char** symbols, int*
 int sus2sh_model_check(const char* mtp_path, int* dims);
 
 /* Host-only check of the lane program (variant 3 of the graph kernel; no reference
- * counterpart): compile it for the model with at most max_slots per-lane slots and `waves`
- * (1, 2, 4 or 8) waves per 64-atom tile (4 accumulators per wave, 2 with 8 waves: the shapes
- * sus2_sh_lane_waves selects), and run it on the host for n basic vectors
+ * counterpart): compile it for the model with at most max_slots per-lane slots, `waves`
+ * (1, 2, 4 or 8) waves per 64-atom tile and `nslot` (1, 2, 4 or 8; 0: 2 with 8 waves, else 4)
+ * accumulators per wave, and run it on the host for n basic vectors
  * basics[n][B] -> e_prime[n] = sum_s w_s M[map[s]] and grad[n][B] = de_prime/dbasic (without
- * the species factor).  info[0..6) = ok, slots, instructions per atom, graph components,
+ * the species factor).  real_bytes = 0 runs the compiled program; 4 or 8 runs its device
+ * encoding for FP32 / FP64 kernels (the records the kernel reads, coefficients rounded to
+ * that precision).  info[0..6) = ok, slots, instructions per atom, graph components,
  * batches, largest component.  When the model has no lane program (info[0] = 0) nothing is
  * evaluated.  Returns a status code. */
-int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int n,
-                      const double* basics, double* e_prime, double* grad, int* info);
+int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot, int real_bytes,
+                      int n, const double* basics, double* e_prime, double* grad, int* info);
 
 #ifdef __cplusplus
 }

@@ -18,13 +18,31 @@ def build():
     subprocess.run(["make", "-C", _HERE, "-s"], check=True)
 
 
+def build_native(out_path):
+    """Compile the oracle with -O3 -march=native on this machine into out_path (for the CPU
+    baseline, which is timed on the host that runs it)."""
+    subprocess.run(["make", "-C", _HERE, "-s", "native", f"NATIVE_OUT={out_path}"], check=True)
+
+
+def use_library(path):
+    """Load the oracle from another build of the same source (e.g. build_native's output);
+    every later call in this process uses it."""
+    global _lib
+    _lib = _configure(ctypes.CDLL(path))
+    return _lib
+
+
 def lib():
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = _configure(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _configure(L):
     P = ctypes.POINTER
     c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
     L.oracle_load.argtypes = [ctypes.c_char_p]
@@ -60,7 +78,6 @@ def lib():
     L.oracle_set_zbl.restype = c_int
     L.oracle_zbl_enabled.argtypes = [c_void_p]
     L.oracle_zbl_enabled.restype = c_int
-    _lib = L
     return L
 
 

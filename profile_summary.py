@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Summarise rocprofv3 output of bench.py runs into the short text files kept in profiles/.
 
-  python gpumd-sus2-sh_amd/profile_summary.py stats  <kernel_stats.csv> [--top 12]
-  python gpumd-sus2-sh_amd/profile_summary.py traffic <fetch_counter_collection.csv>
+  python profile_summary.py stats  <kernel_stats.csv> [--top 12]
+  python profile_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> --kernel <substring> --copy-bytes <bytes>
          [--atoms N] [--command "..."]
 

@@ -70,6 +70,36 @@ def test_l3333_structure(make_model):
     assert info["slots"] <= 160
 
 
+# every lane shape the engine's autotuner tries (waves, accumulators, slot budget)
+LANE_SHAPES = [(4, 4, 160), (8, 2, 160), (4, 2, 160), (8, 2, 192), (8, 2, 256), (2, 4, 160),
+               (8, 2, 208)]
+
+
+@pytest.mark.parametrize("waves,nslot,slots", LANE_SHAPES)
+def test_l3333_every_tuned_shape_and_encoding(make_model, waves, nslot, slots):
+    """The l3333 program at each autotuned shape, run three ways on the host against the
+    oracle's flat graph: the compiled program and its FP64 device encoding (1e-11 of the
+    scale), and its FP32 device encoding -- the records the benchmarked FP32 kernel reads,
+    coefficients rounded to float -- to 2e-6 of the scale (float rounding of ~10^4
+    coefficients)."""
+    path = make_model("lane_l3333", l_max=3, k_max=3, body_order=5, body_l_max=(3, 3, 3, 3),
+                      rb_size=8, max_dist=5.0, seed=3003, scaling=0.5)
+    orc = Oracle(path)
+    B = orc.dims["B"]
+    rng = np.random.default_rng(waves * 100 + nslot * 10 + slots)
+    basics = rng.normal(0.0, 0.4, size=(4, B))
+    e0, g0 = orc.graph(basics)
+    scale = max(1.0, np.abs(e0).max(), np.abs(g0).max())
+    for real_bytes, tol in ((0, 1e-11), (8, 1e-11), (4, 2e-6)):
+        info, e, g = sus2sh.lane_check(path, basics, max_slots=slots, waves=waves, nslot=nslot,
+                                       real_bytes=real_bytes)
+        print(f"[lane] l3333 shape=({waves},{nslot},{slots}) bytes={real_bytes}: {info}",
+              flush=True)
+        assert info["ok"] == 1 and info["slots"] <= slots
+        assert np.abs(e - e0).max() <= tol * scale
+        assert np.abs(g - g0).max() <= tol * scale
+
+
 def test_body6_has_no_lane_program(make_model):
     path = make_model("lane_body6", l_max=2, k_max=2, body_order=6, rb_size=4, max_dist=4.0,
                       seed=5)

@@ -110,7 +110,7 @@ def test_engine_site_energy_closed_form(body2_model, fp32):
     for r in (2.3, 3.7, 4.9):
         pos, types, cell = _dimer_structure(r)
         print(f"[case] body-2 dimer r={r} fp32={fp32}", flush=True)
-        e, f, v = sus2sh.evaluate(path, cell, types, pos, options=["A", "B", f"sus2_float={fp32}"],
+        e, f, v = sus2sh.evaluate(path, cell, types, pos, options=["A", "B", f"sus2_float={int(fp32)}"],
                                   pbc=(0, 0, 0))
         want = _expected_all(spec, r)
         print(f"  engine {e}, closed form {want}", flush=True)

@@ -105,6 +105,21 @@ int sus2sh_compute(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
   });
 }
 
+int sus2sh_compute_noalloc(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                           const int* d_type, const double* d_position, double* d_potential,
+                           double* d_force, double* d_virial, void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  if (!box) {
+    handle->error = "sus2sh_compute_noalloc: box is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  return guarded(&handle->error, [&] {
+    handle->engine->compute_noalloc(*box, num_atoms, d_type, d_position, d_potential, d_force,
+                                    d_virial, static_cast<hipStream_t>(stream));
+  });
+}
+
 int sus2sh_compute_param_grad(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                               const int* d_type, const double* d_position,
                               double* d_potential, double* d_force, double* d_virial,

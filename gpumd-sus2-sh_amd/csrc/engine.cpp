@@ -1,5 +1,7 @@
 #include "engine.hpp"
 
+#include "scratch.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -160,6 +162,9 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   }
 
   hip_check(hipGetDevice(&device_), "hipGetDevice");
+  // the allocation-free evaluation's scratch array: its address on this device, looked up
+  // once per device (compute_noalloc then makes no runtime call to find it)
+  hip_check(scratch_init_current_device(), "hipGetSymbolAddress(scratch)");
   int lds = 0;
   hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_),
             "hipDeviceGetAttribute");
@@ -420,6 +425,87 @@ void Engine::ensure_atoms(int N, size_t edge_bytes)
   nl_.resize(static_cast<size_t>(cap_) * n * edge_bytes);
 }
 
+void Engine::bind_buffers(int N, int ncell, int n_totals)
+{
+  (void)N;
+  (void)ncell;
+  (void)n_totals;
+  ws_.key = key_.as<int>();
+  ws_.slot = slot_.as<int>();
+  ws_.arrival = arrival_.as<int>();
+  ws_.skey = skey_.as<int>();
+  ws_.satom = satom_.as<int>();
+  ws_.rank = rank_.as<int>();
+  ws_.type_s = type_s_.as<int>();
+  ws_.count = count_.as<int>();
+  ws_.wrapped = wrapped_.as<double>();
+  ws_.pos_s = pos_s_.as<double>();
+  ws_.e_s = e_s_.as<double>();
+  ws_.f_s = f_s_.as<double>();
+  ws_.v_s = v_s_.as<double>();
+  ws_.packed = packed_.as<PackedAtom>();
+  ws_.nl = nl_.as<void>();
+  ws_.basics = basics_.as<void>();
+  ws_.grads = grads_.as<void>();
+  ws_.cell_count = cell_count_.as<int>();
+  ws_.cell_begin = cell_begin_.as<int>();
+  ws_.scan_totals = scan_totals_.as<int>();
+  ws_.max_count = max_count_.as<int>();
+  ws_.win = win_.as<int>();
+  ws_.cap = cap_;
+}
+
+// Work buffers of one evaluation as 256-byte aligned pieces of the device's scratch array;
+// what is left after the per-atom and per-cell buffers holds the neighbour list, whose
+// capacity (neighbours per atom) follows from it.  Nothing here allocates or calls the HIP
+// runtime.
+void Engine::carve_scratch(int N, int ncell, int n_totals, size_t edge_bytes, int win_words)
+{
+  unsigned char* base = static_cast<unsigned char*>(scratch_device_ptr(device_));
+  if (!base) throw DeviceError("compute_noalloc: no scratch array on this device");
+  const size_t n = static_cast<size_t>(N);
+  const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = base + off;
+    off += (bytes + 255) & ~size_t(255);
+    return p;
+  };
+  Workspace w;
+  w.key = static_cast<int*>(take(n * sizeof(int)));
+  w.slot = static_cast<int*>(take(n * sizeof(int)));
+  w.arrival = static_cast<int*>(take(n * sizeof(int)));
+  w.skey = static_cast<int*>(take(n * sizeof(int)));
+  w.satom = static_cast<int*>(take(n * sizeof(int)));
+  w.rank = static_cast<int*>(take(n * sizeof(int)));
+  w.type_s = static_cast<int*>(take(n * sizeof(int)));
+  w.count = static_cast<int*>(take(n * sizeof(int)));
+  w.wrapped = static_cast<double*>(take(3 * n * sizeof(double)));
+  w.pos_s = static_cast<double*>(take(3 * n * sizeof(double)));
+  w.e_s = static_cast<double*>(take(n * sizeof(double)));
+  w.f_s = static_cast<double*>(take(3 * n * sizeof(double)));
+  w.v_s = static_cast<double*>(take(9 * n * sizeof(double)));
+  w.packed = static_cast<PackedAtom*>(take(n * sizeof(PackedAtom)));
+  w.basics = take(static_cast<size_t>(model_.basic_count) * n * rsz);
+  w.grads = take(static_cast<size_t>(model_.basic_count) * n * rsz);
+  w.cell_count = static_cast<int*>(take((static_cast<size_t>(ncell) + 1) * sizeof(int)));
+  w.cell_begin = static_cast<int*>(take((static_cast<size_t>(ncell) + 1) * sizeof(int)));
+  w.scan_totals = static_cast<int*>(take(static_cast<size_t>(n_totals) * sizeof(int)));
+  w.max_count = static_cast<int*>(take(16 * sizeof(int)));
+  w.win = static_cast<int*>(take(static_cast<size_t>(win_words) * sizeof(int)));
+  const size_t left = off < kScratchBytes ? kScratchBytes - off : 0;
+  const size_t cap = std::min<size_t>(left / (n * edge_bytes), 4096);
+  if (off > kScratchBytes || cap < 1) {
+    std::ostringstream msg;
+    msg << "SUS2_SH: compute_noalloc: " << N << " atoms do not fit the "
+        << (kScratchBytes >> 20) << " MiB scratch array of this device";
+    throw ArgumentError(msg.str());
+  }
+  w.nl = base + off;
+  w.cap = static_cast<int>(cap);
+  ws_ = w;
+}
+
 CellGrid Engine::make_grid(const sus2sh_box& box, int N, int* ncell) const
 {
   CellGrid g{};
@@ -470,9 +556,27 @@ void Engine::compute(const sus2sh_box& box, int N, const int* type, const double
     throw ArgumentError("null device pointer passed to compute");
   const int offsets[2] = {0, N};
   if (opt_.use_float)
-    run<float>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad);
+    run<float>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad, false);
   else
-    run<double>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad);
+    run<double>(1, &box, offsets, N, type, pos, pot, force, virial, stream, param_grad, false);
+}
+
+void Engine::compute_noalloc(const sus2sh_box& box, int N, const int* type, const double* pos,
+                             double* pot, double* force, double* virial, hipStream_t stream)
+{
+  if (N < 0) throw ArgumentError("num_atoms must be >= 0");
+  if (N == 0) return;
+  if (!type || !pos || !pot || !force || !virial)
+    throw ArgumentError("null device pointer passed to compute_noalloc");
+  int dev = -1;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != device_)
+    throw ArgumentError("compute_noalloc: the current device is not the handle's device");
+  const int offsets[2] = {0, N};
+  if (opt_.use_float)
+    run<float>(1, &box, offsets, N, type, pos, pot, force, virial, stream, nullptr, true);
+  else
+    run<double>(1, &box, offsets, N, type, pos, pot, force, virial, stream, nullptr, true);
 }
 
 void Engine::compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom_offsets,
@@ -489,9 +593,11 @@ void Engine::compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom
   if (!type || !pos || !pot || !force || !virial)
     throw ArgumentError("null device pointer passed to compute_batch");
   if (opt_.use_float)
-    run<float>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
+    run<float>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad,
+               false);
   else
-    run<double>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad);
+    run<double>(nstruct, boxes, atom_offsets, N, type, pos, pot, force, virial, stream, param_grad,
+                false);
 }
 
 template <typename Real>
@@ -512,8 +618,8 @@ bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable
     lt.slots = lane_.slots;
     lt.nslot = lane_.nslot;
     lt.waves = lane_.waves;
-    return launch_lane_graph<Real>(N, basics_.as<Real>(), type_s_.as<int>(), lt, e_s_.as<double>(),
-                                   grads_.as<Real>(), s, max_lds_);
+    return launch_lane_graph<Real>(N, static_cast<const Real*>(ws_.basics), ws_.type_s, lt,
+                                   ws_.e_s, static_cast<Real*>(ws_.grads), s, max_lds_);
   }
   GraphTable<Real> t = gt;
   const int k = variant == 2 ? 1 : 0;
@@ -523,14 +629,15 @@ bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable
   t.bwd_ins = bwd_ins_[k].as<Term<Real>>();
   t.bwd_sbegin = bwd_sbegin_[k].as<int>();
   t.bwd_slen = bwd_slen_[k].as<int>();
-  return launch_graph<Real>(tile, variant == 0 ? 0 : 1, N, basics_.as<Real>(), type_s_.as<int>(),
-                            t, e_s_.as<double>(), grads_.as<Real>(), nullptr, s, max_lds_, 0);
+  return launch_graph<Real>(tile, variant == 0 ? 0 : 1, N, static_cast<const Real*>(ws_.basics),
+                            ws_.type_s, t, ws_.e_s, static_cast<Real*>(ws_.grads), nullptr, s,
+                            max_lds_, 0);
 }
 
 template <typename Real>
 void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, int N,
                  const int* type, const double* pos, double* pot, double* force, double* virial,
-                 hipStream_t s, double* param_grad)
+                 hipStream_t s, double* param_grad, bool noalloc)
 {
   if (N >= kMaxEdgeAtoms)
     throw ArgumentError("SUS2_SH: at most 134217727 atoms per call (neighbour index width)");
@@ -541,16 +648,26 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   hip_check(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
   const bool capturing = capture != hipStreamCaptureStatusNone;
   if (capturing && batch) throw ArgumentError("SUS2_SH: batched evaluations cannot be captured");
-  std::vector<CellGrid> grids(nstruct);
-  std::vector<int> cell_off(nstruct + 1, 0);
-  for (int st = 0; st < nstruct; ++st) {
-    int nc = 0;
-    grids[st] = make_grid(boxes[st], atom_offsets[st + 1] - atom_offsets[st], &nc);
-    if (static_cast<long>(cell_off[st]) + nc > (1L << 30)) throw ArgumentError("too many cells");
-    cell_off[st + 1] = cell_off[st] + nc;
+  // cell grids (host; the per-structure tables only for a batch, so that a single structure
+  // makes no host allocation either)
+  CellGrid grid{};
+  int ncell = 0;
+  std::vector<CellGrid> grids;
+  std::vector<int> cell_off;
+  if (!batch) {
+    grid = make_grid(boxes[0], N, &ncell);
+  } else {
+    grids.resize(nstruct);
+    cell_off.assign(nstruct + 1, 0);
+    for (int st = 0; st < nstruct; ++st) {
+      int nc = 0;
+      grids[st] = make_grid(boxes[st], atom_offsets[st + 1] - atom_offsets[st], &nc);
+      if (static_cast<long>(cell_off[st]) + nc > (1L << 30)) throw ArgumentError("too many cells");
+      cell_off[st + 1] = cell_off[st] + nc;
+    }
+    ncell = cell_off[nstruct];
+    grid = grids[0];
   }
-  const int ncell = cell_off[nstruct];
-  const CellGrid& grid = grids[0];
   if (batch) {
     batch_grids_.upload(grids.data(), grids.size() * sizeof(CellGrid));
     batch_cell_off_.upload(cell_off.data(), cell_off.size() * sizeof(int));
@@ -561,23 +678,35 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   // deferred: no host synchronisation; the neighbour capacity was validated by an earlier
   // synchronised call with the same atom count, and overflows are latched for check()
   const bool deferred = (opt_.deferred_check || md_capture_) && cap_known_ && N == cap_atoms_ &&
-                        !batch && !param_grad;
+                        !batch && !param_grad && !noalloc;
   if (capturing && !deferred)
     throw ArgumentError(
       "SUS2_SH: a captured evaluation needs sus2_sh_deferred_check=1, one structure, no "
       "parameter gradient, and a preceding uncaptured call with the same atom count");
-  const bool profile = opt_.profile && !capturing;
+  const bool profile = opt_.profile && !capturing && !noalloc;
   if (cap_ == 0) cap_ = 64;
   const int n_totals = static_cast<int>(cell_sort_totals(ncell));
-  if (capturing && (N > atoms_alloc_ || nl_.bytes() < static_cast<size_t>(cap_) * N * sizeof(Edge<Real>) ||
-                    cell_begin_.bytes() < (static_cast<size_t>(ncell) + 1) * sizeof(int) ||
-                    scan_totals_.bytes() < static_cast<size_t>(n_totals) * sizeof(int)))
-    throw ArgumentError("SUS2_SH: a captured evaluation cannot resize work buffers (the atom "
-                        "count or the cell grid grew since the last uncaptured call)");
-  ensure_atoms(N, sizeof(Edge<Real>));
-  cell_count_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
-  cell_begin_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
-  scan_totals_.resize(static_cast<size_t>(n_totals) * sizeof(int));
+  // force-tile neighbour windows (single structure, no parameter gradient)
+  const bool use_win = !batch && !param_grad;
+  const int win_words =
+    use_win ? static_cast<int>((static_cast<size_t>(N) + kWinTile - 1) / kWinTile) * kWinStride : 0;
+  if (noalloc) {
+    if (capturing) throw ArgumentError("SUS2_SH: compute_noalloc cannot be captured");
+    carve_scratch(N, ncell, n_totals, sizeof(Edge<Real>), win_words);
+  } else {
+    if (capturing && (N > atoms_alloc_ || nl_.bytes() < static_cast<size_t>(cap_) * N * sizeof(Edge<Real>) ||
+                      cell_begin_.bytes() < (static_cast<size_t>(ncell) + 1) * sizeof(int) ||
+                      scan_totals_.bytes() < static_cast<size_t>(n_totals) * sizeof(int) ||
+                      win_.bytes() < static_cast<size_t>(win_words) * sizeof(int)))
+      throw ArgumentError("SUS2_SH: a captured evaluation cannot resize work buffers (the atom "
+                          "count or the cell grid grew since the last uncaptured call)");
+    ensure_atoms(N, sizeof(Edge<Real>));
+    cell_count_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
+    cell_begin_.resize((static_cast<size_t>(ncell) + 1) * sizeof(int));
+    scan_totals_.resize(static_cast<size_t>(n_totals) * sizeof(int));
+    if (use_win) win_.resize(static_cast<size_t>(win_words) * sizeof(int));
+    bind_buffers(N, ncell, n_totals);
+  }
 
   if (profile && events_.empty()) {
     events_.resize(6);
@@ -585,7 +714,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (profile) hip_check(hipEventRecord(events_[0], s), "hipEventRecord");
 
-  launch_fill_int(max_count_.as<int>(), 0, 4, s);
+  launch_fill_int(ws_.max_count, 0, 4, s);
   CellSortArgs cs{};
   cs.N = N;
   cs.S = model_.species_count;
@@ -597,24 +726,24 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   cs.cell_off = batch ? batch_cell_off_.as<int>() : nullptr;
   cs.atom_off = batch ? batch_atom_off_.as<int>() : nullptr;
   cs.nstruct = nstruct;
-  cs.key = key_.as<int>();
-  cs.slot = slot_.as<int>();
-  cs.arrival = arrival_.as<int>();
-  cs.cell_count = cell_count_.as<int>();
-  cs.cell_begin = cell_begin_.as<int>();
-  cs.scan_totals = scan_totals_.as<int>();
-  cs.wrapped = wrapped_.as<double>();
+  cs.key = ws_.key;
+  cs.slot = ws_.slot;
+  cs.arrival = ws_.arrival;
+  cs.cell_count = ws_.cell_count;
+  cs.cell_begin = ws_.cell_begin;
+  cs.scan_totals = ws_.scan_totals;
+  cs.wrapped = ws_.wrapped;
   cs.sid = batch ? sid_.as<int>() : nullptr;
-  cs.satom = satom_.as<int>();
-  cs.rank = rank_.as<int>();
-  cs.skey = skey_.as<int>();
-  cs.pos_s = pos_s_.as<double>();
-  cs.packed = packed_.as<PackedAtom>();
-  cs.type_s = type_s_.as<int>();
-  cs.bad_type = max_count_.as<int>() + 1;
+  cs.satom = ws_.satom;
+  cs.rank = ws_.rank;
+  cs.skey = ws_.skey;
+  cs.pos_s = ws_.pos_s;
+  cs.packed = ws_.packed;
+  cs.type_s = ws_.type_s;
+  cs.bad_type = ws_.max_count + 1;
   cs.sid_s = batch ? sid_s_.as<int>() : nullptr;
   launch_cell_sort(cs, s);
-  const int* cell_begin = cell_begin_.as<int>();
+  const int* cell_begin = ws_.cell_begin;
   const int* cell_end = cell_begin + 1;
   NeighborArgs na{};
   na.N = N;
@@ -624,20 +753,18 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   na.sid_s = batch ? sid_s_.as<int>() : nullptr;
   na.rc2 = model_.max_dist * model_.max_dist;
   if (deferred) {
-    na.cap = cap_;
-    launch_neighbors<Real>(na, pos_s_.as<double>(), packed_.as<PackedAtom>(), skey_.as<int>(),
-                           cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
-                           max_count_.as<int>(), s);
-    launch_latch(max_count_.as<int>(), latch_.as<int>(), s);
+    na.cap = ws_.cap;
+    launch_neighbors<Real>(na, ws_.pos_s, ws_.packed, ws_.skey, cell_begin, cell_end, ws_.count,
+                           static_cast<Edge<Real>*>(ws_.nl), ws_.max_count, s);
+    launch_latch(ws_.max_count, latch_.as<int>(), s);
   } else {
     for (int attempt = 0; attempt < 2; ++attempt) {
-      na.cap = cap_;
-      if (attempt > 0) launch_fill_int(max_count_.as<int>(), 0, 1, s);
-      launch_fill_int(max_count_.as<int>() + 2, 0, 2, s);
-      launch_neighbors<Real>(na, pos_s_.as<double>(), packed_.as<PackedAtom>(), skey_.as<int>(),
-                             cell_begin, cell_end, count_.as<int>(), nl_.as<Edge<Real>>(),
-                             max_count_.as<int>(), s);
-      hip_check(hipMemcpyAsync(host_max_, max_count_.as<int>(), 4 * sizeof(int),
+      na.cap = ws_.cap;
+      if (attempt > 0) launch_fill_int(ws_.max_count, 0, 1, s);
+      launch_fill_int(ws_.max_count + 2, 0, 2, s);
+      launch_neighbors<Real>(na, ws_.pos_s, ws_.packed, ws_.skey, cell_begin, cell_end, ws_.count,
+                             static_cast<Edge<Real>*>(ws_.nl), ws_.max_count, s);
+      hip_check(hipMemcpyAsync(host_max_, ws_.max_count, 4 * sizeof(int),
                                hipMemcpyDeviceToHost, s),
                 "hipMemcpyAsync D2H");
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -648,11 +775,20 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       mean_neighbors_ = static_cast<double>(total) / N;
       if (host_max_[1] != 0)
         throw ArgumentError("atom type outside [0, species_count) passed to compute");
-      if (maxc <= cap_) break;
+      if (maxc <= ws_.cap) break;
+      if (noalloc) {
+        std::ostringstream msg;
+        msg << "SUS2_SH: compute_noalloc: " << maxc << " neighbours of one atom exceed the "
+            << ws_.cap << " that the scratch array (" << (kScratchBytes >> 20)
+            << " MiB) holds for " << N << " atoms";
+        throw ArgumentError(msg.str());
+      }
       cap_ = ((maxc + 8 + 7) / 8) * 8;
       ensure_atoms(N, sizeof(Edge<Real>));
+      ws_.nl = nl_.as<void>();
+      ws_.cap = cap_;
     }
-    if (!batch) {
+    if (!batch && !noalloc) {
       cap_known_ = true;
       cap_atoms_ = N;
     }
@@ -664,9 +800,9 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.L = model_.l_max;
   el.K = model_.k_max;
   el.B = model_.basic_count;
-  el.count = count_.as<int>();
-  el.nl = nl_.as<Edge<Real>>();
-  el.type_s = type_s_.as<int>();
+  el.count = ws_.count;
+  el.nl = static_cast<const Edge<Real>*>(ws_.nl);
+  el.type_s = ws_.type_s;
   el.rt.coeff = radial_coeff_.as<Real>();
   el.rt.scal = radial_scal_.as<Real>();
   el.rt.S = model_.species_count;
@@ -675,7 +811,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.rt.kind = static_cast<int>(model_.radial_kind);  // 0 tanh, 1 rational, 2 Laguerre
   el.rt.uniform = radial_uniform_ ? 1 : 0;
   el.fmap = fmap_.as<int>();
-  if (!launch_basics<Real>(el, basics_.as<Real>(), s))
+  if (!launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s))
     throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
   if (profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
 
@@ -716,7 +852,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     pgo.Q = Q;
     pgo.S = S;
   }
-  if (opt_.autotune && !tuned_ && !param_grad && !capturing && N >= 4096) {
+  if (opt_.autotune && !tuned_ && !param_grad && !capturing && !noalloc && N >= 4096) {
     // Autotuner: time every feasible candidate -- (variant, tile) of the workgroup kernel
     // and the lane program -- and keep the fastest (HIP events on the engine's stream; the
     // graph kernels are idempotent, so the trial launches only rewrite e_s and grads).
@@ -777,8 +913,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (param_grad) {
     // the parameter gradient needs per-scalar sums: persistent row-mode kernel
-    if (!launch_graph<Real>(tile_, 0, N, basics_.as<Real>(), type_s_.as<int>(), gt,
-                            e_s_.as<double>(), grads_.as<Real>(), &pgo, s, max_lds_,
+    if (!launch_graph<Real>(tile_, 0, N, static_cast<const Real*>(ws_.basics), ws_.type_s, gt,
+                            ws_.e_s, static_cast<Real*>(ws_.grads), &pgo, s, max_lds_,
                             4 * cu_count_))
       throw UnsupportedError("SUS2_SH: graph tile does not fit in LDS.");
   } else if (!launch_graph_variant<Real>(variant_, tile_, N, gt, s)) {
@@ -790,27 +926,24 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   // neighbour forces, budget 48 KiB (FP32) / 64 KiB (FP64) of accumulators per tile
   const int* win = nullptr;
   int win_cap = 0;
-  if (!batch && !param_grad) {
+  if (use_win) {
     win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * sizeof(Real)));
-    const size_t tiles = (static_cast<size_t>(N) + kWinTile - 1) / kWinTile;
-    win_.resize(tiles * kWinStride * sizeof(int));
-    launch_windows(N, grid, skey_.as<int>(), win_.as<int>(), win_cap, s);
-    win = win_.as<int>();
+    launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
+    win = ws_.win;
   }
-  launch_fill_double(f_s_.as<double>(), 0.0, 3 * static_cast<size_t>(N), s);
-  if (!launch_forces<Real>(el, grads_.as<Real>(), f_s_.as<double>(), v_s_.as<double>(),
+  launch_fill_double(ws_.f_s, 0.0, 3 * static_cast<size_t>(N), s);
+  if (!launch_forces<Real>(el, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
                            param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
   if (model_.zbl_enabled) {
     const ZblTable zt{zbl_z_.as<int>(), zbl_inner_.as<double>(), zbl_outer_.as<double>(),
                       model_.species_count};
-    launch_zbl<Real>(N, count_.as<int>(), nl_.as<Edge<Real>>(), type_s_.as<int>(), zt,
-                     e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(), s);
+    launch_zbl<Real>(N, ws_.count, static_cast<const Edge<Real>*>(ws_.nl), ws_.type_s, zt,
+                     ws_.e_s, ws_.f_s, ws_.v_s, s);
   }
   if (profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
-  launch_scatter(N, rank_.as<int>(), e_s_.as<double>(), f_s_.as<double>(), v_s_.as<double>(),
-                 pot, force, virial, s);
+  launch_scatter(N, ws_.rank, ws_.e_s, ws_.f_s, ws_.v_s, pot, force, virial, s);
   hip_check(hipGetLastError(), "kernel launch");
   if (param_grad) finish_param_grad(param_grad, s);
   if (profile) {

@@ -75,6 +75,21 @@ struct CapacityError : public std::runtime_error {
   explicit CapacityError(const std::string& msg) : std::runtime_error(msg) {}
 };
 
+// Device work buffers of one single-structure evaluation: views of the engine's own
+// allocations, or of the fixed scratch array in the allocation-free path.
+struct Workspace {
+  int *key = nullptr, *slot = nullptr, *arrival = nullptr, *skey = nullptr, *satom = nullptr,
+      *rank = nullptr, *type_s = nullptr, *count = nullptr;
+  double *wrapped = nullptr, *pos_s = nullptr, *e_s = nullptr, *f_s = nullptr, *v_s = nullptr;
+  PackedAtom* packed = nullptr;
+  void* nl = nullptr;      // Edge<Real>[cap][N]
+  void* basics = nullptr;  // Real[B][N]
+  void* grads = nullptr;   // Real[B][N]
+  int *cell_count = nullptr, *cell_begin = nullptr, *scan_totals = nullptr,
+      *max_count = nullptr, *win = nullptr;
+  int cap = 0;  // neighbour-list capacity per atom
+};
+
 class Engine {
  public:
   Engine(const std::string& path, int num_atoms, int nopts, const char** opts);
@@ -82,6 +97,10 @@ class Engine {
 
   void compute(const sus2sh_box& box, int N, const int* type, const double* pos, double* pot,
                double* force, double* virial, hipStream_t stream, double* param_grad);
+  // The same evaluation with every work buffer carved from the device's fixed scratch array
+  // (kScratchBytes): no device allocation, no autotuning, a synchronised neighbour count.
+  void compute_noalloc(const sus2sh_box& box, int N, const int* type, const double* pos,
+                       double* pot, double* force, double* virial, hipStream_t stream);
   // Batch of independent periodic structures: structure s owns atoms
   // [atom_offsets[s], atom_offsets[s+1]) and has box boxes[s] (host arrays).
   void compute_batch(int nstruct, const sus2sh_box* boxes, const int* atom_offsets,
@@ -134,8 +153,10 @@ class Engine {
   template <typename Real>
   void run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, int N, const int* type,
            const double* pos, double* pot, double* force, double* virial, hipStream_t stream,
-           double* param_grad);
+           double* param_grad, bool noalloc);
   void ensure_atoms(int N, size_t edge_bytes);
+  void bind_buffers(int N, int ncell, int n_totals);
+  void carve_scratch(int N, int ncell, int n_totals, size_t edge_bytes, int win_words);
   void reset_latch(hipStream_t s);
   void read_latch(hipStream_t s, int* out);
   void finish_param_grad(double* d_param_grad, hipStream_t s);
@@ -187,6 +208,7 @@ class Engine {
   DeviceBuffer batch_grids_, batch_cell_off_, batch_atom_off_, sid_, sid_s_;
   int* host_max_ = nullptr;  // pinned, 8 ints
 
+  Workspace ws_;  // work buffers of the evaluation in progress
   std::vector<float> stage_ms_;
   std::vector<hipEvent_t> events_;
   double mean_neighbors_ = 0.0;

@@ -22,6 +22,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_create",
     "sus2sh_destroy",
     "sus2sh_compute",
+    "sus2sh_compute_noalloc",
     "sus2sh_compute_param_grad",
     "sus2sh_compute_batch",
     "sus2sh_md_nve",
@@ -127,6 +128,9 @@ def lib():
     L.sus2sh_compute.argtypes = [c_void_p, P(Box), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]
     L.sus2sh_compute.restype = c_int
+    L.sus2sh_compute_noalloc.argtypes = [c_void_p, P(Box), c_int, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]
+    L.sus2sh_compute_noalloc.restype = c_int
     L.sus2sh_compute_param_grad.argtypes = [c_void_p, P(Box), c_int, c_void_p, c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p]
     L.sus2sh_compute_param_grad.restype = c_int

@@ -105,6 +105,24 @@ int sus2sh_compute(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                    const int* d_type, const double* d_position, double* d_potential,
                    double* d_force, double* d_virial, void* stream);
 
+/* Allocation-free evaluation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs (+=) and results as sus2sh_compute, but every work buffer (cell sort, neighbour
+ * list, basic moments and adjoints, per-atom energy / force / virial before the scatter) is
+ * carved from one fixed device array of 128 MiB per GPU, held from the library's first
+ * sus2sh_create on that GPU until the process ends; the call makes no device allocation or
+ * free and no allocating library call.  Limit: the per-atom buffers plus a neighbour list
+ * large enough for the densest atom must fit 128 MiB (about 65,000 atoms of an l3k3, rb 8,
+ * rc 5 A model in FP32, 30,000 in FP64); larger inputs return SUS2SH_ERR_ARGUMENT with the
+ * limit in the message.  The call synchronises `stream` once (neighbour count check), never
+ * runs the autotuner (it uses the graph kernel the handle holds) and cannot be captured.
+ * All calls on one GPU share its array and leave their contents in it: calls that overlap
+ * in time on one GPU (other streams, threads or handles) race and must be serialised by the
+ * caller.  The handle must have been created on the current device. */
+int sus2sh_compute_noalloc(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
+                           const int* d_type, const double* d_position, double* d_potential,
+                           double* d_force, double* d_virial, void* stream);
+
 /* Deferred checks.  With sus2_sh_deferred_check=1, a sus2sh_compute call that follows a
  * synchronised call with the same num_atoms enqueues its kernels without any host
  * synchronisation or allocation (so it can be captured in a HIP graph on a non-default

@@ -105,6 +105,22 @@ __device__ __forceinline__ float dev_tanh<float>(float z)
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
 }
 
+// tanh(z) and its derivative 1 - tanh(z)^2 = 4 q (1 - q), q = 1 / (exp(2|z|) + 1) <= 1/2: the
+// derivative keeps its relative precision where tanh saturates (1 - t*t would cancel).
+template <typename Real>
+__device__ __forceinline__ void dev_tanh_sech2(Real z, Real& t, Real& sech2)
+{
+  const Real az = z < Real(0) ? -z : z;
+  Real q;
+  if constexpr (sizeof(Real) == 4)
+    q = __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(az * 2.8853900817779268f) + 1.0f);
+  else
+    q = Real(1) / (exp(Real(2) * az) + Real(1));
+  const Real ta = Real(1) - Real(2) * q;
+  t = z < Real(0) ? -ta : ta;
+  sech2 = Real(4) * q * (Real(1) - q);
+}
+
 // xi(r) and dxi/dr for one (a, s)
 template <typename Real>
 __device__ __forceinline__ void radial_map(int kind, Real a, Real s, Real r, Real& xi, Real& dxi)

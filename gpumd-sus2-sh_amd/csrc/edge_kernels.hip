@@ -59,7 +59,9 @@ __device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr
   __syncthreads();
 }
 
-template <typename Real, int L, int K, int NB>
+// GATE: every edge's radial channel mu is weighted by the neighbour's gate multiplier
+// G_{j,mu} (two-layer mu gate, src/force/sus2_sh.cu:4246-4310).
+template <typename Real, int L, int K, int NB, bool GATE>
 __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __restrict__ out)
 {
   constexpr int MR = K * (L + 1);
@@ -95,6 +97,11 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
     Real R[MR], dR[MR];
     eval_radial<Real, MR, false, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                      a.rt.kind, r, R, dR);
+    if constexpr (GATE) {
+      const Real* gm = a.gate_m + static_cast<size_t>(e.j & kEdgeIndexMask) * MR;
+#pragma unroll
+      for (int mu = 0; mu < MR; ++mu) R[mu] *= gm[mu];
+    }
     Real Sv[NC];
     solid_values<Real, L>(e.dx * inv, e.dy * inv, e.dz * inv, Sv);
 #pragma unroll
@@ -188,7 +195,10 @@ __device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
 // component; a tile without windows (count < 0) falls back to per-edge global atomics.
 // The FP32 windowed kernel with 8-coefficient rows is held to 168 VGPRs (3 waves per SIMD)
 // for l_max <= 4; the l_max = 5, 6 grids would spill far more at that budget.
-template <typename Real, int L, int K, bool PG, int NB, bool WIN>
+// GATE (two-layer mu gate, src/force/sus2_sh.cu:6077-6123, 6555-6693): the edge's radial
+// channels are weighted by the neighbour's multiplier G_{j,mu}, and
+// dE/dh_{j,mu} += dG/dh_{j,mu} R_mu sum_m G_{k,l,m} Y_lm is scattered to the neighbour.
+template <typename Real, int L, int K, bool PG, int NB, bool WIN, bool GATE>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 4) ? 3 : 1))) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
@@ -268,6 +278,29 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
                                     a.rt.kind, r, R, dR);
     Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
     solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
+    if constexpr (GATE) {
+      const Real* gm = a.gate_m + static_cast<size_t>(jn) * MR;
+      const Real* gd = a.gate_d + static_cast<size_t>(jn) * MR;
+      Real* ga = a.gate_adj + static_cast<size_t>(jn) * MR;
+#pragma unroll
+      for (int l = 0; l <= L; ++l) {
+        const Real nl = Real(norm_of_l(l));
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+          const int mu = kk * (L + 1) + l;
+          Real proj = 0;
+#pragma unroll
+          for (int m = -l; m <= l; ++m) proj += G[full_index<L, K>(l, kk, m)] * Sv[sh_index(l, m)];
+          const Real v = gd[mu] * R[mu] * proj * nl;
+          if (v != Real(0)) unsafeAtomicAdd(ga + mu, v);
+        }
+      }
+#pragma unroll
+      for (int mu = 0; mu < MR; ++mu) {
+        R[mu] *= gm[mu];
+        dR[mu] *= gm[mu];
+      }
+    }
     // g = u * sum (V - l U / r) Y + (1/r) sum U N_l grad S,  with U = sum_k G R, V = sum_k G R'
     Real sv = 0, gx = 0, gy = 0, gz = 0;
 #pragma unroll
@@ -337,8 +370,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
     unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
     unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
+    if (a.v_accumulate) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+      for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] += static_cast<double>(w[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] = static_cast<double>(w[c]);
+    }
   }
   if (WIN && w_count >= 0) {
     __syncthreads();
@@ -393,18 +431,20 @@ template <typename Real, int LL>
 bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
 {
   const int blocks = (a.N + 255) / 256;
-  const int nb = uniform_nb(a);
+  const int nb = a.gate_m ? 0 : uniform_nb(a);  // the gated kernel uses the per-mu path
   const size_t lds = table_bytes(a, nb);
 #define SUS2SH_BASICS_CASE(KK)                                                    \
   if (a.K == KK) {                                                                \
-    if (nb == 8)                                                                  \
-      k_basics<Real, LL, KK, 8><<<blocks, 256, lds, s>>>(a, basics);              \
+    if (a.gate_m)                                                                 \
+      k_basics<Real, LL, KK, 0, true><<<blocks, 256, lds, s>>>(a, basics);        \
+    else if (nb == 8)                                                             \
+      k_basics<Real, LL, KK, 8, false><<<blocks, 256, lds, s>>>(a, basics);       \
     else if (nb == 12)                                                            \
-      k_basics<Real, LL, KK, 12><<<blocks, 256, lds, s>>>(a, basics);             \
+      k_basics<Real, LL, KK, 12, false><<<blocks, 256, lds, s>>>(a, basics);      \
     else if (nb == 16)                                                            \
-      k_basics<Real, LL, KK, 16><<<blocks, 256, lds, s>>>(a, basics);             \
+      k_basics<Real, LL, KK, 16, false><<<blocks, 256, lds, s>>>(a, basics);      \
     else                                                                          \
-      k_basics<Real, LL, KK, 0><<<blocks, 256, lds, s>>>(a, basics);              \
+      k_basics<Real, LL, KK, 0, false><<<blocks, 256, lds, s>>>(a, basics);       \
     return true;                                                                  \
   }
   SUS2SH_FOR_EACH_K(SUS2SH_BASICS_CASE)
@@ -418,8 +458,9 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
 {
   const int blocks = (a.N + 255) / 256;
   static_assert(kWinTile == 256, "force tiles are the 256-thread workgroups");
-  const int nb = pg_rad ? 0 : uniform_nb(a);  // the parameter gradient uses the per-mu path
-  const bool use_win = win && !pg_rad;
+  // the parameter gradient and the gated kernel use the per-mu path
+  const int nb = (pg_rad || a.gate_m) ? 0 : uniform_nb(a);
+  const bool use_win = win && !pg_rad && !a.gate_m;
   size_t lds = table_bytes(a, nb);
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
@@ -429,26 +470,28 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   if (use_win)
     lds = ((lds + 15) & ~size_t(15)) + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16) +
           3 * sizeof(Real) * static_cast<size_t>(win_cap);
-#define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV)                                           \
+#define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV, GV)                                       \
   {                                                                                            \
-    auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV>;                                       \
-    if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV>>();               \
+    auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV, GV>;                                   \
+    if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV, GV>>();           \
     kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
   }
 #define SUS2SH_FORCES_CASE(KK)                                                               \
   if (a.K == KK) {                                                                           \
     if (pg_rad) {                                                                            \
-      SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false)                                           \
+      SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false, false)                                    \
+    } else if (a.gate_m) {                                                                   \
+      SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, true)                                    \
     } else if (use_win) {                                                                    \
-      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true)                              \
-      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, true)                       \
-      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true)                       \
-      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true)                                      \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true, false)                       \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, true, false)                \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true, false)                \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true, false)                               \
     } else {                                                                                 \
-      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false)                             \
-      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, false)                      \
-      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false)                      \
-      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false)                                     \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false, false)                      \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, false, false)               \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false, false)               \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, false)                              \
     }                                                                                        \
     return true;                                                                             \
   }

@@ -396,6 +396,91 @@ void Engine::upload_tables()
   stored_index_.upload(g.stored_scalar_index.data(), g.stored_scalar_index.size() * sizeof(int));
   stored_slot_.upload(g.stored_scalar_slot.data(), g.stored_scalar_slot.size() * sizeof(int));
   if (lane_.ok) upload_lane();
+  if (m.gate_enabled) upload_gate_tables<Real>();
+}
+
+// Gate layer tables (two-layer mu gate): the gate radial coefficients in the main table's
+// layout, the gate program -- the model's graph compiled for the gate scalars, each stored
+// in a slot -- and h = weff s with weff[mu][q] the mu-scalar-full weights or, for
+// mu-body-linear-combo, c_{mu, body(q)} w_q (h = sum_p c_{mu,p} sum_{q in p} w_q s_q).
+template <typename Real>
+void Engine::upload_gate_tables()
+{
+  const Model& m = model_;
+  std::vector<double> coeff, scal;
+  build_radial_tables(m, coeff, scal, &m.gate_radial_coeffs);
+  std::vector<Real> cR(coeff.begin(), coeff.end());
+  gate_coeff_.upload(cR.data(), cR.size() * sizeof(Real));
+
+  Model gm = m;  // same graph, the gate scalars as its scalar list
+  const int Qg = static_cast<int>(m.gate_moment_index.size());
+  gm.scalar_count = Qg;
+  gm.moment_mapping = m.gate_moment_index;
+  gm.moment_coeffs.assign(Qg, 1.0);
+  gate_prog_ = compile_graph(gm, 8, 256, true);
+  const GraphProgram& g = gate_prog_;
+  for (int t : g.fwd_row_target)
+    if (t < 0) throw DeviceError("gate program: a row without a slot");
+  for (uint32_t idx : g.bwd_term_idx)
+    if ((idx >> 16) == kNoTarget) throw DeviceError("gate program: a terminal back term");
+  std::vector<int> slot(Qg);
+  for (int q = 0; q < Qg; ++q) {
+    slot[q] = g.node_slot[m.gate_moment_index[q]];
+    if (slot[q] < 0) throw DeviceError("gate program: a gate scalar without a slot");
+  }
+  const int Mr = m.radial_funcs_count, K = m.gate_body_count;
+  std::vector<Real> weff(static_cast<size_t>(Mr) * Qg);
+  for (int mu = 0; mu < Mr; ++mu)
+    for (int q = 0; q < Qg; ++q)
+      weff[static_cast<size_t>(mu) * Qg + q] = static_cast<Real>(
+        m.gate_mode == kGateScalarFull
+          ? m.gate_weights[static_cast<size_t>(mu) * Qg + q]
+          : m.gate_body_mix[static_cast<size_t>(mu) * K + m.gate_body_id[q]] * m.gate_weights[q]);
+  gate_weff_.upload(weff.data(), weff.size() * sizeof(Real));
+  gate_slot_.upload(slot.data(), slot.size() * sizeof(int));
+  gate_add_.upload(m.gate_additive.data(), m.gate_additive.size() * sizeof(double));
+
+  const int rows = static_cast<int>(g.fwd_row_target.size());
+  std::vector<int4> frows(rows);
+  for (int r = 0; r < rows; ++r)
+    frows[r] = make_int4(g.fwd_row_target[r], g.fwd_row_begin[r], g.fwd_row_count[r], 0);
+  g_fwd_rows_.upload(frows.data(), frows.size() * sizeof(int4));
+  std::vector<Term<Real>> ft(g.fwd_term_idx.size());
+  for (size_t t = 0; t < ft.size(); ++t) {
+    ft[t] = Term<Real>{};
+    ft[t].idx = g.fwd_term_idx[t];
+    ft[t].c = static_cast<Real>(g.fwd_term_coeff[t]);
+  }
+  g_fwd_terms_.upload(ft.data(), ft.size() * sizeof(Term<Real>));
+  g_fwd_layer_.upload(g.fwd_layer_begin.data(), g.fwd_layer_begin.size() * sizeof(int));
+  const int chunks = static_cast<int>(g.bwd_chunk_src.size());
+  std::vector<int4> bch(chunks);
+  for (int c = 0; c < chunks; ++c)
+    bch[c] = make_int4(g.bwd_chunk_src[c], g.bwd_chunk_begin[c], g.bwd_chunk_count[c],
+                       g.bwd_chunk_atomic[c]);
+  g_bwd_chunks_.upload(bch.data(), bch.size() * sizeof(int4));
+  std::vector<Term<Real>> bt(g.bwd_term_idx.size());
+  for (size_t t = 0; t < bt.size(); ++t) {
+    bt[t] = Term<Real>{};
+    bt[t].idx = g.bwd_term_idx[t];
+    bt[t].c = static_cast<Real>(g.bwd_term_coeff[t]);
+  }
+  g_bwd_terms_.upload(bt.data(), bt.size() * sizeof(Term<Real>));
+  g_bwd_layer_.upload(g.bwd_layer_begin.data(), g.bwd_layer_begin.size() * sizeof(int));
+
+  // tile: the largest power of two whose values and adjoints fit 64 KiB of LDS
+  const size_t budget = std::min<size_t>(max_lds_, 64 * 1024);
+  gate_tile_ = 0;
+  for (int t : {16, 8, 4, 2, 1})
+    if (gate_graph_lds_bytes(t, g.slot_count, static_cast<int>(sizeof(Real))) <= budget) {
+      gate_tile_ = t;
+      break;
+    }
+  if (gate_tile_ == 0) {
+    if (gate_graph_lds_bytes(1, g.slot_count, static_cast<int>(sizeof(Real))) > max_lds_)
+      throw UnsupportedError("SUS2_SH: the gate layer's graph state exceeds the LDS of a workgroup.");
+    gate_tile_ = 1;
+  }
 }
 
 void Engine::ensure_atoms(int N, size_t edge_bytes)
@@ -419,6 +504,14 @@ void Engine::ensure_atoms(int N, size_t edge_bytes)
     e_s_.resize(n * sizeof(double));
     f_s_.resize(3 * n * sizeof(double));
     v_s_.resize(9 * n * sizeof(double));
+    if (model_.gate_enabled) {
+      const size_t nm = n * static_cast<size_t>(model_.radial_funcs_count) * rsz;
+      gbasics_.resize(static_cast<size_t>(model_.basic_count) * n * rsz);
+      ggrads_.resize(static_cast<size_t>(model_.basic_count) * n * rsz);
+      gate_m_.resize(nm);
+      gate_d_.resize(nm);
+      gate_adj_.resize(nm);
+    }
     atoms_alloc_ = N;
   }
   max_count_.resize(64);
@@ -452,6 +545,11 @@ void Engine::bind_buffers(int N, int ncell, int n_totals)
   ws_.scan_totals = scan_totals_.as<int>();
   ws_.max_count = max_count_.as<int>();
   ws_.win = win_.as<int>();
+  ws_.gbasics = gbasics_.as<void>();
+  ws_.ggrads = ggrads_.as<void>();
+  ws_.gate_m = gate_m_.as<void>();
+  ws_.gate_d = gate_d_.as<void>();
+  ws_.gate_adj = gate_adj_.as<void>();
   ws_.cap = cap_;
 }
 
@@ -641,6 +739,12 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
 {
   if (N >= kMaxEdgeAtoms)
     throw ArgumentError("SUS2_SH: at most 134217727 atoms per call (neighbour index width)");
+  const bool gate = model_.gate_enabled;
+  if (gate && param_grad)
+    throw UnsupportedError("SUS2_SH: dE/dtheta is not implemented for two-layer gate models.");
+  if (gate && noalloc)
+    throw UnsupportedError("SUS2_SH: compute_noalloc does not evaluate two-layer gate models "
+                           "(use sus2sh_compute).");
   if (param_grad && model_.radial_kind == RadialKind::LaguerreLog1p)
     throw UnsupportedError("SUS2_SH: dE/dtheta is not implemented for RBLaguerre_log1p models.");
   const bool batch = nstruct > 1;
@@ -811,7 +915,40 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.rt.kind = static_cast<int>(model_.radial_kind);  // 0 tanh, 1 rational, 2 Laguerre
   el.rt.uniform = radial_uniform_ ? 1 : 0;
   el.fmap = fmap_.as<int>();
-  if (!launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s))
+  // two-layer gate (src/force/sus2_sh.cu:9821-9893): gate moments with the gate radial
+  // coefficients, the gate layer's graph, G = 1 + A tanh(a h) per atom and channel; the main
+  // moments then weight every edge by the neighbour's G
+  GraphTable<Real> gg{};
+  GateTable<Real> ga{};
+  if (gate) {
+    EdgeLaunch<Real> eg = el;
+    eg.rt.coeff = gate_coeff_.as<Real>();
+    if (!launch_basics<Real>(eg, static_cast<Real*>(ws_.gbasics), s))
+      throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
+    gg.fwd_rows = g_fwd_rows_.as<int4>();
+    gg.fwd_terms = g_fwd_terms_.as<Term<Real>>();
+    gg.fwd_layer = g_fwd_layer_.as<int>();
+    gg.bwd_chunks = g_bwd_chunks_.as<int4>();
+    gg.bwd_terms = g_bwd_terms_.as<Term<Real>>();
+    gg.bwd_layer = g_bwd_layer_.as<int>();
+    gg.slots = gate_prog_.slot_count;
+    gg.basics = model_.basic_count;
+    gg.layers = gate_prog_.layer_count;
+    ga.weff = gate_weff_.as<Real>();
+    ga.slot = gate_slot_.as<int>();
+    ga.additive = gate_add_.as<double>();
+    ga.amplitude = model_.gate_amplitude;
+    ga.Mr = model_.radial_funcs_count;
+    ga.Qg = static_cast<int>(model_.gate_moment_index.size());
+    if (!launch_gate_graph<Real>(0, gate_tile_, N, static_cast<const Real*>(ws_.gbasics),
+                                 ws_.type_s, gg, ga, static_cast<Real*>(ws_.gate_m),
+                                 static_cast<Real*>(ws_.gate_d), nullptr, nullptr, s, max_lds_))
+      throw UnsupportedError("SUS2_SH: gate graph tile does not fit in LDS.");
+    el.gate_m = static_cast<const Real*>(ws_.gate_m);
+  }
+  const bool basics_ok = launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s);
+  el.gate_m = nullptr;
+  if (!basics_ok)
     throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
   if (profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
 
@@ -932,7 +1069,30 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     win = ws_.win;
   }
   launch_fill_double(ws_.f_s, 0.0, 3 * static_cast<size_t>(N), s);
-  if (!launch_forces<Real>(el, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
+  if (gate) {
+    // gated main forces and dE/dh scattered to the neighbours, the reverse through the gate
+    // layer's graph, then the gate moments' own forces (src/force/sus2_sh.cu:10052-10200)
+    const size_t nm = static_cast<size_t>(N) * model_.radial_funcs_count;
+    launch_fill_int(static_cast<int*>(ws_.gate_adj), 0, nm * sizeof(Real) / sizeof(int), s);
+    EdgeLaunch<Real> ef = el;
+    ef.gate_m = static_cast<const Real*>(ws_.gate_m);
+    ef.gate_d = static_cast<const Real*>(ws_.gate_d);
+    ef.gate_adj = static_cast<Real*>(ws_.gate_adj);
+    if (!launch_forces<Real>(ef, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s, nullptr,
+                             nullptr, 0, s))
+      throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+    if (!launch_gate_graph<Real>(1, gate_tile_, N, static_cast<const Real*>(ws_.gbasics),
+                                 ws_.type_s, gg, ga, nullptr, nullptr,
+                                 static_cast<const Real*>(ws_.gate_adj),
+                                 static_cast<Real*>(ws_.ggrads), s, max_lds_))
+      throw UnsupportedError("SUS2_SH: gate graph tile does not fit in LDS.");
+    EdgeLaunch<Real> e2 = el;
+    e2.rt.coeff = gate_coeff_.as<Real>();
+    e2.v_accumulate = 1;
+    if (!launch_forces<Real>(e2, static_cast<const Real*>(ws_.ggrads), ws_.f_s, ws_.v_s, nullptr,
+                             win, win_cap, s))
+      throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+  } else if (!launch_forces<Real>(el, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
                            param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
     throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
   if (model_.zbl_enabled) {
@@ -1210,5 +1370,7 @@ void Engine::finish_param_grad(double* d_param_grad, hipStream_t s)
 
 template void Engine::upload_tables<float>();
 template void Engine::upload_tables<double>();
+template void Engine::upload_gate_tables<float>();
+template void Engine::upload_gate_tables<double>();
 
 }  // namespace sus2sh

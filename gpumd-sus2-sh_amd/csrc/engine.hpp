@@ -87,6 +87,10 @@ struct Workspace {
   void* grads = nullptr;   // Real[B][N]
   int *cell_count = nullptr, *cell_begin = nullptr, *scan_totals = nullptr,
       *max_count = nullptr, *win = nullptr;
+  // two-layer gate models: gate moments and their adjoints Real[B][N]; per atom and radial
+  // channel Real[N][Mr]: multiplier G, dG/dh and dE/dh
+  void *gbasics = nullptr, *ggrads = nullptr, *gate_m = nullptr, *gate_d = nullptr,
+       *gate_adj = nullptr;
   int cap = 0;  // neighbour-list capacity per atom
 };
 
@@ -138,6 +142,8 @@ class Engine {
  private:
   template <typename Real>
   void upload_tables();
+  template <typename Real>
+  void upload_gate_tables();
   struct LaneShape {
     int waves;      // waves per 64-atom tile
     int nslot;      // accumulators per wave
@@ -164,6 +170,9 @@ class Engine {
 
   Model model_;
   GraphProgram prog_;
+  // two-layer gate: the gate layer's graph program (gate scalars stored), its workgroup tile
+  GraphProgram gate_prog_;
+  int gate_tile_ = 1;
   EngineOptions opt_;
   int tile_ = 1;
   int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane
@@ -191,6 +200,11 @@ class Engine {
   DeviceBuffer win_;  // neighbour windows of the force tiles
   ZblTables zbl_;     // model.zbl_enabled: atomic numbers and pair cutoffs
   DeviceBuffer zbl_z_, zbl_inner_, zbl_outer_;
+  // two-layer gate tables: radial coefficients, gate program, weights [Mr][Qg], slots,
+  // additive coefficients
+  DeviceBuffer gate_coeff_, g_fwd_rows_, g_fwd_terms_, g_fwd_layer_, g_bwd_chunks_, g_bwd_terms_,
+    g_bwd_layer_, gate_weff_, gate_slot_, gate_add_;
+  DeviceBuffer gbasics_, ggrads_, gate_m_, gate_d_, gate_adj_;
   int cu_count_ = 256;
   bool radial_uniform_ = false;
 

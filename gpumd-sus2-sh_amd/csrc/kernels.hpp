@@ -162,6 +162,12 @@ struct EdgeLaunch {
   const int* type_s;
   RadialTable<Real> rt;
   const int* fmap;  // [K (L+1)^2] full-grid index -> basic index, or -1
+  // two-layer gate (null otherwise), per sorted atom and radial channel [N][Mr]:
+  // gate_m = G multiplier of a neighbour, gate_d = its derivative dG/dh, gate_adj += dE/dh
+  const Real* gate_m = nullptr;
+  const Real* gate_d = nullptr;
+  Real* gate_adj = nullptr;
+  int v_accumulate = 0;  // forces: add the per-atom virial to v_s instead of storing it
 };
 
 template <typename Real>
@@ -217,6 +223,26 @@ bool lane_shape_supported(int waves, int nslot);
 template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);
+
+// --- two-layer gate (gate_kernels.hip) ---
+// Gate values h_{a,mu} = sum_q weff[mu][q] s_q(a) over the gate scalars s_q of the gate
+// program (stored slots), G = 1 + A tanh(a_t h).
+template <typename Real>
+struct GateTable {
+  const Real* weff;        // [Mr][Qg]
+  const int* slot;         // [Qg] gate-program slot of scalar q
+  const double* additive;  // [S] a_t
+  double amplitude;        // A
+  int Mr;
+  int Qg;
+};
+// mode 0: gate moments gbasics [B][N] -> mult, dmult [N][Mr]; mode 1: per-atom scalar seeds
+// sum_mu adj[a][mu] weff[mu][q] -> gate basic adjoints ggrad [B][N] (no species factor).
+template <typename Real>
+bool launch_gate_graph(int mode, int tile, int N, const Real* gbasics, const int* type_s,
+                       const GraphTable<Real>& gt, const GateTable<Real>& ga, Real* mult,
+                       Real* dmult, const Real* adj, Real* ggrad, hipStream_t s, size_t max_lds);
+size_t gate_graph_lds_bytes(int tile, int slots, int real_bytes);
 
 // --- velocity-Verlet NVE driver (md.hip) ---
 void launch_half_kick(int N, double half_dt, const int* type, const double* mass, const double* f,

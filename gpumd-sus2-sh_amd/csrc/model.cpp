@@ -690,6 +690,117 @@ void prepare_graph_metadata(Model& m)
 }
 
 // ---------------------------------------------------------------------------
+// Two-layer mu gate: header keys (sus2_sh.cu:2153-2266, mode names :495-518) and the checks
+// of the gate scalar list against sh_scalar_info and the graph (sus2_sh.cu:2412-2478).  The
+// checks and their messages are the reference's, so that a gate file accepted or rejected
+// there is accepted or rejected here.
+// ---------------------------------------------------------------------------
+static void parse_gate_header(const std::string& text, Model& m)
+{
+  const std::string mode =
+    has_token(text, "two_layer_gate_mode =") ? string_after(text, "two_layer_gate_mode =") : "legacy";
+  if (mode == "mu-body-linear-combo")
+    m.gate_mode = kGateLinearCombo;
+  else if (mode == "mu-scalar-full" || mode == "mu-body-order")
+    m.gate_mode = kGateScalarFull;
+  else if (mode != "legacy")
+    fail("Unsupported SUS2_SH two_layer_gate_mode in GPUMD: " + mode);
+  if (m.gate_mode == 0)
+    fail("GPUMD SUS2_SH mu-body-order branch requires two_layer_gate_mode = "
+         "mu-body-linear-combo or mu-scalar-full; legacy gate models are not supported here.");
+  if (optional_bool_after(text, "two_layer_gate_include_one_body =", false))
+    fail("GPUMD SUS2_SH gate backend requires two_layer_gate_include_one_body = false.");
+  if (has_token(text, "two_layer_gate_scale_mode ="))
+    fail("GPUMD SUS2_SH mu-body gate does not support two_layer_gate_scale_mode.");
+  if (has_token(text, "two_layer_gate_bias ="))
+    fail("GPUMD SUS2_SH mu-body gate does not support two_layer_gate_bias.");
+  if (optional_bool_after(text, "two_layer_residual_enabled =", false))
+    fail("GPUMD SUS2_SH mu-body gate does not support residual gate fields.");
+  if (has_token(text, "two_layer_gate_site_mode =")) {
+    const std::string site = string_after(text, "two_layer_gate_site_mode =");
+    if (site != "neighbor" && site != "double")
+      fail("Unsupported SUS2_SH two_layer_gate_site_mode in GPUMD: " + site);
+    if (site != "neighbor")
+      fail("GPUMD SUS2_SH mu-body gate currently supports two_layer_gate_site_mode = neighbor. "
+           "Double-site gate will be added after the single-gate CUDA path is verified.");
+  }
+  m.gate_amplitude = has_token(text, "two_layer_gate_tanh_amplitude =")
+                       ? double_after(text, "two_layer_gate_tanh_amplitude =")
+                       : 0.8;
+  if (!std::isfinite(m.gate_amplitude) || m.gate_amplitude < 0.0 || m.gate_amplitude > 1.0)
+    fail("SUS2_SH two_layer_gate_tanh_amplitude should be finite and in [0,1].");
+  const std::string radial_mode = has_token(text, "two_layer_gate_radial_mode =")
+                                    ? string_after(text, "two_layer_gate_radial_mode =")
+                                    : std::string();
+  if (radial_mode != "shared-radial")
+    fail("GPUMD SUS2_SH gate backend currently requires shared-radial gate mode.");
+  if (int_after(text, "two_layer_gate_body_order_max =") != m.k_max + 1)
+    fail("SUS2_SH two_layer_gate_body_order_max should equal sh_k_max + 1.");
+  m.gate_body_count = m.k_max;
+  const int radial_count = int_after(text, "two_layer_gate_radial_coeff_count =");
+  if (radial_count != m.radial_funcs_count * m.rb_size)
+    fail("SUS2_SH two-layer gate radial coefficient count is inconsistent.");
+  m.gate_radial_coeffs = numbers<double>(braced_after(text, "two_layer_gate_radial_coeffs ="));
+  if (static_cast<int>(m.gate_radial_coeffs.size()) != radial_count)
+    fail("SUS2_SH two-layer gate radial coeff list has wrong size.");
+  if (has_token(text, "two_layer_gate_additive_coeff_count =")) {
+    const int n = int_after(text, "two_layer_gate_additive_coeff_count =");
+    if (n != m.species_count)
+      fail("SUS2_SH two-layer gate additive coefficient count is inconsistent.");
+    m.gate_additive = numbers<double>(braced_after(text, "two_layer_gate_additive_coeffs ="));
+    if (static_cast<int>(m.gate_additive.size()) != n)
+      fail("SUS2_SH two-layer gate additive coeff list has wrong size.");
+  } else {
+    m.gate_additive.assign(m.species_count, 1.0);
+  }
+  const int weight_count = int_after(text, "two_layer_gate_weight_count =");
+  if (weight_count <= 0) fail("SUS2_SH two-layer gate should contain at least one scalar weight.");
+  m.gate_scalar_indices = numbers<int>(braced_after(text, "two_layer_gate_scalar_indices ="));
+  m.gate_weights = numbers<double>(braced_after(text, "two_layer_gate_weights ="));
+  const int Qg = static_cast<int>(m.gate_scalar_indices.size());
+  if (Qg <= 0) fail("SUS2_SH two-layer gate scalar index list is empty.");
+  const int expected = m.gate_mode == kGateScalarFull ? m.radial_funcs_count * Qg : Qg;
+  if (weight_count != expected || static_cast<int>(m.gate_weights.size()) != expected)
+    fail("SUS2_SH two-layer gate scalar index/weight dimensions are inconsistent.");
+  if (m.gate_mode == kGateLinearCombo) {
+    const int n = int_after(text, "two_layer_gate_body_mix_weight_count =");
+    if (n != m.radial_funcs_count * m.gate_body_count)
+      fail("SUS2_SH two-layer gate body-mix weight count is inconsistent.");
+    m.gate_body_mix = numbers<double>(braced_after(text, "two_layer_gate_body_mix_weights ="));
+    if (static_cast<int>(m.gate_body_mix.size()) != n)
+      fail("SUS2_SH two-layer gate body-mix weight list has wrong size.");
+  } else if (has_token(text, "two_layer_gate_body_mix_weights =") ||
+             has_token(text, "two_layer_gate_body_mix_weight_count =")) {
+    fail("SUS2_SH mu-scalar-full gate should not contain body-mix weights.");
+  }
+  if (!has_token(text, "sh_scalar_info_count ="))
+    fail("GPUMD SUS2_SH mu-body gate requires sh_scalar_info metadata.");
+}
+
+static void check_gate_scalars(Model& m)
+{
+  const int Qg = static_cast<int>(m.gate_scalar_indices.size());
+  m.gate_moment_index.assign(Qg, 0);
+  m.gate_body_id.assign(Qg, 0);
+  std::vector<char> present(m.gate_body_count, 0);
+  for (int q = 0; q < Qg; ++q) {
+    const int s = m.gate_scalar_indices[q];
+    if (s < 0 || s >= m.scalar_count) fail("SUS2_SH two-layer gate scalar index is out of range.");
+    const int body = m.scalar_body_orders[s] - 2;
+    if (body < 0 || body >= m.gate_body_count)
+      fail("SUS2_SH two-layer gate scalar body order is outside the gate range.");
+    m.gate_body_id[q] = body;
+    present[body] = 1;
+    const int node = m.moment_mapping[s];
+    if (node < 0 || node >= m.moments_count)
+      fail("SUS2_SH two-layer gate mapped moment index is out of range.");
+    m.gate_moment_index[q] = node;
+  }
+  for (int b = 0; b < m.gate_body_count; ++b)
+    if (!present[b]) fail("SUS2_SH two-layer gate scalar list misses a required body-order bucket.");
+}
+
+// ---------------------------------------------------------------------------
 // Loader (sus2_sh.cu:2105-2482)
 // ---------------------------------------------------------------------------
 Model parse_model_text(const std::string& text)
@@ -730,8 +841,8 @@ Model parse_model_text(const std::string& text)
   m.moments_count = int_after(text, "alpha_moments_count =");
   m.basic_count = int_after(text, "alpha_index_basic_count =");
   m.scalar_count = int_after(text, "alpha_scalar_moments =");
-  if (optional_bool_after(text, "two_layer_gate_enabled =", false))
-    fail("SUS2_SH two-layer mu-body gate models are not supported by this engine.");
+  m.gate_enabled = optional_bool_after(text, "two_layer_gate_enabled =", false);
+  if (m.gate_enabled) parse_gate_header(text, m);
 
   if (m.scaling_map != "LK") fail("SUS2_SH requires scaling_map = LK.");
   if (m.l_max < 0 || m.l_max > kMaxL) fail("SUS2_SH supports sh_l_max in [0,6].");
@@ -830,6 +941,7 @@ Model parse_model_text(const std::string& text)
       static_cast<int>(m.moment_coeffs.size()) != m.scalar_count)
     fail("Unexpected SUS2_SH coefficient dimensions.");
 
+  if (m.gate_enabled) check_gate_scalars(m);
   prepare_graph_metadata(m);
   return m;
 }
@@ -849,8 +961,9 @@ bool probe_model_file(const std::string& path)
 }
 
 void build_radial_tables(const Model& m, std::vector<double>& coeffs,
-                         std::vector<double>& scal_shift)
+                         std::vector<double>& scal_shift, const std::vector<double>* radial)
 {
+  const std::vector<double>& rc = radial ? *radial : m.radial_coeffs;
   const int S = m.species_count;
   const int Mr = m.radial_funcs_count;
   const int rb = m.rb_size;
@@ -867,7 +980,7 @@ void build_radial_tables(const Model& m, std::vector<double>& coeffs,
           m.scal_coeffs[so + static_cast<size_t>(S) * S];
         for (int n = 0; n < rb; ++n)
           coeffs[(static_cast<size_t>(pair) * Mr + mu) * rb + n] =
-            m.radial_coeffs[static_cast<size_t>(mu) * rb + n] * scale;
+            rc[static_cast<size_t>(mu) * rb + n] * scale;
       }
     }
 }

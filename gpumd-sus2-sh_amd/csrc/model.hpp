@@ -31,6 +31,9 @@ constexpr int kMaxRbSize = 16;        // sus2_sh.cu:32
 constexpr int kMaxBasics = 320;       // sus2_sh.cu:33
 constexpr int kMaxSpecies = 16;
 
+constexpr int kGateLinearCombo = 1;  // two_layer_gate_mode = mu-body-linear-combo
+constexpr int kGateScalarFull = 2;   // two_layer_gate_mode = mu-scalar-full (or mu-body-order)
+
 enum class RadialKind : int {
   ChebyshevSSS = 0,          // xi = tanh(a (r - s) / 2)
   ChebyshevSSSRational = 1,  // xi = z / sqrt(1 + z^2), z = a (r - s) / 2 (README.md:55-56)
@@ -95,6 +98,21 @@ struct Model {
   double zbl_typewise_factor = 0.7;
   std::vector<int> zbl_atomic_numbers;  // optional metadata, one per species
 
+  // two-layer mu gate (docs/mu-body-order-gate.md:25-63; loader sus2_sh.cu:2153-2266,
+  // 2412-2478): a first graph pass on gate moments gives per-atom scalars s_q, combined into
+  // h_{a,mu}; every main-moment edge i-j is weighted by G_{ij,mu} = 1 + A tanh(a_{t_j} h_{j,mu})
+  bool gate_enabled = false;
+  int gate_mode = 0;                          // kGateLinearCombo or kGateScalarFull
+  double gate_amplitude = 0.8;                // A in [0, 1]
+  std::vector<double> gate_radial_coeffs;     // [M_r * rb] (shared (a, s) and type coeffs)
+  std::vector<double> gate_additive;          // [S] a_t
+  std::vector<int> gate_scalar_indices;       // [Qg] scalar index q -> alpha scalar
+  std::vector<double> gate_weights;           // linear combo [Qg]; scalar full [M_r][Qg]
+  std::vector<double> gate_body_mix;          // linear combo [M_r][K]: c_{mu, body}
+  int gate_body_count = 0;                    // body orders 2 .. k_max + 1
+  std::vector<int> gate_moment_index;         // [Qg] graph node of scalar q
+  std::vector<int> gate_body_id;              // [Qg] body order - 2
+
   // explicit metadata (sus2_sh.cu:1913-2093)
   bool standard_graph_matched = false;
   int layer_count = 0;
@@ -134,8 +152,11 @@ bool probe_model_file(const std::string& path);
 void prepare_graph_metadata(Model& model);
 
 // Radial tables for pair p = t_i S + t_j: coefficient c[p][mu][n] and (a, s)[p][mu].
+// `radial` replaces the model's radial_coeffs (the gate layer's own coefficients share the
+// (a, s) and type coefficients, sus2_sh.cu:2484-2517 with its override argument).
 void build_radial_tables(const Model& model, std::vector<double>& coeffs,
-                         std::vector<double>& scal_shift);
+                         std::vector<double>& scal_shift,
+                         const std::vector<double>* radial = nullptr);
 
 // True when the basics are exactly the standard full (l desc, k desc, m asc) layout.
 bool has_full_basic_layout(const Model& model);

@@ -56,7 +56,7 @@ void emit_streams(const std::vector<std::vector<Ins>>& items, int threads, bool 
 
 }  // namespace
 
-GraphProgram compile_graph(const Model& m, int chunk_terms, int threads)
+GraphProgram compile_graph(const Model& m, int chunk_terms, int threads, bool store_mapped)
 {
   const int A = m.moments_count;
   const int B = m.basic_count;
@@ -86,7 +86,7 @@ GraphProgram compile_graph(const Model& m, int chunk_terms, int threads)
   int slots = 0;
   for (int b = 0; b < B; ++b) g.node_slot[b] = slots++;
   for (int n = B; n < A; ++n)
-    if (live[n] && consumed[n]) g.node_slot[n] = slots++;
+    if (live[n] && (consumed[n] || (store_mapped && mapped[n]))) g.node_slot[n] = slots++;
   if (slots + 1 >= static_cast<int>(kNoTarget) - 1)
     throw InputError("SUS2_SH graph too large for the on-chip program.");
   g.slot_count = slots;

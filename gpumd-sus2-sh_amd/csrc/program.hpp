@@ -92,6 +92,10 @@ struct GraphProgram {
 constexpr uint32_t kNoTarget = 0xFFFFu;
 constexpr int kMaxSlots = 0xFFFF;
 
-GraphProgram compile_graph(const Model& model, int chunk_terms = 8, int threads = 256);
+// store_mapped: every mapped scalar node gets a slot (no terminal rows fused into the site
+// energy), so that a kernel can read scalar values and seed their adjoints per atom -- the
+// gate layer's program, compiled for a model whose scalars are the gate scalars.
+GraphProgram compile_graph(const Model& model, int chunk_terms = 8, int threads = 256,
+                           bool store_mapped = false);
 
 }  // namespace sus2sh

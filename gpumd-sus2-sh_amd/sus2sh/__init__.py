@@ -3,7 +3,7 @@
 The compute path is libsus2sh.so (HIP kernels for gfx950); this package only binds it.
 """
 from ._capi import EXPORTED_SYMBOLS, LIB_PATH, Box, Sus2shError, lib, last_error  # noqa: F401
-from .model_gen import (ModelSpec, StandardGraph, param_vector, standard_graph,  # noqa: F401
+from .model_gen import (ModelSpec, StandardGraph, gate_model, param_vector, standard_graph,  # noqa: F401
                         subgraph, synthetic_model, with_params, write_mtp)
 from .potential import (SUS2_SH, box_struct, compute_batch, evaluate,  # noqa: F401
                         maxwell_velocities, md_nve)

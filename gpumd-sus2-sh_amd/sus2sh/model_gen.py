@@ -121,6 +121,10 @@ class ModelSpec:
     min_dist: float = 1.0
     name: str = "sus2sh_synthetic"
     extra: dict = field(default_factory=dict)
+    # keys written after every other field, e.g. the two-layer gate block (its
+    # two_layer_gate_radial_coeffs key must come after the radial_coeffs section, which the
+    # loader finds by the first occurrence of "radial_coeffs")
+    tail: dict = field(default_factory=dict)
 
     @property
     def radial_funcs_count(self):
@@ -238,9 +242,63 @@ def write_mtp(spec: ModelSpec, path):
         "species_coeffs = " + _braced(spec.species_coeffs),
         "moment_coeffs = " + _braced(spec.moment_coeffs),
     ]
+    for key, val in spec.tail.items():
+        if isinstance(val, bool):
+            val = "true" if val else "false"
+        elif isinstance(val, (list, tuple, np.ndarray)):
+            val = _braced(list(val), _fmt if np.asarray(val).dtype.kind == "f" else str)
+        elif isinstance(val, float):
+            val = _fmt(val)
+        lines.append(f"{key} = {val}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return path
+
+
+def gate_model(spec: ModelSpec, mode="mu-scalar-full", seed=0, amplitude=0.8, coeff_scale=0.1,
+               scalars_per_body=None):
+    """Copy of `spec` with a two-layer mu gate (docs/mu-body-order-gate.md:25-63): seeded gate
+    radial coefficients N(0, coeff_scale), additive coefficients 1 + N(0, 0.2) per species,
+    gate scalars = the model's scalars of body orders 2 .. k_max + 1 (at most
+    `scalars_per_body` of each), weights N(0, coeff_scale) ([M_r][Qg] for mu-scalar-full,
+    [Qg] plus body-mix weights N(0, 1) [M_r][k_max] for mu-body-linear-combo)."""
+    import copy
+    g = spec.graph
+    rng = np.random.default_rng(seed)
+    K, Mr, S = g.k_max, spec.radial_funcs_count, spec.species_count
+    picks = []
+    for body in range(2, K + 2):
+        idx = np.flatnonzero(g.scalar_body == body)
+        if scalars_per_body is not None:
+            idx = idx[:scalars_per_body]
+        picks.extend(int(i) for i in idx)
+    Qg = len(picks)
+    out = copy.deepcopy(spec)
+    tail = {
+        "two_layer_gate_enabled": True,
+        "two_layer_gate_mode": mode,
+        "two_layer_gate_site_mode": "neighbor",
+        "two_layer_gate_radial_mode": "shared-radial",
+        "two_layer_gate_tanh_amplitude": float(amplitude),
+        "two_layer_gate_body_order_max": K + 1,
+        "two_layer_gate_radial_coeff_count": Mr * spec.rb_size,
+        "two_layer_gate_radial_coeffs": rng.normal(0.0, coeff_scale, size=Mr * spec.rb_size),
+        "two_layer_gate_additive_coeff_count": S,
+        "two_layer_gate_additive_coeffs": 1.0 + rng.normal(0.0, 0.2, size=S),
+    }
+    if mode == "mu-body-linear-combo":
+        tail["two_layer_gate_weight_count"] = Qg
+        tail["two_layer_gate_scalar_indices"] = np.array(picks, dtype=np.int64)
+        tail["two_layer_gate_weights"] = rng.normal(0.0, coeff_scale, size=Qg)
+        tail["two_layer_gate_body_mix_weight_count"] = Mr * K
+        tail["two_layer_gate_body_mix_weights"] = rng.normal(0.0, 1.0, size=Mr * K)
+    else:
+        tail["two_layer_gate_weight_count"] = Mr * Qg
+        tail["two_layer_gate_scalar_indices"] = np.array(picks, dtype=np.int64)
+        tail["two_layer_gate_weights"] = rng.normal(0.0, coeff_scale, size=Mr * Qg)
+    out.tail = tail
+    out.name = spec.name + "_gate"
+    return out
 
 
 def param_vector(spec: ModelSpec):

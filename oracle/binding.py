@@ -131,7 +131,7 @@ class Oracle:
             raise ValueError(f"oracle could not parse {mtp_path}")
         d = (ctypes.c_int * 12)()
         lib().oracle_dims(self._m, d)
-        keys = ("S", "L", "K", "body", "rb", "Mr", "A", "B", "Q", "T", "kind")
+        keys = ("S", "L", "K", "body", "rb", "Mr", "A", "B", "Q", "T", "kind", "gate_scalars")
         self.dims = dict(zip(keys, list(d)))
         self.rc = lib().oracle_cutoff(self._m)
         self.param_count = lib().oracle_param_count(self._m)
@@ -182,9 +182,12 @@ class Oracle:
         pg = np.zeros(self.param_count) if param_grad else None
         pb = np.asarray(pbc, dtype=np.int32)
         box = _box(cell)
-        if lib().oracle_compute(self._m, n, _ip(t), _dp(pos), _dp(box), _ip(pb),
-                                1 if brute else 0, _dp(e), _dp(f), _dp(v),
-                                _dp(pg) if pg is not None else None) != 0:
+        rc = lib().oracle_compute(self._m, n, _ip(t), _dp(pos), _dp(box), _ip(pb),
+                                  1 if brute else 0, _dp(e), _dp(f), _dp(v),
+                                  _dp(pg) if pg is not None else None)
+        if rc == -2:
+            raise ValueError("oracle_compute: gate models have no dE/dtheta or ZBL here")
+        if rc != 0:
             raise ValueError("oracle_compute: ZBL model without set_zbl()")
         out = (e, f.reshape(3, n), v.reshape(9, n))
         return out + (pg,) if param_grad else out

@@ -14,6 +14,12 @@
  *   edge derivative, forces, virial  src/force/sus2_sh.cu:5996-6035, 6347-6446
  *   multi-image neighbours           src/force/sus2_sh.cu:7657-7767, 9547-9618
  *   parameter gradient dE/dtheta     no reference counterpart (adjoint of the above)
+ *   two-layer mu gate                docs/mu-body-order-gate.md:25-63 with the loader fields
+ *                                    of src/force/sus2_sh.cu:2153-2266, 2412-2478 and the
+ *                                    per-step order of :9821-10256 (gate moments, gate graph,
+ *                                    h, 1 + A tanh(a h) per neighbour and channel, gated
+ *                                    moments, main graph, forces with the gate adjoint,
+ *                                    reverse through the gate graph, gate-layer forces)
  *   ZBL pair term                    src/force/sus2_zbl_common.cuh:171-225 applied per edge as
  *                                    in src/force/sus2_sh.cu:7522-7627; the universal screened
  *                                    Coulomb form of Ziegler, Biersack and Littmark (1985)
@@ -48,6 +54,12 @@ typedef struct {
   int zbl, zbl_ready;
   int zbl_z[16];
   double zbl_inner[256], zbl_outer[256];
+  /* two-layer mu gate: h_{a,mu} = sum_q weff[mu][q] s_q(a) with weff = the mu-scalar-full
+   * weights, or c_{mu,body(q)} w_q for mu-body-linear-combo; G_{ij,mu} = 1 + A tanh(a_tj h) */
+  int gate, gate_mode, gate_Qg;
+  double gate_A;
+  double *gate_radial, *gate_add, *gate_weff; /* [Mr rb], [S], [Mr Qg] */
+  int* gate_node;                             /* [Qg] graph node of gate scalar q */
 } oracle_model;
 
 /* ------------------------------------------------------------------------ */
@@ -185,7 +197,69 @@ void oracle_free(oracle_model* m)
   free(m->p_right);
   free(m->p_target);
   free(m->p_coeff);
+  free(m->gate_radial);
+  free(m->gate_add);
+  free(m->gate_weff);
+  free(m->gate_node);
   free(m);
+}
+
+/* Gate fields of a two_layer_gate_enabled model (only the accepted form: mu-body-linear-combo
+ * or mu-scalar-full, neighbour site, shared radial scaling; the engine's loader enforces the
+ * reference's rejections).  Returns 0 on a parse problem. */
+static int or_load_gate(const char* text, oracle_model* m)
+{
+  int ok = 1;
+  char word[128];
+  or_word(text, "two_layer_gate_mode =", word, sizeof word, &ok);
+  if (!ok) return 0;
+  if (strcmp(word, "mu-body-linear-combo") == 0)
+    m->gate_mode = 1;
+  else if (strcmp(word, "mu-scalar-full") == 0 || strcmp(word, "mu-body-order") == 0)
+    m->gate_mode = 2;
+  else
+    return 0;
+  m->gate_A = strstr(text, "two_layer_gate_tanh_amplitude =")
+                ? or_double(text, "two_layer_gate_tanh_amplitude =", &ok)
+                : 0.8;
+  const int Mr = m->Mr, S = m->S, K = m->K;
+  m->gate_radial = or_array(text, "two_layer_gate_radial_coeffs =", Mr * m->rb, &ok);
+  if (strstr(text, "two_layer_gate_additive_coeffs ="))
+    m->gate_add = or_array(text, "two_layer_gate_additive_coeffs =", S, &ok);
+  else {
+    m->gate_add = (double*)calloc((size_t)S, sizeof(double));
+    for (int t = 0; t < S; ++t) m->gate_add[t] = 1.0;
+  }
+  const int Qg = or_int(text, "two_layer_gate_weight_count =", &ok) / (m->gate_mode == 2 ? Mr : 1);
+  if (!ok || Qg <= 0) return 0;
+  m->gate_Qg = Qg;
+  double* idx = or_array(text, "two_layer_gate_scalar_indices =", Qg, &ok);
+  double* w = or_array(text, "two_layer_gate_weights =", m->gate_mode == 2 ? Mr * Qg : Qg, &ok);
+  double* info = or_array(text, "sh_scalar_info =", m->Q, &ok);
+  double* mix = m->gate_mode == 1 ? or_array(text, "two_layer_gate_body_mix_weights =", Mr * K, &ok)
+                                  : NULL;
+  m->gate_node = (int*)calloc((size_t)Qg, sizeof(int));
+  m->gate_weff = (double*)calloc((size_t)(Mr * Qg), sizeof(double));
+  for (int q = 0; ok && q < Qg; ++q) {
+    const int s = (int)idx[q];
+    if (s < 0 || s >= m->Q) {
+      ok = 0;
+      break;
+    }
+    m->gate_node[q] = m->mapping[s];
+    const int body = (int)info[s] - 2;
+    if (body < 0 || body >= K) {
+      ok = 0;
+      break;
+    }
+    for (int mu = 0; mu < Mr; ++mu)
+      m->gate_weff[mu * Qg + q] = m->gate_mode == 2 ? w[mu * Qg + q] : mix[mu * K + body] * w[q];
+  }
+  free(idx);
+  free(w);
+  free(info);
+  free(mix);
+  return ok;
 }
 
 /* Returns NULL on any parse problem (the oracle only reads files it is given by tests). */
@@ -296,6 +370,16 @@ oracle_model* oracle_load(const char* path)
     ++row;
     p = e + 1;
   }
+  {
+    char gw[32];
+    int gok = 1;
+    gw[0] = 0;
+    if (strstr(text, "two_layer_gate_enabled ="))
+      or_word(text, "two_layer_gate_enabled =", gw, sizeof gw, &gok);
+    m->gate = gok && (strcmp(gw, "1") == 0 || strcmp(gw, "true") == 0 || strcmp(gw, "TRUE") == 0 ||
+                      strcmp(gw, "yes") == 0 || strcmp(gw, "on") == 0);
+    if (ok && m->gate && !or_load_gate(text, m)) ok = 0;
+  }
   free(text);
   if (!ok) {
     oracle_free(m);
@@ -317,7 +401,7 @@ void oracle_dims(const oracle_model* m, int* out /* 12 */)
   out[8] = m->Q;
   out[9] = m->T;
   out[10] = m->kind;
-  out[11] = 0;
+  out[11] = m->gate ? m->gate_Qg : 0;
 }
 
 double oracle_cutoff(const oracle_model* m) { return m->rc; }
@@ -865,6 +949,179 @@ int oracle_set_zbl(oracle_model* m, const int* z, const double* inner, const dou
 
 int oracle_zbl_enabled(const oracle_model* m) { return m->zbl; }
 
+/* Basic moments of atom i with the radial coefficients of `mr` (the model, or its copy with
+ * the gate radial coefficients), each edge's radial channel mu weighted by mult[j][mu] when
+ * mult is given (src/force/sus2_sh.cu:3486-3550, gated form :4246-4310). */
+static void or_basics(const oracle_model* mr, const or_nlist* nl, const int* type, int i,
+                      const double* mult, double* M)
+{
+  const int L = mr->L, Mr = mr->Mr;
+  double R[64], Y[OR_MAX_COMP];
+  for (int b = 0; b < mr->B; ++b) M[b] = 0.0;
+  for (int e = nl->start[i]; e < nl->start[i + 1]; ++e) {
+    const int j = nl->j[e];
+    const double dx = nl->d[3 * e], dy = nl->d[3 * e + 1], dz = nl->d[3 * e + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    oracle_radial(mr, type[i], type[j], r, R, NULL);
+    if (mult)
+      for (int mu = 0; mu < Mr; ++mu) R[mu] *= mult[(size_t)j * Mr + mu];
+    oracle_ylm(L, dx, dy, dz, Y, NULL);
+    for (int b = 0; b < mr->B; ++b) {
+      const int mu = mr->basic_k[b] * (L + 1) + mr->basic_l[b];
+      const int y = mr->basic_l[b] * mr->basic_l[b] + mr->basic_m[b] + mr->basic_l[b];
+      M[b] += R[mu] * Y[y];
+    }
+  }
+}
+
+/* Forces and virial of the edges of atom i for basic adjoints G (src/force/sus2_sh.cu:
+ * 5996-6035, 6347-6446): g_ij = sum_b G_b m_j,mu (R' u Y + R grad Y); F_i += g, F_j -= g,
+ * W_i -= r_ij (x) g.  With mult (gated main moments) the gate adjoint
+ * adj[j][mu] += dmult[j][mu] sum_b G_b R_mu Y_y is accumulated too (:6077-6123). */
+static void or_edge_forces(const oracle_model* mr, const or_nlist* nl, const int* type, int i,
+                           const double* G, const double* mult, const double* dmult,
+                           double* adj, double* f, int N, double* vir)
+{
+  const int L = mr->L, Mr = mr->Mr;
+  double R[64], dR[64], Y[OR_MAX_COMP], dY[3 * OR_MAX_COMP];
+  double fs[3] = {0, 0, 0};
+  for (int e = nl->start[i]; e < nl->start[i + 1]; ++e) {
+    const int j = nl->j[e];
+    const double dx = nl->d[3 * e], dy = nl->d[3 * e + 1], dz = nl->d[3 * e + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double u[3] = {dx / r, dy / r, dz / r};
+    oracle_radial(mr, type[i], type[j], r, R, dR);
+    oracle_ylm(L, dx, dy, dz, Y, dY);
+    double g[3] = {0, 0, 0};
+    for (int b = 0; b < mr->B; ++b) {
+      const int mu = mr->basic_k[b] * (L + 1) + mr->basic_l[b];
+      const int y = mr->basic_l[b] * mr->basic_l[b] + mr->basic_m[b] + mr->basic_l[b];
+      const double w = mult ? mult[(size_t)j * Mr + mu] : 1.0;
+      for (int c = 0; c < 3; ++c) g[c] += G[b] * w * (dR[mu] * u[c] * Y[y] + R[mu] * dY[3 * y + c]);
+      if (adj) adj[(size_t)j * Mr + mu] += G[b] * R[mu] * Y[y] * dmult[(size_t)j * Mr + mu];
+    }
+    for (int c = 0; c < 3; ++c) {
+      fs[c] += g[c];
+      f[(size_t)c * N + j] -= g[c];
+    }
+    vir[0] -= dx * g[0];
+    vir[1] -= dy * g[1];
+    vir[2] -= dz * g[2];
+    vir[3] -= dx * g[1];
+    vir[4] -= dx * g[2];
+    vir[5] -= dy * g[2];
+    vir[6] -= dy * g[0];
+    vir[7] -= dz * g[0];
+    vir[8] -= dz * g[1];
+  }
+  for (int c = 0; c < 3; ++c) f[(size_t)c * N + i] += fs[c];
+}
+
+/* Two-layer mu gate evaluation (FP64), in the reference's per-step order
+ * (src/force/sus2_sh.cu:9821-10256):
+ *   1. gate moments B^g_a (gate radial coefficients), gate graph forward, gate scalars s_q;
+ *      h_{a,mu} = sum_q weff[mu][q] s_q(a); mult = 1 + A tanh(a_ta h), dmult = A a (1 - tanh^2)
+ *      (:4565-4664)
+ *   2. main moments B_i = sum_j mult_j,mu R_mu Y, main graph, E_i, reverse -> G_i (sigma-scaled)
+ *   3. forces / virial of the main moments and the gate adjoints adj_{j,mu}
+ *   4. seeds dE/ds_q(j) = sum_mu adj_{j,mu} weff[mu][q] (:4666-4790), reverse through the
+ *      gate graph -> gate basic adjoints, gate-layer forces / virial (:6695-6808). */
+static int or_compute_gate(const oracle_model* m, int N, const int* type, const or_nlist* nl,
+                           double* energy, double* force, double* virial)
+{
+  const int A = m->A, B = m->B, Mr = m->Mr, Qg = m->gate_Qg;
+  oracle_model mg = *m; /* same model with the gate radial coefficients */
+  mg.radial = m->gate_radial;
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  double* mult = (double*)calloc((size_t)N * Mr, sizeof(double));
+  double* dmult = (double*)calloc((size_t)N * Mr, sizeof(double));
+  double* fbuf = (double*)calloc((size_t)nthreads * 3 * N, sizeof(double));
+  double* abuf = (double*)calloc((size_t)nthreads * N * Mr, sizeof(double));
+  double* adj = (double*)calloc((size_t)N * Mr, sizeof(double));
+  if (!mult || !dmult || !fbuf || !abuf || !adj) {
+    free(mult), free(dmult), free(fbuf), free(abuf), free(adj);
+    return -1;
+  }
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* M = (double*)calloc((size_t)A, sizeof(double));
+    double* G = (double*)calloc((size_t)A, sizeof(double));
+    double* f = fbuf + (size_t)tid * 3 * N;
+    double* ad = abuf + (size_t)tid * N * Mr;
+    /* 1. gate values */
+#pragma omp for schedule(dynamic, 32)
+    for (int a = 0; a < N; ++a) {
+      for (int k = B; k < A; ++k) M[k] = 0.0;
+      or_basics(&mg, nl, type, a, NULL, M);
+      or_graph_forward(m, M);
+      for (int mu = 0; mu < Mr; ++mu) {
+        double h = 0.0;
+        for (int q = 0; q < Qg; ++q) h += m->gate_weff[mu * Qg + q] * M[m->gate_node[q]];
+        const double ac = m->gate_add[type[a]];
+        const double t = tanh(ac * h);
+        mult[(size_t)a * Mr + mu] = 1.0 + m->gate_A * t;
+        dmult[(size_t)a * Mr + mu] = m->gate_A * ac * (1.0 - t * t);
+      }
+    }
+    /* 2-3. gated main moments, graph, energy, forces and gate adjoints */
+#pragma omp for schedule(dynamic, 32)
+    for (int i = 0; i < N; ++i) {
+      const int ti = type[i];
+      for (int k = B; k < A; ++k) M[k] = 0.0;
+      or_basics(m, nl, type, i, mult, M);
+      const double sigma = m->species[ti];
+      energy[i] = m->shift[ti] + sigma + sigma * or_graph_forward(m, M);
+      or_graph_reverse(m, sigma, M, G);
+      double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      or_edge_forces(m, nl, type, i, G, mult, dmult, ad, f, N, vir);
+      for (int c = 0; c < 9; ++c) virial[(size_t)c * N + i] = vir[c];
+    }
+#pragma omp for schedule(static)
+    for (size_t k = 0; k < (size_t)N * Mr; ++k) {
+      double v = 0.0;
+      for (int t = 0; t < nthreads; ++t) v += abuf[(size_t)t * N * Mr + k];
+      adj[k] = v;
+    }
+    /* 4. reverse through the gate graph from the per-atom seeds, gate-layer forces */
+#pragma omp for schedule(dynamic, 32)
+    for (int a = 0; a < N; ++a) {
+      for (int k = B; k < A; ++k) M[k] = 0.0;
+      or_basics(&mg, nl, type, a, NULL, M);
+      or_graph_forward(m, M);
+      for (int k = 0; k < A; ++k) G[k] = 0.0;
+      for (int q = 0; q < Qg; ++q) {
+        double seed = 0.0;
+        for (int mu = 0; mu < Mr; ++mu) seed += adj[(size_t)a * Mr + mu] * m->gate_weff[mu * Qg + q];
+        G[m->gate_node[q]] += seed;
+      }
+      for (int p = m->T - 1; p >= 0; --p) {
+        const double g = G[m->p_target[p]] * m->p_coeff[p];
+        G[m->p_left[p]] += g * M[m->p_right[p]];
+        G[m->p_right[p]] += g * M[m->p_left[p]];
+      }
+      double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      or_edge_forces(&mg, nl, type, a, G, NULL, NULL, NULL, f, N, vir);
+      for (int c = 0; c < 9; ++c) virial[(size_t)c * N + a] += vir[c];
+    }
+    free(M);
+    free(G);
+  }
+  for (size_t k = 0; k < (size_t)3 * N; ++k) {
+    double v = 0.0;
+    for (int t = 0; t < nthreads; ++t) v += fbuf[(size_t)t * 3 * N + k];
+    force[k] = v;
+  }
+  free(mult), free(dmult), free(fbuf), free(abuf), free(adj);
+  return 0;
+}
+
 int oracle_compute(const oracle_model* m, int N, const int* type, const double* pos,
                    const double* h, const int* pbc, int mode, double* energy, double* force,
                    double* virial, double* dparam)
@@ -876,6 +1133,12 @@ int oracle_compute(const oracle_model* m, int N, const int* type, const double* 
     or_neighbors_brute(N, pos, h, pbc, m->rc, &nl);
   else
     or_neighbors_cells(N, pos, h, pbc, m->rc, &nl);
+  if (m->gate) {
+    /* gate models: energy, forces and virial only (no dE/dtheta, no ZBL) */
+    const int rc = (dparam || m->zbl) ? -2 : or_compute_gate(m, N, type, &nl, energy, force, virial);
+    or_free_nlist(&nl);
+    return rc;
+  }
 
   const int L = m->L, B = m->B, A = m->A, Mr = m->Mr, rb = m->rb, S = m->S;
   int nthreads = 1;

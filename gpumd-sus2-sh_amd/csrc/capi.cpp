@@ -230,6 +230,7 @@ int sus2sh_get_model_info(const sus2sh_handle* handle, sus2sh_model_info* out)
   out->lane_instructions = lp.ok ? static_cast<int>(lp.ins.size()) : 0;
   out->lane_waves = lp.ok ? lp.waves : 0;
   out->lane_nslot = lp.ok ? lp.nslot : 0;
+  out->lane_atoms = lp.ok ? lp.atoms : 0;
   return SUS2SH_OK;
 }
 
@@ -332,13 +333,14 @@ int sus2sh_graph_export(const sus2sh_graph* graph, int* basic_klm, int* product_
 void sus2sh_graph_free(sus2sh_graph* graph) { delete graph; }
 
 int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot, int real_bytes,
-                      int n, const double* basics, double* e_prime, double* grad, int* info)
+                      int atoms, int n, const double* basics, double* e_prime, double* grad,
+                      int* info)
 {
   if (nslot == 0) nslot = waves == 8 ? 2 : 4;
   if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
       (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
       (nslot != 1 && nslot != 2 && nslot != 4 && nslot != 8) ||
-      (real_bytes != 0 && real_bytes != 4 && real_bytes != 8))
+      (real_bytes != 0 && real_bytes != 4 && real_bytes != 8) || (atoms != 1 && atoms != 2))
     return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);
@@ -359,7 +361,7 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot,
                                                    grad + static_cast<size_t>(i) * B);
       return;
     }
-    const sus2sh::LaneRecords rec = sus2sh::encode_lane_records(p, real_bytes);
+    const sus2sh::LaneRecords rec = sus2sh::encode_lane_records(p, real_bytes, 64 * atoms);
     for (int i = 0; i < n; ++i)
       e_prime[i] = sus2sh::run_lane_records_host(p, rec, basics + static_cast<size_t>(i) * B,
                                                  grad + static_cast<size_t>(i) * B);

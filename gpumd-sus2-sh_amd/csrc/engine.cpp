@@ -63,16 +63,20 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
                key == "sh_factor_pruning") {
       pruning = val;
     } else if (key == "sus2_sh_lane_shape") {
-      // waves x accumulators x minimum slot budget, e.g. 8x2x208
-      int w = 0, a = 0, s = 0;
-      char x1 = 0, x2 = 0;
+      // waves x accumulators x minimum slot budget [x atoms per lane], e.g. 8x2x208 or
+      // 8x2x160x2
+      int w = 0, a = 0, s = 0, na = 1;
+      char x1 = 0, x2 = 0, x3 = 0;
       std::istringstream in(val);
       in >> w >> x1 >> a >> x2 >> s;
-      if (!in || x1 != 'x' || x2 != 'x' || !in.eof())
-        throw InputError("sus2_sh_lane_shape must be <waves>x<accumulators>x<slots>: " + val);
+      if (in && !in.eof()) in >> x3 >> na;
+      if (!in || x1 != 'x' || x2 != 'x' || (x3 != 0 && x3 != 'x') || !in.eof())
+        throw InputError("sus2_sh_lane_shape must be <waves>x<accumulators>x<slots>[x<atoms>]: " +
+                         val);
       o.lane_waves = w;
       o.lane_nslot = a;
       o.lane_min_slots = s;
+      o.lane_atoms = na;
     } else if (key == "sus2_float" || key == "sus2_sh_float") {
       o.use_float = parse_bool(val, key);
     } else if (key == "sus2_sh_profile") {
@@ -92,6 +96,7 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.lane_waves = w;
       o.lane_nslot = w == 8 ? 2 : 4;
       o.lane_min_slots = 160;
+      o.lane_atoms = 1;
     } else if (key == "sus2_sh_deferred_check") {
       o.deferred_check = parse_bool(val, key);
     } else if (key == "sus2_sh_md_graph") {
@@ -110,7 +115,7 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
   if (!radial_direct)
     throw InputError("SUS2_SH first backend currently requires direct radial evaluation.");
   if (o.lane_nslot > 0) {
-    if (!lane_shape_supported(o.lane_waves, o.lane_nslot))
+    if (!lane_shape_supported(o.lane_waves, o.lane_nslot, o.lane_atoms))
       throw InputError("sus2_sh_lane_shape: no lane kernel for this waves x accumulators pair");
     if (o.lane_min_slots < 1 || o.lane_min_slots > kLaneMaxSlot + 1)
       throw InputError("sus2_sh_lane_shape: slot budget out of range");
@@ -256,8 +261,9 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
   // 208 FP32 slots: three 64-atom tiles (52 KiB each) still share one CU's 160 KiB
   for (int max_slots : {160, 192, 208, 256, 320, 384, 512, 640, 1024}) {
     if (max_slots < shape.min_slots) continue;
-    if (static_cast<size_t>(max_slots) * 64 * rsz > max_lds_) break;
+    if (static_cast<size_t>(max_slots) * 64 * shape.atoms * rsz > max_lds_) break;
     LaneProgram p = compile_lane_program(model_, max_slots, shape.nslot, shape.waves);
+    p.atoms = shape.atoms;
     if (p.ok) return p;
     best.reason = p.reason;
     if (p.reason.find("slot budget") == std::string::npos) break;  // not a budget limit
@@ -265,19 +271,23 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
   return best;
 }
 
-// Shapes the autotuner tries (the first feasible one is the default): waves per 64-atom
-// tile x accumulators per wave x minimum slot budget.
+// Shapes the autotuner tries (the first feasible one is the default): waves per tile x
+// accumulators per wave x minimum slot budget x atoms per lane.  Two atoms per lane share
+// each LDS address and each program record (half the record traffic per atom) at twice the
+// LDS per tile.
 const std::vector<Engine::LaneShape>& Engine::lane_shapes()
 {
   static const std::vector<LaneShape> shapes = {
-    {4, 4, 160}, {8, 2, 160}, {4, 2, 160}, {8, 2, 192}, {8, 2, 256}, {2, 4, 160}, {8, 2, 208}};
+    {4, 4, 160, 1}, {8, 2, 160, 1}, {4, 2, 160, 1}, {8, 2, 192, 1}, {8, 2, 256, 1},
+    {2, 4, 160, 1}, {8, 2, 208, 1}, {8, 2, 160, 2}, {4, 2, 160, 2}, {8, 2, 208, 2}};
   return shapes;
 }
 
 void Engine::compile_lane()
 {
   if (opt_.lane_nslot > 0) {
-    lane_ = build_lane(LaneShape{opt_.lane_waves, opt_.lane_nslot, opt_.lane_min_slots});
+    lane_ = build_lane(
+      LaneShape{opt_.lane_waves, opt_.lane_nslot, opt_.lane_min_slots, opt_.lane_atoms});
     return;
   }
   lane_ = LaneProgram{};
@@ -290,7 +300,8 @@ void Engine::compile_lane()
 void Engine::upload_lane()
 {
   const LaneRecords rec =
-    encode_lane_records(lane_, opt_.use_float ? static_cast<int>(sizeof(float)) : static_cast<int>(sizeof(double)));
+    encode_lane_records(lane_, opt_.use_float ? static_cast<int>(sizeof(float)) : static_cast<int>(sizeof(double)),
+                        64 * lane_.atoms);
   const size_t nseg = lane_.seg_kind.size();
   std::vector<int4> segs(nseg);
   for (size_t k = 0; k < nseg; ++k)
@@ -716,6 +727,7 @@ bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable
     lt.slots = lane_.slots;
     lt.nslot = lane_.nslot;
     lt.waves = lane_.waves;
+    lt.atoms = lane_.atoms;
     return launch_lane_graph<Real>(N, static_cast<const Real*>(ws_.basics), ws_.type_s, lt,
                                    ws_.e_s, static_cast<Real*>(ws_.grads), s, max_lds_);
   }
@@ -1042,7 +1054,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     if (best_variant < 3) tile_ = best_tile;
     variant_ = best_variant;
     if (best_lane.ok && (best_lane.waves != lane_.waves || best_lane.nslot != lane_.nslot ||
-                         best_lane.slots != lane_.slots)) {
+                         best_lane.slots != lane_.slots || best_lane.atoms != lane_.atoms)) {
       lane_ = std::move(best_lane);
       upload_lane();
     }

@@ -65,6 +65,7 @@ struct EngineOptions {
   int lane_waves = 0;    // lane kernel: waves per 64-atom tile (0: autotuned shape)
   int lane_nslot = 0;    // lane kernel: accumulators per wave (0: 2 with 8 waves, else 4)
   int lane_min_slots = 160;  // lane kernel: smallest per-lane slot budget tried
+  int lane_atoms = 1;        // lane kernel: atoms per lane
   // no host synchronisation in steady calls (capturable); the neighbour-capacity and type
   // checks are latched on the device and reported by check()
   bool deferred_check = false;
@@ -145,9 +146,10 @@ class Engine {
   template <typename Real>
   void upload_gate_tables();
   struct LaneShape {
-    int waves;      // waves per 64-atom tile
+    int waves;      // waves per tile
     int nslot;      // accumulators per wave
     int min_slots;  // smallest per-lane slot budget tried
+    int atoms;      // atoms per lane
   };
   static const std::vector<LaneShape>& lane_shapes();
   LaneProgram build_lane(const LaneShape& shape) const;

@@ -213,13 +213,15 @@ struct LaneTable {
   int v_base;              // first component-value slot (reused for the per-wave energies)
   int slots;               // per-lane slots: LDS of a workgroup = slots * 64 * sizeof(Real)
   int nslot;               // accumulators per wave (instructions per group)
-  int waves;               // waves sharing one 64-atom tile
+  int waves;               // waves sharing one tile
+  int atoms;               // atoms per lane (the tile holds 64 atoms x atoms)
 };
 
 // Lane graph kernel (lane_kernel.hip): one 64-atom tile per workgroup of lt.waves waves;
 // returns false when the tile's LDS would exceed max_lds or the program shape is unknown.
-// Whether a lane kernel is compiled for `waves` waves per tile x `nslot` accumulators.
-bool lane_shape_supported(int waves, int nslot);
+// Whether a lane kernel is compiled for `waves` waves per tile x `nslot` accumulators x
+// `atoms` atoms per lane.
+bool lane_shape_supported(int waves, int nslot, int atoms);
 template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);

@@ -540,13 +540,14 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
   return e;
 }
 
-LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes)
+LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes, int col_words)
 {
   LaneRecords r;
   r.nslot = p.nslot;
   r.real_bytes = real_bytes;
+  r.col = col_words;
   const int ns = p.nslot, hw = r.header_words(), rw = r.rec_words();
-  const uint32_t col = 64u;  // words per slot row: slot s of lane i is word s * 64 + i
+  const uint32_t col = static_cast<uint32_t>(col_words);  // words per slot row
   size_t nrec = 0;
   for (size_t s = 0; s < p.seg_kind.size(); ++s) nrec += static_cast<size_t>(p.seg_groups[s]) * p.waves;
   r.words.assign(nrec * rw, 0u);
@@ -588,7 +589,7 @@ double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const d
 {
   const int B = p.basic_count, ns = p.nslot, hw = r.header_words(), rw = r.rec_words();
   const int rs = r.real_bytes / 4;
-  const uint32_t col = 64u;
+  const uint32_t col = static_cast<uint32_t>(r.col);
   auto get_real = [&](const uint32_t* at) {
     if (r.real_bytes == 4) {
       float f;

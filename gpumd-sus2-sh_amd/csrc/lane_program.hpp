@@ -44,6 +44,7 @@ struct LaneProgram {
   int basic_count = 0;
   int nslot = 8;
   int waves = 1;  // waves that share one 64-atom tile and split every segment's rows
+  int atoms = 1;  // atoms per lane of the kernel that runs it (its LDS column width / 64)
   int cap_v = 0, cap_g = 0;  // value / adjoint region sizes
   int slots = 0;  // per-lane slots (LDS = slots * 64 * sizeof(Real) per wave)
   int ones_slot = 0, trash_slot = 0, v_base = 0, g_base = 0;
@@ -86,7 +87,8 @@ double run_lane_program_host(const LaneProgram& p, const double* basics, double*
 // Device encoding of a lane program: one record of rec_words() 32-bit words per group of
 // nslot instructions, wave-major inside a segment (record (w * groups + g) of the segment),
 // every word used so that a batch of records is a few wide scalar loads:
-//   words [0, nslot)          x word offsets in the lane column layout (slot * 64)
+//   words [0, nslot)          x word offsets in the lane column layout (slot * col, col =
+//                             64 x atoms per lane)
 //   words [nslot, 2 nslot)    y word offsets
 //   words [2 nslot, 3 nslot)  target word offsets: where an accumulator goes at its row's
 //                             end (stored by store segments, added by add segments; energy
@@ -106,12 +108,13 @@ constexpr int lane_record_words(int ns, int real_bytes)
 
 struct LaneRecords {
   int nslot = 0, real_bytes = 0;
+  int col = 64;  // LDS words per slot row (64 x atoms per lane)
   int header_words() const { return lane_header_words(nslot, real_bytes); }
   int rec_words() const { return lane_record_words(nslot, real_bytes); }
   std::vector<uint32_t> words;
   std::vector<int> seg_first;  // first record of each segment
 };
-LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes);
+LaneRecords encode_lane_records(const LaneProgram& p, int real_bytes, int col = 64);
 
 // Host execution of the device encoding (same arithmetic order as the kernel, in double).
 double run_lane_records_host(const LaneProgram& p, const LaneRecords& r, const double* basics,

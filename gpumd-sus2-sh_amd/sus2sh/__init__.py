@@ -37,18 +37,19 @@ def model_check(path):
     return dict(zip(keys, list(dims)))
 
 
-def lane_check(path, basics=None, max_slots=160, waves=1, nslot=0, real_bytes=0):
+def lane_check(path, basics=None, max_slots=160, waves=1, nslot=0, real_bytes=0, atoms=1):
     """Host-only run of the lane program (the one-atom-per-lane graph kernel's instruction
     stream) for basic vectors basics [n, B], compiled for `waves` waves per tile and `nslot`
     accumulators per wave (0: 2 with 8 waves, else 4).  real_bytes 0 runs the compiled
-    program, 4 / 8 its FP32 / FP64 device encoding.  Returns (info dict, e_prime [n],
+    program, 4 / 8 its FP32 / FP64 device encoding for `atoms` atoms per lane.  Returns (info dict, e_prime [n],
     grad [n, B]); e_prime / grad are None when the model has no lane program."""
     import ctypes
     import numpy as np
     from ._capi import check
     P = ctypes.POINTER(ctypes.c_double)
     info = (ctypes.c_int * 6)()
-    args = (str(path).encode(), int(max_slots), int(waves), int(nslot), int(real_bytes))
+    args = (str(path).encode(), int(max_slots), int(waves), int(nslot), int(real_bytes),
+            int(atoms))
     if basics is None:
         check(lib().sus2sh_lane_check(*args, 0, None, None, None, info))
         e = g = None

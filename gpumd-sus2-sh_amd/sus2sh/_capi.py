@@ -87,6 +87,7 @@ class ModelInfo(ctypes.Structure):
         ("lane_instructions", ctypes.c_int),
         ("lane_waves", ctypes.c_int),
         ("lane_nslot", ctypes.c_int),
+        ("lane_atoms", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -173,8 +174,8 @@ def lib():
     L.sus2sh_graph_free.restype = None
     L.sus2sh_model_check.argtypes = [c_char_p, P(c_int)]
     L.sus2sh_model_check.restype = c_int
-    L.sus2sh_lane_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, c_int, P(c_double),
-                                    P(c_double), P(c_double), P(c_int)]
+    L.sus2sh_lane_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                    P(c_double), P(c_double), P(c_double), P(c_int)]
     L.sus2sh_lane_check.restype = c_int
     L.sus2sh_zbl_tables.argtypes = [c_char_p, c_int, P(c_char_p), P(c_int), P(c_double),
                                     P(c_double)]

@@ -62,6 +62,7 @@ typedef struct sus2sh_model_info {
   int lane_instructions;  /* lane-program instructions per atom (including padding) */
   int lane_waves;         /* lane-program waves per 64-atom tile */
   int lane_nslot;         /* lane-program accumulators per wave */
+  int lane_atoms;         /* lane-program atoms per lane */
 } sus2sh_model_info;
 
 /* Model identity: first token "MTP" and the substring "potential_tag = SUS2-SH".
@@ -77,8 +78,8 @@ int sus2sh_probe(const char* path);
  * Honoured options (like the reference, only key=value tokens are options; bare tokens are
  * ignored): sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
  * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8>,
- * sus2_sh_lane_shape=<waves>x<accumulators>x<min slots> (e.g. 8x2x208: fixes the lane
- * kernel's full shape), sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
+ * sus2_sh_lane_shape=<waves>x<accumulators>x<min slots>[x<atoms per lane>] (e.g. 8x2x208 or
+ * 8x2x160x2: fixes the lane kernel's full shape), sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
  * sus2_sh_deferred_check=<0|1> (steady calls make no host synchronisation and can be
  * captured in a HIP graph; see sus2sh_check), sus2_sh_md_graph=<0|1> (sus2sh_md_nve replays
  * captured steps, default 1).  Validated like the reference: sus2_sh_factor_pruning=
@@ -193,7 +194,8 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
  * ms[v * 5 + log2(tile)] for variant v = 0..2 and tile 1..16, and ms[15 + c] for lane-program
  * (variant 3) shape c = 0..6: (waves per 64-atom tile, accumulators per wave, minimum
  * per-lane slots) = (4,4,160), (8,2,160), (4,2,160), (8,2,192), (8,2,256), (2,4,160),
- * (8,2,208).  When the lane shape is fixed (sus2_sh_lane_waves / sus2_sh_lane_shape) only
+ * (8,2,208), then with two atoms per lane (8,2,160), (4,2,160), (8,2,208) at c = 7..9.  When
+ * the lane shape is fixed (sus2_sh_lane_waves / sus2_sh_lane_shape) only
  * that shape is timed and its time is in ms[15].  Returns the count written. */
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
@@ -245,11 +247,12 @@ int sus2sh_model_check(const char* mtp_path, int* dims);
  * basics[n][B] -> e_prime[n] = sum_s w_s M[map[s]] and grad[n][B] = de_prime/dbasic (without
  * the species factor).  real_bytes = 0 runs the compiled program; 4 or 8 runs its device
  * encoding for FP32 / FP64 kernels (the records the kernel reads, coefficients rounded to
- * that precision).  info[0..6) = ok, slots, instructions per atom, graph components,
+ * that precision) laid out for `atoms` (1 or 2) atoms per lane.  info[0..6) = ok, slots, instructions per atom, graph components,
  * batches, largest component.  When the model has no lane program (info[0] = 0) nothing is
  * evaluated.  Returns a status code. */
 int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot, int real_bytes,
-                      int n, const double* basics, double* e_prime, double* grad, int* info);
+                      int atoms, int n, const double* basics, double* e_prime, double* grad,
+                      int* info);
 
 #ifdef __cplusplus
 }

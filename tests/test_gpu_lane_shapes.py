@@ -1,8 +1,8 @@
 """GPU parity of every lane-kernel shape on the benchmark model (through the C ABI).
 
 The FP32 benchmark runs the shape the autotuner picks for the l3333 graph (body order 5,
-sh_body_l_max = {3,3,3,3}: 1349 moments, 5475 products, 572 scalars) among the seven lane
-shapes (waves per 64-atom tile, accumulators per wave, slot budget).  Each shape is pinned
+sh_body_l_max = {3,3,3,3}: 1349 moments, 5475 products, 572 scalars) among the ten lane
+shapes (waves per tile, accumulators per wave, slot budget, atoms per lane).  Each shape is pinned
 with sus2_sh_lane_shape and compared with the FP64 oracle on a 12x12x8-cell fcc Cu-Zr piece
 of the benchmark structure (4,608 atoms, a = 3.85 A, jitter 0.08 A, rc 5.0 A, rb 8,
 seed 3003) -- the graph semantics of src/force/sus2_sh.cu:4799-4846.  A last case leaves the
@@ -20,8 +20,8 @@ from oracle.binding import Oracle
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(4, 4, 160), (8, 2, 160), (4, 2, 160), (8, 2, 192), (8, 2, 256), (2, 4, 160),
-          (8, 2, 208)]
+SHAPES = [(4, 4, 160, 1), (8, 2, 160, 1), (4, 2, 160, 1), (8, 2, 192, 1), (8, 2, 256, 1),
+          (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2)]
 
 
 @pytest.fixture(scope="module")
@@ -65,15 +65,18 @@ def _evaluate(path, cell, types, pos, opts):
 
 
 @pytest.mark.parametrize("fp32", [True, False])
-@pytest.mark.parametrize("waves,nslot,slots", SHAPES)
-def test_lane_shape_matches_oracle(bench_case, waves, nslot, slots, fp32):
+@pytest.mark.parametrize("waves,nslot,slots,atoms", SHAPES)
+def test_lane_shape_matches_oracle(bench_case, waves, nslot, slots, atoms, fp32):
     path, pos, types, cell, ref = bench_case
+    if slots * 64 * atoms * (4 if fp32 else 8) > 160 * 1024:
+        pytest.skip("the tile's LDS (slots x 64 x atoms x value bytes) exceeds 160 KiB")
     opts = ["Cu", "Zr", f"sus2_float={int(fp32)}", "sus2_sh_graph_variant=3",
-            f"sus2_sh_lane_shape={waves}x{nslot}x{slots}"]
-    print(f"[case] lane shape ({waves},{nslot},{slots}) fp32={fp32} N={len(types)}", flush=True)
+            f"sus2_sh_lane_shape={waves}x{nslot}x{slots}x{atoms}"]
+    print(f"[case] lane shape ({waves},{nslot},{slots},{atoms}) fp32={fp32} N={len(types)}",
+          flush=True)
     got, info = _evaluate(path, cell, types, pos, opts)
     assert info["graph_variant"] == 3
-    assert (info["lane_waves"], info["lane_nslot"]) == (waves, nslot)
+    assert (info["lane_waves"], info["lane_nslot"], info["lane_atoms"]) == (waves, nslot, atoms)
     assert info["lane_slots"] <= slots
     _compare(ref, got, fp32)
 
@@ -86,5 +89,6 @@ def test_autotuned_shape_matches_oracle(bench_case, fp32):
     print(f"[case] autotuned fp32={fp32} N={len(types)}", flush=True)
     got, info = _evaluate(path, cell, types, pos, ["Cu", "Zr", f"sus2_float={int(fp32)}"])
     print(f"  kept variant {info['graph_variant']} tile {info['graph_tile']} lane shape "
-          f"({info['lane_waves']},{info['lane_nslot']},{info['lane_slots']})", flush=True)
+          f"({info['lane_waves']},{info['lane_nslot']},{info['lane_slots']},"
+          f"{info['lane_atoms']})", flush=True)
     _compare(ref, got, fp32)

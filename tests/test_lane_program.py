@@ -70,13 +70,14 @@ def test_l3333_structure(make_model):
     assert info["slots"] <= 160
 
 
-# every lane shape the engine's autotuner tries (waves, accumulators, slot budget)
-LANE_SHAPES = [(4, 4, 160), (8, 2, 160), (4, 2, 160), (8, 2, 192), (8, 2, 256), (2, 4, 160),
-               (8, 2, 208)]
+# every lane shape the engine's autotuner tries (waves, accumulators, slot budget, atoms per
+# lane)
+LANE_SHAPES = [(4, 4, 160, 1), (8, 2, 160, 1), (4, 2, 160, 1), (8, 2, 192, 1), (8, 2, 256, 1),
+               (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2)]
 
 
-@pytest.mark.parametrize("waves,nslot,slots", LANE_SHAPES)
-def test_l3333_every_tuned_shape_and_encoding(make_model, waves, nslot, slots):
+@pytest.mark.parametrize("waves,nslot,slots,atoms", LANE_SHAPES)
+def test_l3333_every_tuned_shape_and_encoding(make_model, waves, nslot, slots, atoms):
     """The l3333 program at each autotuned shape, run three ways on the host against the
     oracle's flat graph: the compiled program and its FP64 device encoding (1e-11 of the
     scale), and its FP32 device encoding -- the records the benchmarked FP32 kernel reads,
@@ -86,14 +87,14 @@ def test_l3333_every_tuned_shape_and_encoding(make_model, waves, nslot, slots):
                       rb_size=8, max_dist=5.0, seed=3003, scaling=0.5)
     orc = Oracle(path)
     B = orc.dims["B"]
-    rng = np.random.default_rng(waves * 100 + nslot * 10 + slots)
+    rng = np.random.default_rng(waves * 1000 + nslot * 100 + slots + atoms)
     basics = rng.normal(0.0, 0.4, size=(4, B))
     e0, g0 = orc.graph(basics)
     scale = max(1.0, np.abs(e0).max(), np.abs(g0).max())
     for real_bytes, tol in ((0, 1e-11), (8, 1e-11), (4, 2e-6)):
         info, e, g = sus2sh.lane_check(path, basics, max_slots=slots, waves=waves, nslot=nslot,
-                                       real_bytes=real_bytes)
-        print(f"[lane] l3333 shape=({waves},{nslot},{slots}) bytes={real_bytes}: {info}",
+                                       real_bytes=real_bytes, atoms=atoms)
+        print(f"[lane] l3333 shape=({waves},{nslot},{slots},{atoms}) bytes={real_bytes}: {info}",
               flush=True)
         assert info["ok"] == 1 and info["slots"] <= slots
         assert np.abs(e - e0).max() <= tol * scale

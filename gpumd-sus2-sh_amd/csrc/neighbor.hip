@@ -298,6 +298,189 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
   }
 }
 
+// The same search for one structure whose periodic cells are all at least rc wide (each
+// direction has >= 3 cells and a +-1 cell reach), with the candidates staged in LDS.  A
+// wave (one 64-atom workgroup) spans one or two cell rows (y, z); for each of the 9 row
+// offsets (oz, oy) -- in the order of k_neighbors, so every list comes out identical -- it
+// copies the atoms of the cells its lanes can reach in that band (x from the first lane's
+// cell - 1 to the last lane's + 1 per row) from the cell-sorted packed copy into LDS with
+// coalesced loads, then every lane scans its three cells there.  A band with more than
+// kStageCap atoms, and a wave over more than two rows, read the candidates from global
+// memory as k_neighbors does.
+constexpr int kStageCap = 512;  // staged candidate atoms per band (16 KiB)
+constexpr int kStageTab = 128;  // staged cells per band
+
+template <typename Real>
+__device__ __forceinline__ void test_candidate(const PackedAtom& pq, int q, int p, bool self_image,
+                                               double xp, double yp, double zp, double shx,
+                                               double shy, double shz, double rc2, bool valid,
+                                               int cap, int N, int& c, Edge<Real>* __restrict__ nl)
+{
+  if (self_image && q == p) return;
+  const double dx = pq.x - xp + shx;
+  const double dy = pq.y - yp + shy;
+  const double dz = pq.z - zp + shz;
+  if (dx * dx + dy * dy + dz * dz < rc2) {
+    if (valid && c < cap) {
+      Edge<Real> e;
+      e.dx = static_cast<Real>(dx);
+      e.dy = static_cast<Real>(dy);
+      e.dz = static_cast<Real>(dz);
+      e.j = q | (static_cast<int>(pq.t) << kEdgeTypeShift);
+      nl[static_cast<size_t>(c) * N + p] = e;
+    }
+    ++c;
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(64) void k_neighbors_staged(NeighborArgs a,
+                                                         const double* __restrict__ pos,
+                                                         const PackedAtom* __restrict__ packed,
+                                                         const int* __restrict__ skey,
+                                                         const int* __restrict__ cbeg,
+                                                         const int* __restrict__ cend,
+                                                         int* __restrict__ count,
+                                                         Edge<Real>* __restrict__ nl,
+                                                         int* __restrict__ max_count)
+{
+  __shared__ PackedAtom s_cand[kStageCap];
+  __shared__ int s_off[kStageTab + 1];
+  __shared__ int s_src[kStageTab];
+  const CellGrid& g = a.grid;
+  const int N = a.N;
+  const int lane = threadIdx.x;
+  const int p0 = xcd_block() * 64;
+  const int p = p0 + lane < N ? p0 + lane : N - 1;
+  const bool valid = p0 + lane < N;
+  const int nx = g.n[0], ny = g.n[1];
+  const int key = skey[p];
+  const int kf = __builtin_amdgcn_readfirstlane(skey[p0]);
+  const int kl = __builtin_amdgcn_readfirstlane(skey[min(p0 + 63, N - 1)]);
+  const int rf = kf / nx, rl = kl / nx;
+  const bool wave_direct = rl - rf > 1;
+  // x extent of the wave's cells per row, and each row's offset in the band's cell table
+  const int xlo0 = kf % nx, xhi0 = rl > rf ? nx - 1 : kl % nx;
+  const int xlo1 = 0, xhi1 = kl % nx;
+  const int w0 = xhi0 - xlo0 + 3, w1 = rl > rf ? xhi1 - xlo1 + 3 : 0;
+  const int ntab = w0 + w1;
+  const bool tab_ok = !wave_direct && ntab <= kStageTab;
+  const int cx = key % nx;
+  const int row = key / nx;
+  const int cy = row % ny, cz = row / ny;
+  const int ri = row - rf;  // the lane's row: 0 or 1
+  const int tbase = ri == 0 ? 0 : w0;
+  const int txlo = ri == 0 ? xlo0 : xlo1;
+  const double xp = pos[p], yp = pos[N + p], zp = pos[2 * N + p];
+  int c = 0;
+  for (int oz = -1; oz <= 1; ++oz) {
+    for (int oy = -1; oy <= 1; ++oy) {
+      // --- the band's cell table and staged atoms (wave-uniform decisions) ---
+      bool staged = false;
+      if (tab_ok) {
+        for (int e2 = 2 * lane; e2 < 2 * lane + 2; ++e2) {
+          int src = 0, cnt = 0;
+          if (e2 < ntab) {
+            const int r = e2 < w0 ? 0 : 1;
+            const int rr = rf + r;
+            int zb = rr / ny + oz, yb = rr % ny + oy, xb = (r == 0 ? xlo0 : xlo1) - 1 + (e2 - (r == 0 ? 0 : w0));
+            int sh;
+            bool ok = true;
+            if (g.pbc[2]) zb = wrap_cell(zb, g.n[2], sh); else ok = ok && zb >= 0 && zb < g.n[2];
+            if (g.pbc[1]) yb = wrap_cell(yb, ny, sh); else ok = ok && yb >= 0 && yb < ny;
+            if (g.pbc[0]) xb = wrap_cell(xb, nx, sh); else ok = ok && xb >= 0 && xb < nx;
+            if (ok) {
+              const int cc = (zb * ny + yb) * nx + xb;
+              src = cbeg[cc];
+              cnt = cend[cc] - src;
+            }
+          }
+          if (e2 < kStageTab) s_src[e2] = src;
+          // counts go through s_off for the scan below
+          if (e2 < kStageTab) s_off[e2] = cnt;
+        }
+        __syncthreads();
+        // exclusive prefix over the (at most 128) cell counts: two entries per lane
+        const int c0 = s_off[2 * lane], c1 = s_off[2 * lane + 1];
+        int incl = c0 + c1;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
+        }
+        const int excl = incl - (c0 + c1);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        __syncthreads();
+        s_off[2 * lane] = excl;
+        s_off[2 * lane + 1] = excl + c0;
+        if (lane == 0) s_off[kStageTab] = total;
+        __syncthreads();
+        staged = total <= kStageCap;
+        if (staged) {
+          for (int t = lane; t < total; t += 64) {
+            int lo = 0, hi = kStageTab - 1;  // last entry with s_off[e] <= t
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (s_off[mid] <= t)
+                lo = mid;
+              else
+                hi = mid - 1;
+            }
+            s_cand[t] = packed[s_src[lo] + (t - s_off[lo])];
+          }
+          __syncthreads();
+        }
+      }
+      // --- this lane's three cells of the band ---
+      int sz = 0, z = cz + oz;
+      bool band_ok = true;
+      if (g.pbc[2]) z = wrap_cell(z, g.n[2], sz); else band_ok = z >= 0 && z < g.n[2];
+      int sy = 0, y = cy + oy;
+      if (g.pbc[1]) y = wrap_cell(y, ny, sy); else band_ok = band_ok && y >= 0 && y < ny;
+      if (band_ok) {
+        for (int ox = -1; ox <= 1; ++ox) {
+          int sx = 0, x = cx + ox;
+          if (g.pbc[0]) {
+            x = wrap_cell(x, nx, sx);
+          } else if (x < 0 || x >= nx) {
+            continue;
+          }
+          const double shx = sx * g.h[0] + sy * g.h[1] + sz * g.h[2];
+          const double shy = sx * g.h[3] + sy * g.h[4] + sz * g.h[5];
+          const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
+          const bool self_image = (sx | sy | sz) == 0;
+          if (staged) {
+            const int e = tbase + (cx + ox) - (txlo - 1);
+            const int o = s_off[e], n = s_off[e + 1] - o, q0 = s_src[e];
+            for (int k = 0; k < n; ++k)
+              test_candidate<Real>(s_cand[o + k], q0 + k, p, self_image, xp, yp, zp, shx, shy, shz,
+                                   a.rc2, valid, a.cap, N, c, nl);
+          } else {
+            const int cc = (z * ny + y) * nx + x;
+            const int qe = cend[cc];
+            for (int q = cbeg[cc]; q < qe; ++q)
+              test_candidate<Real>(packed[q], q, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
+                                   valid, a.cap, N, c, nl);
+          }
+        }
+      }
+      if (tab_ok) __syncthreads();  // the next band rewrites the staged atoms
+    }
+  }
+  if (valid) count[p] = c < a.cap ? c : a.cap;
+  int cmax = valid ? c : 0;
+  unsigned long long csum = valid ? static_cast<unsigned long long>(c) : 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cmax = max(cmax, __shfl_xor(cmax, off, 64));
+    csum += __shfl_xor(csum, off, 64);
+  }
+  if (lane == 0) {
+    atomicMax(max_count, cmax);
+    atomicAdd(reinterpret_cast<unsigned long long*>(max_count + 2), csum);
+  }
+}
+
 // Caller order: lane i adds the results of its sorted position p = rank[i] -- the caller's
 // arrays are read-modify-written with coalesced accesses, the sorted arrays are gathered.
 __global__ void k_scatter(int N, const int* __restrict__ rank, const double* __restrict__ e_s,
@@ -510,8 +693,15 @@ void launch_neighbors(const NeighborArgs& a, const double* pos_s, const PackedAt
                       const int* sorted_key, const int* cell_begin, const int* cell_end,
                       int* count, Edge<Real>* nl, int* max_count, hipStream_t s)
 {
+  // the staged search needs every periodic direction split into >= 3 cells reached by +-1
+  bool stage = !a.grids;
+  for (int d = 0; d < 3 && stage; ++d)
+    stage = a.grid.R[d] == 1 && (a.grid.n[d] >= 3 || !a.grid.pbc[d]);
   if (a.grids)
     k_neighbors<Real, true><<<blocks_for(a.N, 256), 256, 0, s>>>(
+      a, pos_s, packed, sorted_key, cell_begin, cell_end, count, nl, max_count);
+  else if (stage)
+    k_neighbors_staged<Real><<<blocks_for(a.N, 64), 64, 0, s>>>(
       a, pos_s, packed, sorted_key, cell_begin, cell_end, count, nl, max_count);
   else
     k_neighbors<Real, false><<<blocks_for(a.N, 256), 256, 0, s>>>(

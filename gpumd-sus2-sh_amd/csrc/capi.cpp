@@ -173,6 +173,33 @@ int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
   });
 }
 
+int sus2sh_md_npt_mttk(sus2sh_handle* handle, sus2sh_box* box, int num_atoms, const int* d_type,
+                       const double* masses, double* d_position, double* d_velocity,
+                       double dt_fs, int nsteps, double t_start, double t_stop, double p_start,
+                       double p_stop, double tperiod, double pperiod, int couple,
+                       double* d_potential, double* d_force, double* d_virial, double* trace,
+                       void* stream)
+{
+  if (!handle) return SUS2SH_ERR_ARGUMENT;
+  if (!box) {
+    handle->error = "sus2sh_md_npt_mttk: box is NULL";
+    return SUS2SH_ERR_ARGUMENT;
+  }
+  return guarded(&handle->error, [&] {
+    sus2sh::Engine::NptParams prm;
+    prm.dt = dt_fs;
+    prm.t_start = t_start;
+    prm.t_stop = t_stop;
+    prm.p_start = p_start;
+    prm.p_stop = p_stop;
+    prm.tperiod = tperiod;
+    prm.pperiod = pperiod;
+    prm.couple = couple;
+    handle->engine->md_npt(*box, num_atoms, d_type, masses, d_position, d_velocity, prm, nsteps,
+                           d_potential, d_force, d_virial, trace, static_cast<hipStream_t>(stream));
+  });
+}
+
 int sus2sh_set_params(sus2sh_handle* handle, const double* theta, int n)
 {
   if (!handle) return SUS2SH_ERR_ARGUMENT;

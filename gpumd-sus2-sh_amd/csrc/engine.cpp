@@ -242,6 +242,7 @@ Engine::~Engine()
   if (md_event_) (void)hipEventDestroy(md_event_);
   if (md_stream_) (void)hipStreamDestroy(md_stream_);
   if (host_max_) (void)hipHostFree(host_max_);
+  if (host_red_) (void)hipHostFree(host_red_);
 }
 
 int Engine::param_count() const

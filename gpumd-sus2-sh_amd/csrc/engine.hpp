@@ -118,6 +118,21 @@ class Engine {
               double* vel, double dt_fs, int nsteps, double* pot, double* force, double* virial,
               double* energy_trace, hipStream_t stream);
 
+  // NPT (MTTK barostat + Nose-Hoover chains, npt.cpp): nsteps steps of dt fs; `box` is
+  // updated in place; trace (host, may be null) receives double[nsteps + 1][8] = (potential,
+  // kinetic, conserved extended energy (eV), volume (A^3), temperature (K), pressure (GPa),
+  // cube root of the volume (A), target temperature (K)).
+  struct NptParams {
+    double dt = 1.0;
+    double t_start = 300.0, t_stop = 300.0;  // K
+    double p_start = 0.0, p_stop = 0.0;      // GPa
+    double tperiod = 100.0, pperiod = 1000.0;  // in time steps
+    int couple = 1;  // 0 iso, 1 aniso
+  };
+  void md_npt(sus2sh_box& box, int N, const int* type, const double* masses, double* pos,
+              double* vel, const NptParams& prm, int nsteps, double* pot, double* force,
+              double* virial, double* trace, hipStream_t stream);
+
   // Synchronise `stream` and report the checks latched by deferred-check evaluations since
   // the previous check (neighbour capacity: CapacityError, with the capacity raised for the
   // next call; atom type out of range: ArgumentError).
@@ -223,6 +238,8 @@ class Engine {
   DeviceBuffer pg_rad_, pg_misc_, md_mass_, md_energy_;
   DeviceBuffer batch_grids_, batch_cell_off_, batch_atom_off_, sid_, sid_s_;
   int* host_max_ = nullptr;  // pinned, 8 ints
+  double* host_red_ = nullptr;  // pinned, 8 doubles (NPT reductions)
+  DeviceBuffer md_red_;
 
   Workspace ws_;  // work buffers of the evaluation in progress
   std::vector<float> stage_ms_;

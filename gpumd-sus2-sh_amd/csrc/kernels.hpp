@@ -255,6 +255,10 @@ void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s);
 void launch_step_advance(int* step, hipStream_t s);
 void launch_energies(int N, const int* type, const double* mass, const double* pot,
                      const double* v, double* out, const int* step, hipStream_t s);
+// NPT: out[0..7) += (sum m v_x^2, m v_y^2, m v_z^2 in eV, sum potential, sum virial xx, yy, zz)
+void launch_npt_reduce(int N, const int* type, const double* mass, const double* pot,
+                       const double* v, const double* vir, double* out, hipStream_t s);
+void launch_scale3(int N, double s0, double s1, double s2, double* x, hipStream_t s);
 
 // --- scatter of per-sorted-atom results to the caller's arrays ---
 void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,

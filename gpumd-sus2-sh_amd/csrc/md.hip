@@ -67,6 +67,43 @@ __global__ void k_energies(int N, const int* __restrict__ type, const double* __
   }
 }
 
+// NPT: out[0..2] += sum m v_a^2 (eV, per Cartesian axis), out[3] += sum potential,
+// out[4..6] += sum of the diagonal per-atom virials xx yy zz
+__global__ void k_npt_reduce(int N, const int* __restrict__ type, const double* __restrict__ mass,
+                             const double* __restrict__ pot, const double* __restrict__ v,
+                             const double* __restrict__ vir, double* __restrict__ out)
+{
+  __shared__ double part[4][7];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double q[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (i < N) {
+    const double m = mass[type[i]] * kKinetic;
+    for (int a = 0; a < 3; ++a) q[a] = m * v[static_cast<size_t>(a) * N + i] * v[static_cast<size_t>(a) * N + i];
+    q[3] = pot[i];
+    for (int a = 0; a < 3; ++a) q[4 + a] = vir[static_cast<size_t>(a) * N + i];
+  }
+  for (int k = 0; k < 7; ++k)
+    for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 7; ++k) part[threadIdx.x >> 6][k] = q[k];
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    double s = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) s += part[w][threadIdx.x];
+    unsafeAtomicAdd(out + threadIdx.x, s);
+  }
+}
+
+// x_a *= s_a (one factor per Cartesian axis)
+__global__ void k_scale3(int N, double s0, double s1, double s2, double* __restrict__ x)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  x[i] *= s0;
+  x[N + i] *= s1;
+  x[2 * N + i] *= s2;
+}
+
 __global__ void k_step_advance(int* step)
 {
   if (threadIdx.x == 0) *step += 1;
@@ -94,5 +131,16 @@ void launch_energies(int N, const int* type, const double* mass, const double* p
 }
 
 void launch_step_advance(int* step, hipStream_t s) { k_step_advance<<<1, 64, 0, s>>>(step); }
+
+void launch_npt_reduce(int N, const int* type, const double* mass, const double* pot,
+                       const double* v, const double* vir, double* out, hipStream_t s)
+{
+  k_npt_reduce<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, vir, out);
+}
+
+void launch_scale3(int N, double s0, double s1, double s2, double* x, hipStream_t s)
+{
+  k_scale3<<<nblocks(N), 256, 0, s>>>(N, s0, s1, s2, x);
+}
 
 }  // namespace sus2sh

@@ -26,6 +26,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_compute_param_grad",
     "sus2sh_compute_batch",
     "sus2sh_md_nve",
+    "sus2sh_md_npt_mttk",
     "sus2sh_check",
     "sus2sh_set_params",
     "sus2sh_get_params",
@@ -142,6 +143,11 @@ def lib():
                                 c_void_p, c_double, c_int, c_void_p, c_void_p, c_void_p,
                                 P(c_double), c_void_p]
     L.sus2sh_md_nve.restype = c_int
+    L.sus2sh_md_npt_mttk.argtypes = [c_void_p, P(Box), c_int, c_void_p, P(c_double), c_void_p,
+                                     c_void_p, c_double, c_int, c_double, c_double, c_double,
+                                     c_double, c_double, c_double, c_int, c_void_p, c_void_p,
+                                     c_void_p, P(c_double), c_void_p]
+    L.sus2sh_md_npt_mttk.restype = c_int
     L.sus2sh_check.argtypes = [c_void_p, c_void_p]
     L.sus2sh_check.restype = c_int
     L.sus2sh_set_params.argtypes = [c_void_p, P(c_double), c_int]

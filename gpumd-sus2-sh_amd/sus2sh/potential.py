@@ -192,6 +192,43 @@ def md_nve(pot, box, type, position, velocity, masses, dt_fs, nsteps, stream=Non
     return trace, e, f, v
 
 
+def md_npt_mttk(pot, box, type, position, velocity, masses, dt_fs, nsteps, t_start, t_stop,
+                p_start, p_stop, tperiod=100.0, pperiod=1000.0, couple="aniso", stream=None):
+    """NPT with the MTK barostat and Nose-Hoover chains on the GPU (sus2sh_md_npt_mttk), the
+    `ensemble npt_mttk temp T1 T2 aniso p1 p2 tperiod pperiod` step of the reference's
+    benchmark protocol (docs/gpumd-sus2-sh-interface-plan.md:264-271).  box (a Box struct) and
+    position/velocity (float64 [3, N] cuda tensors) are advanced in place; temperatures in K,
+    pressures in GPa, periods in time steps; couple "iso" or "aniso".  Returns (trace
+    [nsteps+1, 8] of (potential, kinetic, conserved extended energy, volume, temperature,
+    pressure, cube root of the volume, target temperature), potential[N], force[3,N],
+    virial[9,N])."""
+    couples = {"iso": 0, "aniso": 1}
+    if couple not in couples:
+        raise ValueError(f"couple must be 'iso' or 'aniso', got {couple!r}")
+    n = int(type.numel())
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    if _is_torch(position):
+        import torch
+        e = torch.zeros(n, dtype=torch.float64, device=position.device)
+        f = torch.zeros(3, n, dtype=torch.float64, device=position.device)
+        v = torch.zeros(9, n, dtype=torch.float64, device=position.device)
+    else:
+        from .hipmem import DeviceArray
+        e = DeviceArray.zeros((n,), np.float64)
+        f = DeviceArray.zeros((3, n), np.float64)
+        v = DeviceArray.zeros((9, n), np.float64)
+    trace = np.zeros((nsteps + 1, 8))
+    s = _stream(type, stream)
+    code = lib().sus2sh_md_npt_mttk(
+        pot._h, ctypes.byref(box), n, _ptr(type),
+        m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(position), _ptr(velocity),
+        float(dt_fs), int(nsteps), float(t_start), float(t_stop), float(p_start), float(p_stop),
+        float(tperiod), float(pperiod), couples[couple], _ptr(e), _ptr(f), _ptr(v),
+        trace.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s)
+    check(code, pot._h)
+    return trace, e, f, v
+
+
 def maxwell_velocities(masses_per_atom, temperature, seed):
     """Maxwell-Boltzmann velocities (A/fs) at `temperature` K with zero total momentum."""
     kB = 8.617333262e-5  # eV/K

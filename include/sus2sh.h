@@ -170,6 +170,26 @@ int sus2sh_md_nve(sus2sh_handle* handle, const sus2sh_box* box, int num_atoms,
                   double* d_velocity, double dt_fs, int nsteps, double* d_potential,
                   double* d_force, double* d_virial, double* energy_trace, void* stream);
 
+/* NPT molecular dynamics, the integrator of the reference's benchmark protocol (GPUMD
+ * `ensemble npt_mttk temp T1 T2 aniso p1 p2 tperiod 50 pperiod 500`,
+ * docs/gpumd-sus2-sh-interface-plan.md:264-271; GPUMD's integrator is not in the reference
+ * overlay): the Martyna-Tobias-Klein barostat with Nose-Hoover chains (length 3) on the
+ * particles and on the barostat, in the reversible splitting of Shinoda et al. (2004).
+ * couple = 0 (iso: the mean pressure drives three identical axes) or 1 (aniso: each axis its
+ * own diagonal pressure); temperatures in K and pressures in GPa ramp linearly from start to
+ * stop over the run; tperiod / pperiod are the thermostat / barostat periods in time steps.
+ * The box must be periodic in x, y, z; its matrix (and inverse) is updated in place, and the
+ * positions are dilated with it.  d_potential / d_force / d_virial hold the last evaluation;
+ * trace (host, may be NULL) receives double[nsteps + 1][8] = (potential, kinetic, conserved
+ * extended energy in eV, volume A^3, temperature K, pressure GPa, cube root of the volume A,
+ * target temperature K).  No counterpart in the reference overlay. */
+int sus2sh_md_npt_mttk(sus2sh_handle* handle, sus2sh_box* box, int num_atoms, const int* d_type,
+                       const double* masses, double* d_position, double* d_velocity,
+                       double dt_fs, int nsteps, double t_start, double t_stop, double p_start,
+                       double p_stop, double tperiod, double pperiod, int couple,
+                       double* d_potential, double* d_force, double* d_virial, double* trace,
+                       void* stream);
+
 /* Read / replace the model coefficients in the dE/dtheta order (host double[param_count]).
  * set_params re-uploads the device tables; the graph topology is unchanged.  For training
  * and gradient checks; no reference counterpart (the reference reads coefficients once). */

@@ -27,6 +27,28 @@ constexpr int kEdgeTypeShift = 27;
 constexpr int kEdgeIndexMask = (1 << kEdgeTypeShift) - 1;
 constexpr int kMaxEdgeAtoms = 1 << kEdgeTypeShift;
 
+// Deterministic force summation: force contributions are rounded to 64-bit fixed point
+// (quantum 2^-34 eV/A) and summed with integer atomics, which are associative, so the totals
+// do not depend on the order in which threads add them.  Bounds that keep every per-atom
+// total inside the signed 64-bit range (2^29 eV/A): one pair contribution below 2^17 eV/A,
+// one lane's own sum below 2^26 eV/A, at most kFixMaxNeighbors neighbours per atom -- twice
+// (many-body and ZBL terms) 2^26 + 1023 * 2^17 < 2^28.  A contribution outside its bound
+// sets *flag and adds nothing.
+constexpr double kFixScale = 17179869184.0;        // 2^34
+constexpr double kFixInvScale = 1.0 / 17179869184.0;
+constexpr double kFixPairMax = 131072.0;           // 2^17 eV/A
+constexpr double kFixSumMax = 67108864.0;          // 2^26 eV/A
+constexpr int kFixMaxNeighbors = 1023;
+
+__device__ __forceinline__ unsigned long long fix_encode(double v, double limit, int* flag)
+{
+  if (!(fabs(v) < limit)) {  // also NaN
+    atomicOr(flag, 1);
+    return 0ull;
+  }
+  return static_cast<unsigned long long>(__double2ll_rn(v * kFixScale));
+}
+
 template <typename Real>
 struct alignas(16) Edge {
   Real dx, dy, dz;  // r_q(image) - r_p

@@ -198,7 +198,9 @@ __device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
 // GATE (two-layer mu gate, src/force/sus2_sh.cu:6077-6123, 6555-6693): the edge's radial
 // channels are weighted by the neighbour's multiplier G_{j,mu}, and
 // dE/dh_{j,mu} += dG/dh_{j,mu} R_mu sum_m G_{k,l,m} Y_lm is scattered to the neighbour.
-template <typename Real, int L, int K, bool PG, int NB, bool WIN, bool GATE>
+// FIX (deterministic summation): f_s holds 64-bit fixed-point sums (fix_encode); the window
+// accumulators are fixed point too, so every total is independent of the order of the adds.
+template <typename Real, int L, int K, bool PG, int NB, bool WIN, bool GATE, bool FIX>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 4) ? 3 : 1))) void k_forces(EdgeLaunch<Real> a, const Real* __restrict__ grads,
                                                 double* __restrict__ f_s,
@@ -227,6 +229,8 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   int* w_first = reinterpret_cast<int*>(smem_raw + pg_off);
   int* w_off = w_first + kWinMax;
   Real* acc_w = reinterpret_cast<Real*>(smem_raw + pg_off + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16));
+  unsigned long long* acc_q = reinterpret_cast<unsigned long long*>(acc_w);  // FIX
+  unsigned long long* f_q = reinterpret_cast<unsigned long long*>(f_s);      // FIX
   int w_count = -1, w_total = 0;
   const int block = xcd_block();  // the force tile: kWinTile atoms of the cell order
   if (WIN) {
@@ -243,7 +247,12 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
         }
         w_off[w_count] = o;
       }
-      for (int i = threadIdx.x; i < 3 * w_total; i += blockDim.x) acc_w[i] = Real(0);
+      for (int i = threadIdx.x; i < 3 * w_total; i += blockDim.x) {
+        if constexpr (FIX)
+          acc_q[i] = 0ull;
+        else
+          acc_w[i] = Real(0);
+      }
     }
   }
   stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);  // ends with a barrier
@@ -347,7 +356,20 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     fy += g1;
     fz += g2;
     const int slot = (WIN && w_count >= 0) ? window_slot(w_first, w_off, w_count, jn) : -1;
-    if (slot >= 0) {
+    if constexpr (FIX) {
+      const unsigned long long q0 = fix_encode(-static_cast<double>(g0), kFixPairMax, a.fix_flag);
+      const unsigned long long q1 = fix_encode(-static_cast<double>(g1), kFixPairMax, a.fix_flag);
+      const unsigned long long q2 = fix_encode(-static_cast<double>(g2), kFixPairMax, a.fix_flag);
+      if (slot >= 0) {
+        atomicAdd(acc_q + slot, q0);
+        atomicAdd(acc_q + w_total + slot, q1);
+        atomicAdd(acc_q + 2 * w_total + slot, q2);
+      } else {
+        atomicAdd(f_q + jn, q0);
+        atomicAdd(f_q + N + jn, q1);
+        atomicAdd(f_q + 2 * N + jn, q2);
+      }
+    } else if (slot >= 0) {
       atomicAdd(acc_w + slot, -g0);
       atomicAdd(acc_w + w_total + slot, -g1);
       atomicAdd(acc_w + 2 * w_total + slot, -g2);
@@ -367,9 +389,15 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     w[8] -= e.dz * g1;
   }
   if (valid) {
-    unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
-    unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
-    unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
+    if constexpr (FIX) {
+      atomicAdd(f_q + p, fix_encode(static_cast<double>(fx), kFixSumMax, a.fix_flag));
+      atomicAdd(f_q + N + p, fix_encode(static_cast<double>(fy), kFixSumMax, a.fix_flag));
+      atomicAdd(f_q + 2 * N + p, fix_encode(static_cast<double>(fz), kFixSumMax, a.fix_flag));
+    } else {
+      unsafeAtomicAdd(f_s + p, static_cast<double>(fx));
+      unsafeAtomicAdd(f_s + N + p, static_cast<double>(fy));
+      unsafeAtomicAdd(f_s + 2 * N + p, static_cast<double>(fz));
+    }
     if (a.v_accumulate) {
 #pragma unroll
       for (int c = 0; c < 9; ++c) v_s[static_cast<size_t>(c) * N + p] += static_cast<double>(w[c]);
@@ -392,8 +420,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
       const int j = w_first[lo] + (i - w_off[lo]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const Real v = acc_w[c * w_total + i];
-        if (v != Real(0)) unsafeAtomicAdd(f_s + static_cast<size_t>(c) * N + j, static_cast<double>(v));
+        if constexpr (FIX) {
+          const unsigned long long v = acc_q[c * w_total + i];
+          if (v != 0ull) atomicAdd(f_q + static_cast<size_t>(c) * N + j, v);
+        } else {
+          const Real v = acc_w[c * w_total + i];
+          if (v != Real(0)) unsafeAtomicAdd(f_s + static_cast<size_t>(c) * N + j, static_cast<double>(v));
+        }
       }
     }
   }
@@ -458,8 +491,9 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
 {
   const int blocks = (a.N + 255) / 256;
   static_assert(kWinTile == 256, "force tiles are the 256-thread workgroups");
-  // the parameter gradient and the gated kernel use the per-mu path
-  const int nb = (pg_rad || a.gate_m) ? 0 : uniform_nb(a);
+  // the parameter gradient, the gated kernel and deterministic summation use the per-mu path
+  const bool fix = a.fix_flag != nullptr;
+  const int nb = (pg_rad || a.gate_m || fix) ? 0 : uniform_nb(a);
   const bool use_win = win && !pg_rad && !a.gate_m;
   size_t lds = table_bytes(a, nb);
   if (pg_rad) {
@@ -469,29 +503,32 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   }
   if (use_win)
     lds = ((lds + 15) & ~size_t(15)) + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16) +
-          3 * sizeof(Real) * static_cast<size_t>(win_cap);
-#define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV, GV)                                       \
+          3 * (fix ? sizeof(unsigned long long) : sizeof(Real)) * static_cast<size_t>(win_cap);
+#define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV, GV, FV)                                   \
   {                                                                                            \
-    auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV, GV>;                                   \
-    if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV, GV>>();           \
+    auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV, GV, FV>;                               \
+    if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV, GV, FV>>();       \
     kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
   }
 #define SUS2SH_FORCES_CASE(KK)                                                               \
   if (a.K == KK) {                                                                           \
     if (pg_rad) {                                                                            \
-      SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false, false)                                    \
+      SUS2SH_FORCES_LAUNCH(LL, KK, true, 0, false, false, false)                             \
     } else if (a.gate_m) {                                                                   \
-      SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, true)                                    \
+      SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, true, false)                             \
+    } else if (fix) {                                                                        \
+      if (use_win) SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true, false, true)                 \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, false, true)                        \
     } else if (use_win) {                                                                    \
-      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true, false)                       \
-      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, true, false)                \
-      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true, false)                \
-      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true, false)                               \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, true, false, false)                \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, true, false, false)         \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, true, false, false)         \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, true, false, false)                        \
     } else {                                                                                 \
-      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false, false)                      \
-      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, false, false)               \
-      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false, false)               \
-      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, false)                              \
+      if (nb == 8) SUS2SH_FORCES_LAUNCH(LL, KK, false, 8, false, false, false)               \
+      else if (nb == 12) SUS2SH_FORCES_LAUNCH(LL, KK, false, 12, false, false, false)        \
+      else if (nb == 16) SUS2SH_FORCES_LAUNCH(LL, KK, false, 16, false, false, false)        \
+      else SUS2SH_FORCES_LAUNCH(LL, KK, false, 0, false, false, false)                       \
     }                                                                                        \
     return true;                                                                             \
   }

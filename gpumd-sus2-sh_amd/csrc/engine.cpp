@@ -101,6 +101,8 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.deferred_check = parse_bool(val, key);
     } else if (key == "sus2_sh_md_graph") {
       o.md_graph = parse_bool(val, key);
+    } else if (key == "sus2_sh_deterministic") {
+      o.deterministic = parse_bool(val, key);
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -156,7 +158,10 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   if (model_.species_count > kMaxSpecies)
     throw UnsupportedError("SUS2_SH: this engine supports at most 16 species.");
   opt_ = parse_options(model_, nopts, opts);
-  prog_ = compile_graph(model_);
+  if (opt_.deterministic && model_.gate_enabled)
+    throw UnsupportedError("SUS2_SH: sus2_sh_deterministic=1 is not available for two-layer gate "
+                           "models.");
+  prog_ = compile_graph(model_, graph_chunk_terms());
   if (model_.zbl_enabled) {
     // atomic numbers and pair cutoffs (src/force/sus2_sh.cu:7824-7860); the neighbour
     // cutoff of the reference is max(max_dist, largest ZBL outer cutoff)
@@ -218,6 +223,12 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
       throw InputError("sus2_sh_graph_variant=3: no lane program for this model (" + lane_.reason +
                        ").");
     tuned_ = true;  // a fixed variant disables the autotuner
+  }
+  if (opt_.deterministic) {
+    // a timed choice of kernel could differ between runs (and so could its rounding): the
+    // lane program when the model has one, else the balanced-stream row kernel
+    tuned_ = true;
+    if (opt_.graph_variant < 0) variant_ = lane_.ok ? 3 : 1;
   }
   if (opt_.use_float)
     upload_tables<float>();
@@ -758,6 +769,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (gate && noalloc)
     throw UnsupportedError("SUS2_SH: compute_noalloc does not evaluate two-layer gate models "
                            "(use sus2sh_compute).");
+  if (param_grad && opt_.deterministic)
+    throw UnsupportedError("SUS2_SH: dE/dtheta is not available with sus2_sh_deterministic=1.");
   if (param_grad && model_.radial_kind == RadialKind::LaguerreLog1p)
     throw UnsupportedError("SUS2_SH: dE/dtheta is not implemented for RBLaguerre_log1p models.");
   const bool batch = nstruct > 1;
@@ -892,6 +905,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       mean_neighbors_ = static_cast<double>(total) / N;
       if (host_max_[1] != 0)
         throw ArgumentError("atom type outside [0, species_count) passed to compute");
+      if (opt_.deterministic && maxc > kFixMaxNeighbors) throw ArgumentError(fix_range_message());
       if (maxc <= ws_.cap) break;
       if (noalloc) {
         std::ostringstream msg;
@@ -928,6 +942,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.rt.kind = static_cast<int>(model_.radial_kind);  // 0 tanh, 1 rational, 2 Laguerre
   el.rt.uniform = radial_uniform_ ? 1 : 0;
   el.fmap = fmap_.as<int>();
+  el.fix_flag = opt_.deterministic ? latch_.as<int>() + kLatchFix : nullptr;
   // two-layer gate (src/force/sus2_sh.cu:9821-9893): gate moments with the gate radial
   // coefficients, the gate layer's graph, G = 1 + A tanh(a h) per atom and channel; the main
   // moments then weight every edge by the neighbour's G
@@ -1077,7 +1092,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   const int* win = nullptr;
   int win_cap = 0;
   if (use_win) {
-    win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * sizeof(Real)));
+    const size_t acc_bytes = opt_.deterministic ? sizeof(unsigned long long) : sizeof(Real);
+    win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * acc_bytes));
     launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
     win = ws_.win;
   }
@@ -1112,12 +1128,25 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     const ZblTable zt{zbl_z_.as<int>(), zbl_inner_.as<double>(), zbl_outer_.as<double>(),
                       model_.species_count};
     launch_zbl<Real>(N, ws_.count, static_cast<const Edge<Real>*>(ws_.nl), ws_.type_s, zt,
-                     ws_.e_s, ws_.f_s, ws_.v_s, s);
+                     ws_.e_s, ws_.f_s, ws_.v_s, el.fix_flag, s);
   }
   if (profile) hip_check(hipEventRecord(events_[4], s), "hipEventRecord");
 
-  launch_scatter(N, ws_.rank, ws_.e_s, ws_.f_s, ws_.v_s, pot, force, virial, s);
+  launch_scatter(N, ws_.rank, ws_.e_s, ws_.f_s, ws_.v_s, pot, force, virial,
+                 opt_.deterministic ? 1 : 0, s);
   hip_check(hipGetLastError(), "kernel launch");
+  if (opt_.deterministic && !deferred) {
+    // out-of-range contributions of the fixed-point sums (a deferred call reports them at
+    // check())
+    hip_check(hipMemcpyAsync(host_max_ + kLatchFix, latch_.as<int>() + kLatchFix, sizeof(int),
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (host_max_[kLatchFix] != 0) {
+      launch_fill_int(latch_.as<int>() + kLatchFix, 0, 1, s);
+      throw ArgumentError(fix_range_message());
+    }
+  }
   if (param_grad) finish_param_grad(param_grad, s);
   if (profile) {
     hip_check(hipEventRecord(events_[5], s), "hipEventRecord");
@@ -1157,12 +1186,22 @@ void Engine::set_params(const double* theta, int n)
   take(m.radial_coeffs);
   take(m.radial_type_coeffs);
   take(m.scal_coeffs);
-  prog_ = compile_graph(m);
+  prog_ = compile_graph(m, graph_chunk_terms());
   compile_lane();
   if (opt_.use_float)
     upload_tables<float>();
   else
     upload_tables<double>();
+}
+
+std::string Engine::fix_range_message()
+{
+  std::ostringstream msg;
+  msg << "SUS2_SH: sus2_sh_deterministic=1: a force contribution is outside the fixed-point "
+         "range (pair terms below "
+      << kFixPairMax << " eV/A, per-atom sums below " << kFixSumMax << " eV/A, at most "
+      << kFixMaxNeighbors << " neighbours per atom)";
+  return msg.str();
 }
 
 void Engine::reset_latch(hipStream_t s) { launch_fill_int(latch_.as<int>(), 0, 8, s); }
@@ -1181,6 +1220,8 @@ void Engine::check(hipStream_t s)
   read_latch(s, l);
   reset_latch(s);
   if (l[4] == 0) return;
+  if (opt_.deterministic && (l[kLatchFix] != 0 || l[0] > kFixMaxNeighbors))
+    throw ArgumentError(fix_range_message());
   const unsigned long long total = static_cast<unsigned long long>(static_cast<unsigned>(l[2])) |
                                    (static_cast<unsigned long long>(static_cast<unsigned>(l[3])) << 32);
   if (cap_atoms_ > 0) mean_neighbors_ = static_cast<double>(total) / cap_atoms_;
@@ -1221,6 +1262,7 @@ void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double*
   md_mass_.upload(masses, S * sizeof(double));
   md_energy_.resize(2 * static_cast<size_t>(nsteps + 1) * sizeof(double));
   md_step_.resize(sizeof(int));
+  md_partial_.resize(md_partials_size(N) * sizeof(double));
   double* en = md_energy_.as<double>();
   const double* mass = md_mass_.as<double>();
   int* step = md_step_.as<int>();
@@ -1238,12 +1280,12 @@ void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double*
     launch_drift(N, dt_fs, vel, pos, s);
     evaluate();
     launch_half_kick(N, 0.5 * dt_fs, type, mass, force, vel, s);
-    launch_energies(N, type, mass, pot, vel, en, step, s);
+    launch_energies(N, type, mass, pot, vel, en, step, md_partial_.as<double>(), s);
     launch_step_advance(step, s);
   };
   evaluate();  // synchronised: sizes the neighbour capacity
   launch_fill_int(step, 0, 1, s);
-  launch_energies(N, type, mass, pot, vel, en, step, s);
+  launch_energies(N, type, mass, pot, vel, en, step, md_partial_.as<double>(), s);
   launch_step_advance(step, s);
   if (!opt_.md_graph) {
     for (int k = 1; k <= nsteps; ++k) md_step();
@@ -1293,6 +1335,8 @@ void Engine::md_nve(const sus2sh_box& box, int N, const int* type, const double*
         int l[8];
         read_latch(s, l);
         if (l[1] != 0) throw ArgumentError("atom type outside [0, species_count) passed to compute");
+        if (opt_.deterministic && (l[kLatchFix] != 0 || l[0] > kFixMaxNeighbors))
+          throw ArgumentError(fix_range_message());
         if (l[0] > cap_) {
           if (++retries > 8) throw DeviceError("md_nve: neighbour capacity did not settle");
           // roll back to the chunk start and size the capacity with a synchronised call

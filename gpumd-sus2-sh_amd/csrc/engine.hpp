@@ -70,6 +70,9 @@ struct EngineOptions {
   // checks are latched on the device and reported by check()
   bool deferred_check = false;
   bool md_graph = true;  // md_nve replays steady steps from a captured HIP graph
+  // bit-reproducible energies, forces and virials: fixed-point force sums, no autotuner,
+  // graph programs without atomic back-chunk merges
+  bool deterministic = false;
 };
 
 struct CapacityError : public std::runtime_error {
@@ -156,6 +159,11 @@ class Engine {
   double mean_neighbors() const { return mean_neighbors_; }
 
  private:
+  // latch_ word set by a fixed-point force contribution out of range (deterministic mode)
+  static constexpr int kLatchFix = 5;
+  // back-chunk length of the row programs: deterministic mode merges no chunk atomically
+  int graph_chunk_terms() const { return opt_.deterministic ? (1 << 30) : 8; }
+  static std::string fix_range_message();
   template <typename Real>
   void upload_tables();
   template <typename Real>
@@ -240,6 +248,7 @@ class Engine {
   int* host_max_ = nullptr;  // pinned, 8 ints
   double* host_red_ = nullptr;  // pinned, 8 doubles (NPT reductions)
   DeviceBuffer md_red_;
+  DeviceBuffer md_partial_;  // per-block partial sums of the MD reductions
 
   Workspace ws_;  // work buffers of the evaluation in progress
   std::vector<float> stage_ms_;

@@ -148,7 +148,8 @@ struct ZblTable {
 };
 template <typename Real>
 void launch_zbl(int N, const int* count, const Edge<Real>* nl, const int* type_s,
-                const ZblTable& t, double* e_s, double* f_s, double* v_s, hipStream_t s);
+                const ZblTable& t, double* e_s, double* f_s, double* v_s, int* fix_flag,
+                hipStream_t s);  // fix_flag != null: f_s holds fixed-point sums
 
 // --- edge kernels (edge_kernels.hip) ---
 template <typename Real>
@@ -168,6 +169,9 @@ struct EdgeLaunch {
   const Real* gate_d = nullptr;
   Real* gate_adj = nullptr;
   int v_accumulate = 0;  // forces: add the per-atom virial to v_s instead of storing it
+  // deterministic summation (null otherwise): f_s holds 64-bit fixed-point sums
+  // (fix_encode); *fix_flag is set when a contribution is out of range
+  int* fix_flag = nullptr;
 };
 
 template <typename Real>
@@ -250,19 +254,25 @@ size_t gate_graph_lds_bytes(int tile, int slots, int real_bytes);
 void launch_half_kick(int N, double half_dt, const int* type, const double* mass, const double* f,
                       double* v, hipStream_t s);
 void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s);
-// step == nullptr: out[0..1] += (potential, kinetic); otherwise out[2 * *step ..] +=, and
+// The reductions below sum in a fixed order (bit-identical between runs) through `partial`,
+// a device buffer of md_partials_size(N) doubles.
+size_t md_partials_size(int N);
+// step == nullptr: out[0..1] = (potential, kinetic); otherwise out[2 * *step ..] =, and
 // launch_step_advance(step) increments the device step counter
 void launch_step_advance(int* step, hipStream_t s);
 void launch_energies(int N, const int* type, const double* mass, const double* pot,
-                     const double* v, double* out, const int* step, hipStream_t s);
-// NPT: out[0..7) += (sum m v_x^2, m v_y^2, m v_z^2 in eV, sum potential, sum virial xx, yy, zz)
+                     const double* v, double* out, const int* step, double* partial,
+                     hipStream_t s);
+// NPT: out[0..7) = (sum m v_x^2, m v_y^2, m v_z^2 in eV, sum potential, sum virial xx, yy, zz)
 void launch_npt_reduce(int N, const int* type, const double* mass, const double* pot,
-                       const double* v, const double* vir, double* out, hipStream_t s);
+                       const double* v, const double* vir, double* out, double* partial,
+                       hipStream_t s);
 void launch_scale3(int N, double s0, double s1, double s2, double* x, hipStream_t s);
 
 // --- scatter of per-sorted-atom results to the caller's arrays ---
+// fixed != 0: f_s holds 64-bit fixed-point sums (deterministic summation)
 void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,
                     const double* v_s, double* potential, double* force, double* virial,
-                    hipStream_t s);
+                    int fixed, hipStream_t s);
 
 }  // namespace sus2sh

@@ -32,48 +32,48 @@ __global__ void k_drift(int N, double dt, const double* __restrict__ v, double* 
   x[2 * N + i] += dt * v[2 * N + i];
 }
 
-// out[0] += sum potential, out[1] += sum kinetic (one atomic pair per workgroup); with a
-// device step counter the pair is out[2 * step], out[2 * step + 1]
-__global__ void k_energies(int N, const int* __restrict__ type, const double* __restrict__ mass,
-                           const double* __restrict__ pot, const double* __restrict__ v,
-                           double* __restrict__ out, const int* __restrict__ step)
+// Reductions for the MD drivers in two passes with a fixed order of additions (a butterfly
+// within each wave, the waves in index order, the blocks in index order), so that energy
+// traces -- and the NPT barostat driven by them -- are bit-identical between runs.
+// Pass 1: block b writes its NV sums to partial[b * NV + k].
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&q)[NV], double* __restrict__ partial)
 {
-  __shared__ double sp[4], sk[4];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double p = 0.0, k = 0.0;
-  if (i < N) {
-    p = pot[i];
-    const double vv = v[i] * v[i] + v[N + i] * v[N + i] + v[2 * N + i] * v[2 * N + i];
-    k = 0.5 * mass[type[i]] * vv * kKinetic;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    p += __shfl_xor(p, off, 64);
-    k += __shfl_xor(k, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sp[threadIdx.x >> 6] = p;
-    sk[threadIdx.x >> 6] = k;
-  }
+  __shared__ double part[4][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[threadIdx.x >> 6][k] = q[k];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (step) out += 2 * static_cast<size_t>(*step);
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) {
-      a += sp[w];
-      b += sk[w];
-    }
-    unsafeAtomicAdd(out, a);
-    unsafeAtomicAdd(out + 1, b);
+  if (threadIdx.x < NV) {
+    double sum = 0.0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) sum += part[w][threadIdx.x];
+    partial[static_cast<size_t>(blockIdx.x) * NV + threadIdx.x] = sum;
   }
 }
 
-// NPT: out[0..2] += sum m v_a^2 (eV, per Cartesian axis), out[3] += sum potential,
-// out[4..6] += sum of the diagonal per-atom virials xx yy zz
-__global__ void k_npt_reduce(int N, const int* __restrict__ type, const double* __restrict__ mass,
-                             const double* __restrict__ pot, const double* __restrict__ v,
-                             const double* __restrict__ vir, double* __restrict__ out)
+// (potential, kinetic) of each atom
+__global__ void k_energy_partials(int N, const int* __restrict__ type,
+                                  const double* __restrict__ mass, const double* __restrict__ pot,
+                                  const double* __restrict__ v, double* __restrict__ partial)
 {
-  __shared__ double part[4][7];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double q[2] = {0.0, 0.0};
+  if (i < N) {
+    q[0] = pot[i];
+    const double vv = v[i] * v[i] + v[N + i] * v[N + i] + v[2 * N + i] * v[2 * N + i];
+    q[1] = 0.5 * mass[type[i]] * vv * kKinetic;
+  }
+  block_partials<2>(q, partial);
+}
+
+// NPT: (m v_x^2, m v_y^2, m v_z^2 in eV, potential, virial xx, yy, zz) of each atom
+__global__ void k_npt_partials(int N, const int* __restrict__ type, const double* __restrict__ mass,
+                               const double* __restrict__ pot, const double* __restrict__ v,
+                               const double* __restrict__ vir, double* __restrict__ partial)
+{
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double q[7] = {0, 0, 0, 0, 0, 0, 0};
   if (i < N) {
@@ -82,15 +82,34 @@ __global__ void k_npt_reduce(int N, const int* __restrict__ type, const double* 
     q[3] = pot[i];
     for (int a = 0; a < 3; ++a) q[4 + a] = vir[static_cast<size_t>(a) * N + i];
   }
-  for (int k = 0; k < 7; ++k)
+  block_partials<7>(q, partial);
+}
+
+// Pass 2 (one block of 256 threads): out[k] = sum over blocks of partial[b * NV + k]; with a
+// device step counter the NV values go to out + NV * *step.
+template <int NV>
+__global__ __launch_bounds__(256) void k_finish(int nblocks, const double* __restrict__ partial,
+                                                double* __restrict__ out,
+                                                const int* __restrict__ step)
+{
+  double q[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) q[k] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) q[k] += partial[static_cast<size_t>(b) * NV + k];
+  __shared__ double part[4][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
     for (int off = 32; off > 0; off >>= 1) q[k] += __shfl_xor(q[k], off, 64);
   if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 7; ++k) part[threadIdx.x >> 6][k] = q[k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[threadIdx.x >> 6][k] = q[k];
   __syncthreads();
-  if (threadIdx.x < 7) {
-    double s = 0.0;
-    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) s += part[w][threadIdx.x];
-    unsafeAtomicAdd(out + threadIdx.x, s);
+  if (threadIdx.x < NV) {
+    double sum = 0.0;
+    for (int w = 0; w < 4; ++w) sum += part[w][threadIdx.x];
+    out[(step ? static_cast<size_t>(*step) * NV : 0) + threadIdx.x] = sum;
   }
 }
 
@@ -124,18 +143,24 @@ void launch_drift(int N, double dt, const double* v, double* x, hipStream_t s)
   k_drift<<<nblocks(N), 256, 0, s>>>(N, dt, v, x);
 }
 
+size_t md_partials_size(int N) { return 8 * static_cast<size_t>(nblocks(N)); }
+
 void launch_energies(int N, const int* type, const double* mass, const double* pot,
-                     const double* v, double* out, const int* step, hipStream_t s)
+                     const double* v, double* out, const int* step, double* partial,
+                     hipStream_t s)
 {
-  k_energies<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, out, step);
+  k_energy_partials<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, partial);
+  k_finish<2><<<1, 256, 0, s>>>(nblocks(N), partial, out, step);
 }
 
 void launch_step_advance(int* step, hipStream_t s) { k_step_advance<<<1, 64, 0, s>>>(step); }
 
 void launch_npt_reduce(int N, const int* type, const double* mass, const double* pot,
-                       const double* v, const double* vir, double* out, hipStream_t s)
+                       const double* v, const double* vir, double* out, double* partial,
+                       hipStream_t s)
 {
-  k_npt_reduce<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, vir, out);
+  k_npt_partials<<<nblocks(N), 256, 0, s>>>(N, type, mass, pot, v, vir, partial);
+  k_finish<7><<<1, 256, 0, s>>>(nblocks(N), partial, out, nullptr);
 }
 
 void launch_scale3(int N, double s0, double s1, double s2, double* x, hipStream_t s)

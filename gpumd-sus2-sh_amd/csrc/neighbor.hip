@@ -486,14 +486,19 @@ __global__ __launch_bounds__(64) void k_neighbors_staged(NeighborArgs a,
 __global__ void k_scatter(int N, const int* __restrict__ rank, const double* __restrict__ e_s,
                           const double* __restrict__ f_s, const double* __restrict__ v_s,
                           double* __restrict__ pot, double* __restrict__ force,
-                          double* __restrict__ vir)
+                          double* __restrict__ vir, int fixed)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int p = rank[i];
   pot[i] += e_s[p];
+  const long long* f_q = reinterpret_cast<const long long*>(f_s);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) force[static_cast<size_t>(c) * N + i] += f_s[static_cast<size_t>(c) * N + p];
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = static_cast<size_t>(c) * N + p;
+    force[static_cast<size_t>(c) * N + i] +=
+      fixed ? static_cast<double>(f_q[k]) * kFixInvScale : f_s[k];
+  }
 #pragma unroll
   for (int c = 0; c < 9; ++c) vir[static_cast<size_t>(c) * N + i] += v_s[static_cast<size_t>(c) * N + p];
 }
@@ -724,9 +729,10 @@ void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, i
 
 void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,
                     const double* v_s, double* potential, double* force, double* virial,
-                    hipStream_t s)
+                    int fixed, hipStream_t s)
 {
-  k_scatter<<<blocks_for(N, 256), 256, 0, s>>>(N, rank, e_s, f_s, v_s, potential, force, virial);
+  k_scatter<<<blocks_for(N, 256), 256, 0, s>>>(N, rank, e_s, f_s, v_s, potential, force, virial,
+                                               fixed);
 }
 
 }  // namespace sus2sh

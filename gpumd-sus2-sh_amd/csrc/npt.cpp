@@ -126,6 +126,7 @@ void Engine::md_npt(sus2sh_box& box, int N, const int* type, const double* masse
     if (!(masses[t] > 0.0)) throw ArgumentError("md_npt: masses must be positive");
   md_mass_.upload(masses, S * sizeof(double));
   md_red_.resize(8 * sizeof(double));
+  md_partial_.resize(md_partials_size(N) * sizeof(double));
   if (!host_red_) hip_ok(hipHostMalloc(reinterpret_cast<void**>(&host_red_), 8 * sizeof(double), 0),
                          "hipHostMalloc");
   const double* mass = md_mass_.as<double>();
@@ -161,8 +162,7 @@ void Engine::md_npt(sus2sh_box& box, int N, const int* type, const double* masse
     launch_fill_double(virial, 0.0, 9 * n, s);
     compute(box, N, type, pos, pot, force, virial, s, nullptr);
     if (kick) launch_half_kick(N, dt2, type, mass, force, vel, s);
-    launch_fill_double(red, 0.0, 8, s);
-    launch_npt_reduce(N, type, mass, pot, vel, virial, red, s);
+    launch_npt_reduce(N, type, mass, pot, vel, virial, red, md_partial_.as<double>(), s);
     hip_ok(hipMemcpyAsync(host_red_, red, 7 * sizeof(double), hipMemcpyDeviceToHost, s),
            "hipMemcpyAsync D2H");
     hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");

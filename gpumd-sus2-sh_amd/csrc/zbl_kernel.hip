@@ -16,8 +16,9 @@ __global__ __launch_bounds__(256) void k_zbl(int N, const int* __restrict__ coun
                                              const Edge<Real>* __restrict__ nl,
                                              const int* __restrict__ type_s, ZblTable t,
                                              double* __restrict__ e_s, double* __restrict__ f_s,
-                                             double* __restrict__ v_s)
+                                             double* __restrict__ v_s, int* __restrict__ fix_flag)
 {
+  unsigned long long* f_q = reinterpret_cast<unsigned long long*>(f_s);  // fix_flag != null
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const int ti = type_s[p];
@@ -64,7 +65,10 @@ __global__ __launch_bounds__(256) void k_zbl(int N, const int* __restrict__ coun
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       fs[a] += g[a];
-      unsafeAtomicAdd(f_s + static_cast<size_t>(a) * N + j, -g[a]);
+      if (fix_flag)
+        atomicAdd(f_q + static_cast<size_t>(a) * N + j, fix_encode(-g[a], kFixPairMax, fix_flag));
+      else
+        unsafeAtomicAdd(f_s + static_cast<size_t>(a) * N + j, -g[a]);
     }
     // xx yy zz xy xz yz yx zx zy
     w[0] -= d[0] * g[0];
@@ -81,7 +85,12 @@ __global__ __launch_bounds__(256) void k_zbl(int N, const int* __restrict__ coun
   if (!any) return;
   e_s[p] += e;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) unsafeAtomicAdd(f_s + static_cast<size_t>(a) * N + p, fs[a]);
+  for (int a = 0; a < 3; ++a) {
+    if (fix_flag)
+      atomicAdd(f_q + static_cast<size_t>(a) * N + p, fix_encode(fs[a], kFixSumMax, fix_flag));
+    else
+      unsafeAtomicAdd(f_s + static_cast<size_t>(a) * N + p, fs[a]);
+  }
 #pragma unroll
   for (int k = 0; k < 9; ++k) v_s[static_cast<size_t>(k) * N + p] += w[k];
 }
@@ -90,15 +99,16 @@ __global__ __launch_bounds__(256) void k_zbl(int N, const int* __restrict__ coun
 
 template <typename Real>
 void launch_zbl(int N, const int* count, const Edge<Real>* nl, const int* type_s,
-                const ZblTable& t, double* e_s, double* f_s, double* v_s, hipStream_t s)
+                const ZblTable& t, double* e_s, double* f_s, double* v_s, int* fix_flag,
+                hipStream_t s)
 {
   if (N <= 0) return;
-  k_zbl<Real><<<(N + 255) / 256, 256, 0, s>>>(N, count, nl, type_s, t, e_s, f_s, v_s);
+  k_zbl<Real><<<(N + 255) / 256, 256, 0, s>>>(N, count, nl, type_s, t, e_s, f_s, v_s, fix_flag);
 }
 
 template void launch_zbl<float>(int, const int*, const Edge<float>*, const int*, const ZblTable&,
-                                double*, double*, double*, hipStream_t);
+                                double*, double*, double*, int*, hipStream_t);
 template void launch_zbl<double>(int, const int*, const Edge<double>*, const int*,
-                                 const ZblTable&, double*, double*, double*, hipStream_t);
+                                 const ZblTable&, double*, double*, double*, int*, hipStream_t);
 
 }  // namespace sus2sh

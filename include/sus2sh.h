@@ -82,7 +82,12 @@ int sus2sh_probe(const char* path);
  * 8x2x160x2: fixes the lane kernel's full shape), sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
  * sus2_sh_deferred_check=<0|1> (steady calls make no host synchronisation and can be
  * captured in a HIP graph; see sus2sh_check), sus2_sh_md_graph=<0|1> (sus2sh_md_nve replays
- * captured steps, default 1).  Validated like the reference: sus2_sh_factor_pruning=
+ * captured steps, default 1), sus2_sh_deterministic=<0|1> (energies, forces and virials
+ * bit-identical between calls, handles and MD runs: force contributions summed in 64-bit
+ * fixed point, quantum 2^-34 eV/A, with pair terms below 131072 eV/A and at most 1023
+ * neighbours per atom, else SUS2SH_ERR_ARGUMENT; no autotuner; not for dE/dtheta or
+ * two-layer gate models; no counterpart in the reference, whose float force atomics vary
+ * between runs, docs/gpumd-sus2-sh-interface-plan.md:1099-1107).  Validated like the reference: sus2_sh_factor_pruning=
  * <legacy|q-total> (src/force/sus2_sh.cu:558-587; the file's graph is always evaluated as an
  * explicit graph) and sus2_radial_direct=<1> (0 is rejected, src/force/sus2_sh.cu:7914-7916).
  * Other sus2_* tokens are accepted and ignored.  Uses the current HIP device. */

@@ -124,6 +124,16 @@ def test_loader_requires_every_body_bucket_and_scalar_info(model_dir):
     _check_rejected(model_dir, text, "requires sh_scalar_info metadata")
 
 
+def test_deterministic_option_rejects_gate_models(model_dir):
+    """Deterministic summation covers the plain and ZBL force paths; the gate layer's dE/dh
+    scatter has no fixed-point form, so a handle asking for both fails at creation (before
+    any device call, so this runs without a GPU)."""
+    path = _write(model_dir, "gate_det", _gate_spec("mu-scalar-full")[1])
+    with pytest.raises(sus2sh.Sus2shError) as exc:
+        sus2sh.SUS2_SH(path, 8, ["Cu", "Zr", "sus2_sh_deterministic=1"])
+    assert "sus2_sh_deterministic=1 is not available for two-layer gate models" in str(exc.value)
+
+
 # ----------------------------------------------------------------------------- oracle
 def _struct(S=2, n=3, a=3.7, sigma=0.12, seed=1):
     return st.fcc_alloy(n, n, n, a, [1.0 / S] * S, sigma, seed, seed + 1)

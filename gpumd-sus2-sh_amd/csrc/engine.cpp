@@ -154,7 +154,17 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   model_ = load_model(path);
   if (model_.l_max > kMaxL)
     throw UnsupportedError("SUS2_SH: this engine evaluates sh_l_max <= 6 only.");
-  if (model_.k_max > kMaxK) throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 6.");
+  // k chunks of at most kMaxK (sizes differ by at most one)
+  kchunks_ = (model_.k_max + kMaxK - 1) / kMaxK;
+  if (kchunks_ > kMaxKChunks)
+    throw UnsupportedError("SUS2_SH: this engine supports sh_k_max <= 48.");
+  for (int c = 0, k0 = 0; c < kchunks_; ++c) {
+    kchunk_k0_[c] = k0;
+    kchunk_K_[c] = model_.k_max / kchunks_ + (c < model_.k_max % kchunks_ ? 1 : 0);
+    k0 += kchunk_K_[c];
+  }
+  if (model_.gate_enabled && kchunks_ > 1)
+    throw UnsupportedError("SUS2_SH: two-layer gate models are evaluated for sh_k_max <= 6 only.");
   if (model_.species_count > kMaxSpecies)
     throw UnsupportedError("SUS2_SH: this engine supports at most 16 species.");
   opt_ = parse_options(model_, nopts, opts);
@@ -188,9 +198,9 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   const auto padded = [](size_t n) { return n + 31; };
   // (rows of the uniform kernels are zero-padded to 8, 12 or 16 coefficients)
   const size_t row = model_.rb_size <= 8 ? 8 : (model_.rb_size <= 12 ? 12 : 16);
+  const size_t mr_chunk = static_cast<size_t>(kchunk_K_[0]) * (model_.l_max + 1);  // largest
   const size_t table = static_cast<size_t>(model_.species_count) * model_.species_count *
-                       (padded(static_cast<size_t>(model_.radial_funcs_count) * row) +
-                        padded(static_cast<size_t>(model_.radial_funcs_count) * 2)) * rsz;
+                       (padded(mr_chunk * row) + padded(mr_chunk * 2)) * rsz;
   if (table > 65536)
     throw UnsupportedError("SUS2_SH: radial tables exceed 64 KiB of LDS (too many species).");
 
@@ -330,12 +340,24 @@ void Engine::upload_tables()
   const GraphProgram& g = prog_;
   std::vector<double> coeff, scal;
   build_radial_tables(m, coeff, scal);
-  std::vector<Real> cR(coeff.begin(), coeff.end()), sR(scal.begin(), scal.end());
-  radial_coeff_.upload(cR.data(), cR.size() * sizeof(Real));
-  radial_scal_.upload(sR.data(), sR.size() * sizeof(Real));
+  const int L = m.l_max, K = m.k_max, Mr = m.radial_funcs_count, rb = m.rb_size;
+  const int pairs = m.species_count * m.species_count;
+  for (int c = 0; c < kchunks_; ++c) {
+    // the chunk's radial rows mu in [k0 (L+1), (k0 + Kc)(L+1)) of every pair
+    const int mu0 = kchunk_k0_[c] * (L + 1), mrc = kchunk_K_[c] * (L + 1);
+    std::vector<Real> cR(static_cast<size_t>(pairs) * mrc * rb), sR(static_cast<size_t>(pairs) * mrc * 2);
+    for (int p = 0; p < pairs; ++p)
+      for (int mu = 0; mu < mrc; ++mu) {
+        const size_t src = static_cast<size_t>(p) * Mr + mu0 + mu, dst = static_cast<size_t>(p) * mrc + mu;
+        for (int n = 0; n < rb; ++n) cR[dst * rb + n] = static_cast<Real>(coeff[src * rb + n]);
+        sR[dst * 2] = static_cast<Real>(scal[src * 2]);
+        sR[dst * 2 + 1] = static_cast<Real>(scal[src * 2 + 1]);
+      }
+    radial_coeff_[c].upload(cR.data(), cR.size() * sizeof(Real));
+    radial_scal_[c].upload(sR.data(), sR.size() * sizeof(Real));
+  }
   // uniform kernels (one map for every mu, Chebyshev rows) are for the Chebyshev bases
   radial_uniform_ = m.radial_kind != RadialKind::LaguerreLog1p;
-  const int pairs = m.species_count * m.species_count;
   for (int p = 0; p < pairs && radial_uniform_; ++p)
     for (int mu = 1; mu < m.radial_funcs_count; ++mu)
       if (scal[(p * m.radial_funcs_count + mu) * 2] != scal[(p * m.radial_funcs_count) * 2] ||
@@ -344,16 +366,21 @@ void Engine::upload_tables()
         break;
       }
 
-  // full (k,l,m) grid -> basic index
-  const int L = m.l_max, K = m.k_max;
-  std::vector<int> fmap(K * (L + 1) * (L + 1), -1);
-  for (int b = 0; b < m.basic_count; ++b) {
-    int off = 0;
-    for (int lp = L; lp > m.basic_l[b]; --lp) off += K * (2 * lp + 1);
-    const int f = off + (K - 1 - m.basic_k[b]) * (2 * m.basic_l[b] + 1) + m.basic_m[b] + m.basic_l[b];
-    fmap[f] = b;
+  // each chunk's full (k,l,m) grid -> basic index (the kernels' full_index<L, Kc> with the
+  // chunk-local k); a basic belongs to the chunk that holds its k
+  (void)K;
+  for (int c = 0; c < kchunks_; ++c) {
+    const int k0 = kchunk_k0_[c], Kc = kchunk_K_[c];
+    std::vector<int> fmap(Kc * (L + 1) * (L + 1), -1);
+    for (int b = 0; b < m.basic_count; ++b) {
+      const int k = m.basic_k[b] - k0;
+      if (k < 0 || k >= Kc) continue;
+      int off = 0;
+      for (int lp = L; lp > m.basic_l[b]; --lp) off += Kc * (2 * lp + 1);
+      fmap[off + (Kc - 1 - k) * (2 * m.basic_l[b] + 1) + m.basic_m[b] + m.basic_l[b]] = b;
+    }
+    fmap_[c].upload(fmap.data(), fmap.size() * sizeof(int));
   }
-  fmap_.upload(fmap.data(), fmap.size() * sizeof(int));
 
   const int rows = static_cast<int>(g.fwd_row_target.size());
   std::vector<int4> frows(rows);
@@ -929,19 +956,28 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   EdgeLaunch<Real> el{};
   el.N = N;
   el.L = model_.l_max;
-  el.K = model_.k_max;
+  el.K = kchunk_K_[0];
   el.B = model_.basic_count;
   el.count = ws_.count;
   el.nl = static_cast<const Edge<Real>*>(ws_.nl);
   el.type_s = ws_.type_s;
-  el.rt.coeff = radial_coeff_.as<Real>();
-  el.rt.scal = radial_scal_.as<Real>();
+  el.rt.coeff = radial_coeff_[0].as<Real>();
+  el.rt.scal = radial_scal_[0].as<Real>();
   el.rt.S = model_.species_count;
   el.rt.rb = model_.rb_size;
   el.rt.rc = static_cast<Real>(model_.max_dist);
   el.rt.kind = static_cast<int>(model_.radial_kind);  // 0 tanh, 1 rational, 2 Laguerre
   el.rt.uniform = radial_uniform_ ? 1 : 0;
-  el.fmap = fmap_.as<int>();
+  el.fmap = fmap_[0].as<int>();
+  // the edge launch of k chunk c (chunk 0 is el itself)
+  const auto chunk_launch = [&](int c) {
+    EdgeLaunch<Real> ec = el;
+    ec.K = kchunk_K_[c];
+    ec.rt.coeff = radial_coeff_[c].as<Real>();
+    ec.rt.scal = radial_scal_[c].as<Real>();
+    ec.fmap = fmap_[c].as<int>();
+    return ec;
+  };
   el.fix_flag = opt_.deterministic ? latch_.as<int>() + kLatchFix : nullptr;
   // two-layer gate (src/force/sus2_sh.cu:9821-9893): gate moments with the gate radial
   // coefficients, the gate layer's graph, G = 1 + A tanh(a h) per atom and channel; the main
@@ -974,8 +1010,10 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       throw UnsupportedError("SUS2_SH: gate graph tile does not fit in LDS.");
     el.gate_m = static_cast<const Real*>(ws_.gate_m);
   }
-  const bool basics_ok = launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s);
+  bool basics_ok = launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s);
   el.gate_m = nullptr;
+  for (int c = 1; c < kchunks_ && basics_ok; ++c)
+    basics_ok = launch_basics<Real>(chunk_launch(c), static_cast<Real*>(ws_.basics), s);
   if (!basics_ok)
     throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
   if (profile) hip_check(hipEventRecord(events_[2], s), "hipEventRecord");
@@ -1121,9 +1159,17 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     if (!launch_forces<Real>(e2, static_cast<const Real*>(ws_.ggrads), ws_.f_s, ws_.v_s, nullptr,
                              win, win_cap, s))
       throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
-  } else if (!launch_forces<Real>(el, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
-                           param_grad ? pg_rad_.as<double>() : nullptr, win, win_cap, s))
-    throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+  } else {
+    // k chunks after the first add their virials to the first's
+    for (int c = 0; c < kchunks_; ++c) {
+      EdgeLaunch<Real> ec = chunk_launch(c);
+      ec.v_accumulate = c > 0 ? 1 : 0;
+      if (!launch_forces<Real>(ec, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
+                               param_grad ? pg_rad_.as<double>() + pg_chunk_offset(c) : nullptr,
+                               win, win_cap, s))
+        throw UnsupportedError("SUS2_SH: no force kernel for this (l_max, k_max).");
+    }
+  }
   if (model_.zbl_enabled) {
     const ZblTable zt{zbl_z_.as<int>(), zbl_inner_.as<double>(), zbl_outer_.as<double>(),
                       model_.species_count};
@@ -1192,6 +1238,17 @@ void Engine::set_params(const double* theta, int n)
     upload_tables<float>();
   else
     upload_tables<double>();
+}
+
+size_t Engine::pg_chunk_offset(int c) const
+{
+  const size_t S = model_.species_count;
+  size_t off = 0;
+  for (int k = 0; k < c; ++k) {
+    const size_t mrc = static_cast<size_t>(kchunk_K_[k]) * (model_.l_max + 1);
+    off += S * S * (mrc * model_.rb_size + 2 * mrc);
+  }
+  return off;
 }
 
 std::string Engine::fix_range_message()
@@ -1385,8 +1442,7 @@ void Engine::finish_param_grad(double* d_param_grad, hipStream_t s)
 {
   const Model& m = model_;
   const int S = m.species_count, Q = m.scalar_count, Mr = m.radial_funcs_count, rb = m.rb_size;
-  const int stride = Mr * rb + 2 * Mr;
-  std::vector<double> misc(2 * S + Q), rad(static_cast<size_t>(S) * S * stride);
+  std::vector<double> misc(2 * S + Q), rad(static_cast<size_t>(S) * S * (Mr * rb + 2 * Mr));
   const int P = param_count();
   std::vector<double> grad(P);
   hip_check(hipMemcpyAsync(misc.data(), pg_misc_.as<double>(), misc.size() * sizeof(double),
@@ -1403,23 +1459,29 @@ void Engine::finish_param_grad(double* d_param_grad, hipStream_t s)
     grad[o_species + t] += misc[S + t];
   }
   for (int q = 0; q < Q; ++q) grad[o_w + q] += misc[2 * S + q];
-  for (int ti = 0; ti < S; ++ti)
-    for (int tj = 0; tj < S; ++tj) {
-      const double* D = rad.data() + static_cast<size_t>(ti * S + tj) * stride;
-      const double wi = m.radial_type_coeffs[ti], wj = m.radial_type_coeffs[tj];
-      for (int mu = 0; mu < Mr; ++mu) {
-        for (int n = 0; n < rb; ++n) {
-          const double d = D[mu * rb + n];
-          grad[o_rad + mu * rb + n] += d * m.scaling * wi * wj;
-          const double dv = d * m.scaling * m.radial_coeffs[mu * rb + n];
-          grad[o_type + ti] += dv * wj;
-          grad[o_type + tj] += dv * wi;
+  // per k chunk: [S*S][mrc*rb + 2*mrc] for its radial rows mu0 .. mu0 + mrc - 1
+  for (int c = 0; c < kchunks_; ++c) {
+    const int mu0 = kchunk_k0_[c] * (m.l_max + 1), mrc = kchunk_K_[c] * (m.l_max + 1);
+    const int stride = mrc * rb + 2 * mrc;
+    for (int ti = 0; ti < S; ++ti)
+      for (int tj = 0; tj < S; ++tj) {
+        const double* D = rad.data() + pg_chunk_offset(c) + static_cast<size_t>(ti * S + tj) * stride;
+        const double wi = m.radial_type_coeffs[ti], wj = m.radial_type_coeffs[tj];
+        for (int ml = 0; ml < mrc; ++ml) {
+          const int mu = mu0 + ml;
+          for (int n = 0; n < rb; ++n) {
+            const double d = D[ml * rb + n];
+            grad[o_rad + mu * rb + n] += d * m.scaling * wi * wj;
+            const double dv = d * m.scaling * m.radial_coeffs[mu * rb + n];
+            grad[o_type + ti] += dv * wj;
+            grad[o_type + tj] += dv * wi;
+          }
+          const size_t so = static_cast<size_t>(2) * mu * S * S + static_cast<size_t>(S) * ti + tj;
+          grad[o_scal + so] += D[mrc * rb + ml];
+          grad[o_scal + so + static_cast<size_t>(S) * S] += D[mrc * rb + mrc + ml];
         }
-        const size_t so = static_cast<size_t>(2) * mu * S * S + static_cast<size_t>(S) * ti + tj;
-        grad[o_scal + so] += D[Mr * rb + mu];
-        grad[o_scal + so + static_cast<size_t>(S) * S] += D[Mr * rb + Mr + mu];
       }
-    }
+  }
   hip_check(hipMemcpyAsync(d_param_grad, grad.data(), P * sizeof(double), hipMemcpyHostToDevice, s),
             "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");

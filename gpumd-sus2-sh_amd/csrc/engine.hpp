@@ -164,6 +164,8 @@ class Engine {
   // back-chunk length of the row programs: deterministic mode merges no chunk atomically
   int graph_chunk_terms() const { return opt_.deterministic ? (1 << 30) : 8; }
   static std::string fix_range_message();
+  // start of k chunk c's block in the radial parameter-gradient accumulators
+  size_t pg_chunk_offset(int c) const;
   template <typename Real>
   void upload_tables();
   template <typename Real>
@@ -214,7 +216,16 @@ class Engine {
   int device_ = 0;
 
   // model tables (precision-specific ones are uploaded for the active precision only)
-  DeviceBuffer radial_coeff_, radial_scal_, fmap_;
+  // The edge kernels are compiled for sh_k_max <= kMaxK; a larger k_max (up to the loader's
+  // radial_funcs_count <= 48) runs them on chunks of consecutive k: chunk c covers
+  // k in [kchunk_k0_[c], kchunk_k0_[c] + kchunk_K_[c]) with its own radial rows
+  // (mu = k (l_max + 1) + l, contiguous) and (k,l,m) grid map.  Moments and forces are sums
+  // over k, so the chunks' basics are disjoint and their forces add.
+  static constexpr int kMaxKChunks = 8;
+  int kchunks_ = 1;
+  int kchunk_k0_[kMaxKChunks] = {};
+  int kchunk_K_[kMaxKChunks] = {};
+  DeviceBuffer radial_coeff_[kMaxKChunks], radial_scal_[kMaxKChunks], fmap_[kMaxKChunks];
   DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
   DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;
   DeviceBuffer e_slot_, e_w_, seed_, shift_, sigma_;

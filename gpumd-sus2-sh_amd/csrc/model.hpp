@@ -25,7 +25,7 @@ struct InputError : public std::runtime_error {
 
 // Limits of this engine (the reference's own, src/force/sus2_sh.cu:29-33, 2273).
 constexpr int kMaxL = 6;              // sh_l_max in [0,6]
-constexpr int kMaxK = 6;
+constexpr int kMaxK = 6;              // per edge-kernel launch; larger k_max runs in k chunks
 constexpr int kMaxRadialFuncs = 48;   // sus2_sh.cu:31
 constexpr int kMaxRbSize = 16;        // sus2_sh.cu:32
 constexpr int kMaxBasics = 320;       // sus2_sh.cu:33

@@ -67,6 +67,14 @@ CASES = {
     "l6k1_6611": dict(model=dict(l_max=6, k_max=1, body_order=5, body_l_max=(6, 6, 1, 1),
                                  rb_size=10, max_dist=5.0, seed=17, scaling=0.5),
                       struct=("fcc", 4, 3.85, 0.08)),
+    # k_max > 6: the edge kernels run on k chunks (8 = 4 + 4, 13 = 5 + 4 + 4; radial
+    # funcs 32 and 39 of the loader's 48)
+    "l3k8_chunks": dict(model=dict(l_max=3, k_max=8, body_order=4, rb_size=8, max_dist=5.0,
+                                   seed=19, scaling=0.4),
+                        struct=("fcc", 4, 3.85, 0.08)),
+    "l2k13_chunks": dict(model=dict(l_max=2, k_max=13, body_order=3, rb_size=6, max_dist=4.6,
+                                    seed=20, scaling=0.4, scal_mode="random"),
+                         struct=("fcc", 4, 3.7, 0.08)),
 }
 
 
@@ -176,13 +184,22 @@ def test_invalid_model_reports_error(tmp_path):
     assert "Missing token" in err.value.message
 
 
+PG_MODELS = {
+    "l3k2": dict(l_max=3, k_max=2, body_order=5, rb_size=6, max_dist=4.8, seed=41,
+                 scal_mode="random", species_count=3, scaling=0.5),
+    # k chunks 4 + 4: the radial adjoints of each chunk land in its own block
+    "l2k8": dict(l_max=2, k_max=8, body_order=3, rb_size=6, max_dist=4.8, seed=42,
+                 scal_mode="random", species_count=3, scaling=0.4),
+}
+
+
 @pytest.mark.parametrize("fp32", [False, True])
-def test_parameter_gradient_matches_oracle(make_model, fp32):
+@pytest.mark.parametrize("model", sorted(PG_MODELS))
+def test_parameter_gradient_matches_oracle(make_model, fp32, model):
     """dE/dtheta over every model coefficient (no reference counterpart) against the FP64
     oracle's adjoint; the oracle's adjoint is itself pinned by finite differences on CPU."""
     import torch
-    path = make_model("l3k2_pg", l_max=3, k_max=2, body_order=5, rb_size=6, max_dist=4.8, seed=41,
-                      scal_mode="random", species_count=3, scaling=0.5)
+    path = make_model(model + "_pg", **PG_MODELS[model])
     pos, types, cell = st.fcc_alloy(3, 3, 3, 3.7, [0.4, 0.4, 0.2], 0.08, 5, 6)
     ref = Oracle(path).compute(cell, types, pos, param_grad=True)
     pg_ref = ref[3]
@@ -190,7 +207,7 @@ def test_parameter_gradient_matches_oracle(make_model, fp32):
     P = pot.info()["param_count"]
     assert P == len(pg_ref)
     N = len(types)
-    print(f"[case] param grad N={N} P={P} fp32={fp32}", flush=True)
+    print(f"[case] param grad {model} N={N} P={P} fp32={fp32}", flush=True)
     t = torch.as_tensor(types, device="cuda")
     x = torch.as_tensor(pos, device="cuda")
     e = torch.zeros(N, dtype=torch.float64, device="cuda")

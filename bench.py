@@ -28,7 +28,10 @@ REFERENCE_A100 = 5.23633e6  # l3k3 1.024M compact serial product, README.md:152 
 # latest l3333 (rb 10, rc 7.5) 1.024M figure, docs/gpumd-sus2-sh-interface-plan.md:1409
 REFERENCE_A100_L3333 = 1.19500e7
 PEAK_FP32_TFLOPS = 157.3  # published FP32 vector peak (counts packed FP32 instructions)
-PEAK_FP32_UNPACKED_TFLOPS = 78.6  # 256 CU x 4 SIMD x 16 lanes x 2 x 2.4 GHz, one FMA per lane
+# non-packed FP32 FMA rate measured on MI355X by gpumd-sus2-sh_amd/calibration/valu_rate.hip:
+# v_fmac_f32 sustains 2.885 cycles per wave64 instruction -> 1024 SIMDs x 64 lanes x 2 / 2.885
+# x 2.4 GHz (this build disables packed FP32, which the published 157.3 counts)
+PEAK_FP32_NONPACKED_FMA_TFLOPS = 109.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -333,7 +336,7 @@ def main():
         a = flops[k] * N / (stages[k] * 1e-3) / 1e12
         per_kernel[k] = {"launch_ms": stages[k], "flops_per_atom": flops[k], "tflops": a,
                          "frac_of_157_3": a / PEAK_FP32_TFLOPS,
-                         "frac_of_unpacked_78_6": a / PEAK_FP32_UNPACKED_TFLOPS}
+                         "frac_of_nonpacked_fma_109": a / PEAK_FP32_NONPACKED_FMA_TFLOPS}
 
     result = {
         "metric": "atom-steps/s at Cu-Zr l3k3 1.024M (FP32 vs FP64 path); max force err vs FP64",
@@ -368,13 +371,13 @@ def main():
         # the dominant kernel is FP32 vector-ALU / LDS work: it issues no matrix-core (MFMA)
         # instruction, so it is bound by VALU issue and priced against the published FP32
         # vector peak (157.3 TFLOP/s, which counts packed FP32); this build disables packed
-        # FP32, so the unpacked one-FMA-per-lane peak (78.6) is reported beside it.  traffic =
+        # FP32, so the measured non-packed FMA rate (109) is reported beside it.  traffic =
         # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/)
         "roofline": {"bound": "valu", "kernel": kernel_name, "stage": dominant,
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "peak_unpacked_fp32": PEAK_FP32_UNPACKED_TFLOPS,
-                     "frac_of_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "peak_nonpacked_fma_measured": PEAK_FP32_NONPACKED_FMA_TFLOPS,
+                     "frac_of_nonpacked_fma": achieved / PEAK_FP32_NONPACKED_FMA_TFLOPS,
                      "flops_per_atom": flops[dominant],
                      "launch_ms": stages[dominant]},
         "roofline_per_kernel_fp32": per_kernel,

@@ -5,6 +5,9 @@
   python profile_summary.py traffic <fetch_counter_collection.csv>
          <write_counter_collection.csv> --kernel <substring> --copy-bytes <bytes>
          [--atoms N] [--command "..."]
+  python profile_summary.py steady <kernel_trace.csv> [--last 4]
+  python profile_summary.py counters <counter_collection.csv> [--last 4] [--simds 1024]
+         [--xcds 8]
 
 `stats` prints the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (calls,
 average / total duration, share of GPU time).  `traffic` reads one `--pmc FETCH_SIZE` and one
@@ -12,7 +15,14 @@ average / total duration, share of GPU time).  `traffic` reads one `--pmc FETCH_
 contains --kernel, and scales each by the factor that makes the plain copy kernel of the same
 run (bench.py --copy-check) read and write its known --copy-bytes: HBM bytes per launch =
 counter * copy_bytes / counter(copy).  It prints a JSON object that bench.py can report as
-roofline.traffic.
+roofline.traffic.  `steady` averages the last --last dispatches of every kernel of a kernel
+trace (the steady evaluations after the autotuner's trial launches).  `counters` does the same
+for a `--pmc` run of SQ_WAVES, SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_INSTS_SALU, SQ_INSTS_SMEM and
+GRBM_GUI_ACTIVE: the SQ counters are totals over the device (wave-instructions), and
+GRBM_GUI_ACTIVE is summed over the XCDs, so one XCD's busy cycles are GRBM / --xcds.  It
+reports per dispatch the instructions per wave and the issue rates per SIMD-cycle (VALU) and
+per CU-cycle (LDS, SALU, SMEM); "VALU x4" is the fraction of SIMD cycles that the VALU
+instructions would fill at 4 cycles per wave64 instruction.
 """
 import argparse
 import collections
@@ -86,6 +96,51 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes, atoms, command):
     print(json.dumps(out, indent=1))
 
 
+def steady(path, last):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    acc = collections.defaultdict(list)
+    for r in rows:
+        acc[r["Kernel_Name"]].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"]), int(r["Scratch_Size"]),
+             int(r["VGPR_Count"]), int(r["LDS_Block_Size"])))
+    print(f"{'kernel':72s} {'calls':>5s} {'steady us':>10s} {'VGPR':>5s} {'scratch':>7s} "
+          f"{'LDS':>7s}")
+    for name, v in sorted(acc.items(), key=lambda kv: -sum(x[0] for x in kv[1][-last:])):
+        tail = v[-last:]
+        us = sum(x[0] for x in tail) / len(tail) / 1e3
+        if us < 20:
+            continue
+        print(f"{name[:72]:72s} {len(v):5d} {us:10.1f} {tail[-1][2]:5d} {tail[-1][1]:7d} "
+              f"{tail[-1][3]:7d}")
+
+
+def counters(path, last, simds, xcds):
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        per[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    cus = simds // 4
+    print(f"{'kernel':72s} {'VALU/wave':>9s} {'LDS/wave':>8s} {'SALU/wave':>9s} "
+          f"{'SMEM/wave':>9s} {'us@2.4':>7s} {'VALU x4':>7s} {'LDS/CUc':>7s} {'SALU/CUc':>8s}")
+    rows = []
+    for name, d in per.items():
+        if "GRBM_GUI_ACTIVE" not in d or "SQ_WAVES" not in d:
+            continue
+        m = {c: sum(v[-last:]) / len(v[-last:]) for c, v in d.items()}
+        cyc = m["GRBM_GUI_ACTIVE"] / xcds
+        if cyc < 20e3:
+            continue
+        w = max(m["SQ_WAVES"], 1.0)
+        rows.append((cyc, name, m, w))
+    for cyc, name, m, w in sorted(rows, key=lambda r: -r[0]):
+        print(f"{name[:72]:72s} {m.get('SQ_INSTS_VALU', 0) / w:9.0f} "
+              f"{m.get('SQ_INSTS_LDS', 0) / w:8.0f} {m.get('SQ_INSTS_SALU', 0) / w:9.0f} "
+              f"{m.get('SQ_INSTS_SMEM', 0) / w:9.0f} {cyc / 2400.0:7.0f} "
+              f"{4.0 * m.get('SQ_INSTS_VALU', 0) / (simds * cyc):7.2f} "
+              f"{m.get('SQ_INSTS_LDS', 0) / (cus * cyc):7.3f} "
+              f"{m.get('SQ_INSTS_SALU', 0) / (cus * cyc):8.3f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -99,9 +154,21 @@ def main():
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--atoms", type=int, default=0)
     t.add_argument("--command", default="", help="the profiled command, recorded in the output")
+    d = sub.add_parser("steady")
+    d.add_argument("csv")
+    d.add_argument("--last", type=int, default=4)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--last", type=int, default=4)
+    c.add_argument("--simds", type=int, default=1024)
+    c.add_argument("--xcds", type=int, default=8)
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.csv, a.top)
+    elif a.cmd == "steady":
+        steady(a.csv, a.last)
+    elif a.cmd == "counters":
+        counters(a.csv, a.last, a.simds, a.xcds)
     else:
         traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes, a.atoms, a.command)
 

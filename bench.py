@@ -245,6 +245,49 @@ def c3ref(nx, ny, nz, steps):
             "fp32_vs_reference_a100_l3333": r32["atom_steps_per_s"] / REFERENCE_A100_L3333}
 
 
+def c5_batch(evals):
+    """The batched-structure workload: 4,096 triclinic Cu-Zr cells of 8..256 atoms (543,189
+    atoms), l3k3 body 5, one sus2sh_compute_batch call per evaluation with and without
+    dE/dtheta (FP32 and FP64).  Boxes are built once; each call includes the host's cell
+    grids and uploads."""
+    import numpy as np
+    import torch
+    import sus2sh
+    from sus2sh import configs
+    cells, pos, types, offs = configs.batch_c5()
+    path = sus2sh.write_mtp(configs.model_c5(), os.path.join(
+        tempfile.mkdtemp(prefix="sus2sh_c5_"), "c5_l3k3.mtp"))
+    n, ns = len(types), len(cells)
+    boxes = sus2sh.box_array(cells)
+    out = {"structures": ns, "atoms": n, "model": "l3333 body5 rb8 rc5.0 seed5003"}
+    for prec in ("fp32", "fp64"):
+        pot = sus2sh.SUS2_SH(path, n, ["Cu", "Zr", "sus2_float=%d" % (prec == "fp32")])
+        t = torch.as_tensor(types, device="cuda")
+        x = torch.as_tensor(pos, device="cuda").contiguous()
+        e = torch.zeros(n, dtype=torch.float64, device="cuda")
+        f = torch.zeros(3, n, dtype=torch.float64, device="cuda")
+        v = torch.zeros(9, n, dtype=torch.float64, device="cuda")
+        g = torch.zeros(pot.info()["param_count"], dtype=torch.float64, device="cuda")
+        for pg in (False, True):
+            def call():
+                sus2sh.compute_batch(pot, boxes, offs, t, x, e, f, v,
+                                     param_grad=g if pg else None)
+            call()
+            call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(evals):
+                call()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / evals
+            key = f"{prec}{'_param_grad' if pg else ''}"
+            out[key + "_ms"] = dt * 1e3
+            out[key + "_atom_evals_per_s"] = n / dt
+            out[key + "_structures_per_s"] = ns / dt
+        pot.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +302,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c3ref-steps", type=int, default=10,
                     help="FP32 steps of the reference-shaped model run (rc 7.5, rb 10; 0: skip)")
+    ap.add_argument("--c5-evals", type=int, default=5,
+                    help="timed batched evaluations of the 4,096-cell workload (0: skip)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before the runs, copy MIB MiB device-to-device (untimed): the plain "
                          "copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -325,6 +370,11 @@ def main():
         log(f"[bench] reference-shaped model (rc 7.5, rb 10): {args.c3ref_steps} FP32 steps")
         ref_shape = c3ref(nx, ny, nz, args.c3ref_steps)
         log(f"[bench] c3ref {ref_shape}")
+    c5 = None
+    if args.c5_evals > 0:
+        log(f"[bench] batched 4,096-cell workload: {args.c5_evals} evaluations per mode")
+        c5 = c5_batch(args.c5_evals)
+        log(f"[bench] c5 {c5}")
     cpu, oracle_err, oracle_forces = None, None, None
     if not args.no_cpu:
         log("[bench] CPU baseline: FP64 oracle on the full structure (also the error reference)")
@@ -384,6 +434,7 @@ def main():
         "errors_vs_fp64_oracle": oracle_err,
         "oracle_force_scale": oracle_forces,
         "reference_shape_c3ref": ref_shape,
+        "batch_c5": c5,
         "graph_replay_fp32": graph32,
         "cpu_baseline": cpu,
         "vs_baseline_note": "reference A100 figure 5.23633e6 is a whole GPUMD NPT step with an "

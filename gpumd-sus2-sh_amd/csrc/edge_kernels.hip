@@ -125,14 +125,17 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
 // Adjoint of the radial coefficients for one edge: with A_mu = sum_m G_{k,l,m} Y_lm,
 //   dE/dc^p_{mu n} += A_mu (r - rc)^2 T_n(xi_mu),
 //   dE/da^p_mu     += A_mu (r - rc)^2 sum_n c_n T'_n(xi_mu) dxi/da   (same for s),
-// accumulated per pair p in LDS (double).  No reference counterpart (training path).
+// accumulated per pair p in LDS (double), entry i of the pair at D[i * cp]: the caller's lane
+// adds into its own one of cp interleaved copies, so that the lanes of a wave, which mostly
+// hit the same few pairs, do not serialise on one address.  No reference counterpart
+// (training path).
 template <typename Real, int MR>
 __device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ coeff,
                                                      const Real* __restrict__ scal, int rb,
                                                      Real rc, int kind, Real r,
                                                      const Real (&A)[MR], double* __restrict__ D,
                                                      double* __restrict__ Da,
-                                                     double* __restrict__ Ds)
+                                                     double* __restrict__ Ds, int cp)
 {
   const Real env = (r - rc) * (r - rc);
 #pragma unroll
@@ -151,24 +154,24 @@ __device__ __forceinline__ void radial_param_adjoint(const Real* __restrict__ co
     const Real* c = coeff + mu * rb;
     const Real w = A[mu] * env;
     Real t0 = 1, t1 = xi, d0 = 0, d1 = 1;
-    atomicAdd(D + mu * rb, static_cast<double>(w));
+    atomicAdd(D + (mu * rb) * cp, static_cast<double>(w));
     Real dsum = 0;
     if (rb > 1) {
-      atomicAdd(D + mu * rb + 1, static_cast<double>(w * xi));
+      atomicAdd(D + (mu * rb + 1) * cp, static_cast<double>(w * xi));
       dsum = c[1];
     }
     for (int n = 2; n < rb; ++n) {
       const Real t2 = Real(2) * xi * t1 - t0;
       const Real d2 = Real(2) * (t1 + xi * d1) - d0;
-      atomicAdd(D + mu * rb + n, static_cast<double>(w * t2));
+      atomicAdd(D + (mu * rb + n) * cp, static_cast<double>(w * t2));
       dsum += c[n] * d2;
       t0 = t1;
       t1 = t2;
       d0 = d1;
       d1 = d2;
     }
-    atomicAdd(Da + mu, static_cast<double>(w * dsum * Real(0.5) * (r - s) * dxdz));
-    atomicAdd(Ds + mu, static_cast<double>(w * dsum * Real(-0.5) * a * dxdz));
+    atomicAdd(Da + mu * cp, static_cast<double>(w * dsum * Real(0.5) * (r - s) * dxdz));
+    atomicAdd(Ds + mu * cp, static_cast<double>(w * dsum * Real(-0.5) * a * dxdz));
   }
 }
 
@@ -222,8 +225,9 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   const size_t pg_off = ((static_cast<size_t>(S) * S * (cs + ss) * sizeof(Real)) + 15) & ~size_t(15);
   double* s_pg = reinterpret_cast<double*>(smem_raw + pg_off);
   const int pg_stride = MR * rb + 2 * MR;
+  const int pg_cp = a.pg_copies;  // interleaved copies of the accumulators (power of two)
   if (PG) {
-    for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) s_pg[i] = 0.0;
+    for (int i = threadIdx.x; i < S * S * pg_stride * pg_cp; i += blockDim.x) s_pg[i] = 0.0;
   }
   // WIN: window table (first atom, prefix size) and the [3][total] accumulators
   int* w_first = reinterpret_cast<int*>(smem_raw + pg_off);
@@ -345,9 +349,10 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
           A[kk * (L + 1) + l] = acc * nl;
         }
       }
-      double* Dp = s_pg + pair * pg_stride;
+      double* Dp = s_pg + static_cast<size_t>(pair) * pg_stride * pg_cp + (threadIdx.x & (pg_cp - 1));
       radial_param_adjoint<Real, MR>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
-                                     a.rt.kind, r, A, Dp, Dp + MR * rb, Dp + MR * rb + MR);
+                                     a.rt.kind, r, A, Dp, Dp + MR * rb * pg_cp,
+                                     Dp + (MR * rb + MR) * pg_cp, pg_cp);
     }
     const Real g0 = ux * sv + gx * inv;
     const Real g1 = uy * sv + gy * inv;
@@ -433,7 +438,8 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   if (PG) {
     __syncthreads();
     for (int i = threadIdx.x; i < S * S * pg_stride; i += blockDim.x) {
-      const double v = s_pg[i];
+      double v = 0.0;
+      for (int c = 0; c < pg_cp; ++c) v += s_pg[static_cast<size_t>(i) * pg_cp + c];
       if (v != 0.0) unsafeAtomicAdd(pg_rad + i, v);
     }
   }
@@ -499,7 +505,7 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
     lds = ((lds + 15) & ~size_t(15)) +
-          sizeof(double) * a.rt.S * a.rt.S * (Mr * a.rt.rb + 2 * Mr);
+          sizeof(double) * a.rt.S * a.rt.S * (Mr * a.rt.rb + 2 * Mr) * a.pg_copies;
   }
   if (use_win)
     lds = ((lds + 15) & ~size_t(15)) + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16) +

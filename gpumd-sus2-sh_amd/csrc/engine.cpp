@@ -1115,7 +1115,12 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     tuned_ = true;
   }
   if (param_grad) {
-    // the parameter gradient needs per-scalar sums: persistent row-mode kernel
+    // the parameter gradient needs per-scalar sums (dE/dw_s = sum_i sigma_i s_i): the
+    // persistent row-mode kernel; with a lane program the lane kernel writes the energies
+    // and adjoints and the row kernel only runs the forward pass for the sums
+    pgo.fwd_only = lane_.ok ? 1 : 0;
+    if (lane_.ok && !launch_graph_variant<Real>(3, tile_, N, gt, s))
+      throw UnsupportedError("SUS2_SH: graph kernel variant does not fit in LDS.");
     if (!launch_graph<Real>(tile_, 0, N, static_cast<const Real*>(ws_.basics), ws_.type_s, gt,
                             ws_.e_s, static_cast<Real*>(ws_.grads), &pgo, s, max_lds_,
                             4 * cu_count_))
@@ -1164,6 +1169,15 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     for (int c = 0; c < kchunks_; ++c) {
       EdgeLaunch<Real> ec = chunk_launch(c);
       ec.v_accumulate = c > 0 ? 1 : 0;
+      if (param_grad) {
+        // copies of the per-pair dE/dtheta accumulators in LDS (lanes spread over them):
+        // as many as fit 72 KiB, at most 16
+        const size_t mrc = static_cast<size_t>(kchunk_K_[c]) * (model_.l_max + 1);
+        const size_t per_copy = sizeof(double) * model_.species_count * model_.species_count *
+                                (mrc * model_.rb_size + 2 * mrc);
+        ec.pg_copies = 1;
+        while (ec.pg_copies < 16 && 2 * ec.pg_copies * per_copy <= 72 * 1024) ec.pg_copies *= 2;
+      }
       if (!launch_forces<Real>(ec, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
                                param_grad ? pg_rad_.as<double>() + pg_chunk_offset(c) : nullptr,
                                win, win_cap, s))

@@ -13,7 +13,8 @@
 //
 // With PG (parameter gradient) the workgroups are persistent: each walks tiles
 // blockIdx.x, blockIdx.x + gridDim.x, ... and keeps dE/dw_s, dE/dsigma, dE/dshift in LDS
-// (double) until a single flush at the end.
+// (double) until a single flush at the end; with pg.fwd_only it runs the forward pass and
+// these sums only, beside a faster kernel that writes the energies and adjoints.
 #include "kernels.hpp"
 
 namespace sus2sh {
@@ -63,6 +64,7 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
   if (PG) {
     for (int k = tid; k < pg.Q + 2 * pg.S; k += NT) pw[k] = 0.0;
   }
+  const bool reverse = !(PG && pg.fwd_only);
 
   for (int tile = blockIdx.x; tile < tiles; tile += (PG ? gridDim.x : tiles)) {
     const int a0 = tile * W;
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
           sum += static_cast<double>(reinterpret_cast<Real*>(red)[w * W + tid]);
         const int t = type_s[atom];
         const double sg = gt.sigma[t];
-        e_out[atom] = gt.shift[t] + sg + sg * sum;
+        if (reverse) e_out[atom] = gt.shift[t] + sg + sg * sum;
         if (PG) {
           atomicAdd(pw + pg.Q + t, 1.0);
           atomicAdd(pw + pg.Q + pg.S + t, 1.0 + sum);
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
     }
 
     // reverse, layer by layer (descending)
-    if (VM) {
+    if (VM && reverse) {
       for (int layer = gt.layers - 1; layer >= 0; --layer) {
         const int L = gt.bwd_slen[layer];
         const Term<Real>* ins = gt.bwd_ins + gt.bwd_sbegin[layer] + tid;
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
         __syncthreads();
       }
     }
-    for (int layer = gt.layers - 1; !VM && layer >= 0; --layer) {
+    for (int layer = gt.layers - 1; !VM && reverse && layer >= 0; --layer) {
       const int ce = gt.bwd_layer[layer + 1];
       for (int c = gt.bwd_layer[layer] + tid; c < ce; c += NT) {
         const int4 ch = gt.bwd_chunks[c];
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_graph(int N, const Real* __restrict__ ba
     }
 
     // sigma-scaled basic adjoints
-    for (int idx = tid; idx < gt.basics * W; idx += NT) {
+    for (int idx = tid; reverse && idx < gt.basics * W; idx += NT) {
       const int b = idx / W, a = idx - b * W;
       const int atom = a0 + a;
       if (atom < N) {

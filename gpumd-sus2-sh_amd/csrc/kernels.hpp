@@ -68,6 +68,9 @@ struct ParamGradOut {
   int n_stored;
   int Q;
   int S;
+  // 1: the forward pass and the sums above only (site energies and basic adjoints come from
+  // another graph kernel); 0: also the site energies and the reverse pass
+  int fwd_only;
 };
 
 // One cell-sorted atom for the neighbour search: wrapped position and species (as a
@@ -172,6 +175,8 @@ struct EdgeLaunch {
   // deterministic summation (null otherwise): f_s holds 64-bit fixed-point sums
   // (fix_encode); *fix_flag is set when a contribution is out of range
   int* fix_flag = nullptr;
+  // parameter gradient: interleaved copies of the per-pair LDS accumulators (power of two)
+  int pg_copies = 1;
 };
 
 template <typename Real>

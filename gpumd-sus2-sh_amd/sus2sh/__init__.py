@@ -5,7 +5,7 @@ The compute path is libsus2sh.so (HIP kernels for gfx950); this package only bin
 from ._capi import EXPORTED_SYMBOLS, LIB_PATH, Box, Sus2shError, lib, last_error  # noqa: F401
 from .model_gen import (ModelSpec, StandardGraph, gate_model, param_vector, standard_graph,  # noqa: F401
                         subgraph, synthetic_model, with_params, write_mtp)
-from .potential import (SUS2_SH, box_struct, compute_batch, evaluate,  # noqa: F401
+from .potential import (SUS2_SH, box_array, box_struct, compute_batch, evaluate,  # noqa: F401
                         maxwell_velocities, md_npt_mttk, md_nve)
 from . import structures  # noqa: F401
 from . import gpumd_io  # noqa: F401

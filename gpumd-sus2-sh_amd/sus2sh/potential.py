@@ -147,14 +147,24 @@ class SUS2_SH:
         check(code, self._h)
 
 
+def box_array(cells, pbc=(1, 1, 1)):
+    """ctypes array of Box structs for compute_batch (built once, reusable across calls)."""
+    boxes = (Box * len(cells))()
+    for k, c in enumerate(cells):
+        boxes[k] = box_struct(c, pbc)
+    return boxes
+
+
 def compute_batch(pot, cells, atom_offsets, type, position, potential, force, virial,
                   pbc=(1, 1, 1), param_grad=None, stream=None):
     """Batched evaluation (sus2sh_compute_batch) of independent periodic structures whose atoms
-    are concatenated in the SoA arrays; cells: list of 3x3 (rows = cell vectors)."""
-    ns = len(cells)
-    boxes = (Box * ns)()
-    for k, c in enumerate(cells):
-        boxes[k] = box_struct(c, pbc)
+    are concatenated in the SoA arrays; cells: list of 3x3 (rows = cell vectors), or the
+    result of box_array(cells, pbc) (pbc is then ignored)."""
+    if isinstance(cells, ctypes.Array):
+        boxes, ns = cells, len(cells)
+    else:
+        ns = len(cells)
+        boxes = box_array(cells, pbc)
     offs = np.ascontiguousarray(atom_offsets, dtype=np.int32)
     assert len(offs) == ns + 1
     s = _stream(type, stream)

@@ -11,6 +11,8 @@ C4  Cu-Zr-Al l4k4 (rb 10, RBChebyshev_sss_rational, rc 5.5, S = 3, 45/45/10) on 
     gradient, and the FP64 engine's forces and dE/dtheta against central differences of its
     own total energy (h = 1e-4) on 6 coordinates and 8 parameters.  The rational map's
     parameter adjoint (xi = z / sqrt(1 + z^2)) runs here.
+C5  48 cells of the batched workload's generator (triclinic, 8..256 atoms, multi-image) in
+    one batched call with dE/dtheta, both precisions, against the oracle cell by cell.
 
 Tolerances: FP64 E 1e-9 eV/atom, F 1e-8 eV/A, W 1e-8 eV, dE/dtheta 1e-9 of max |dE/dtheta|;
 FP32 E 2e-5 eV/atom + 2e-6 |E|, F 1e-4 eV/A + 2e-5 |F|max, W 1e-4 eV + 2e-5 |W|max,
@@ -180,3 +182,35 @@ def test_c4_central_differences(c4_case):
         assert abs(g_a[k] - fd) <= 1e-5 * max(1.0, abs(fd))
     pot.set_params(theta)
     pot.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_c5_batch_efw_and_param_grad(model_dir, fp32):
+    """C5-shaped batch (48 triclinic cells of 8..256 atoms from the C5 generator, most thinner
+    than 2 rc, so multi-image neighbours) in one sus2sh_compute_batch call with dE/dtheta:
+    per-atom E/F/W against the oracle cell by cell, and dE/dtheta against the sum of the
+    oracle's per-cell gradients."""
+    import torch
+    path = sus2sh.write_mtp(configs.model_c5(), str(model_dir / "c5.mtp"))
+    cells, pos, types, offs = configs.batch_c5(count=48, seed=5101)
+    n = len(types)
+    print(f"[case] C5 batch of {len(cells)} cells, N={n}, fp32={fp32}", flush=True)
+    orc = Oracle(path)
+    refs = [orc.compute(c, types[offs[k]:offs[k + 1]], pos[:, offs[k]:offs[k + 1]],
+                        param_grad=True) for k, c in enumerate(cells)]
+    pot = _engine(path, n, 2, fp32)
+    t, x = _dev(types, np.int32), _dev(pos, np.float64)
+    e = torch.zeros(n, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3, n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9, n, dtype=torch.float64, device="cuda")
+    g = torch.zeros(pot.info()["param_count"], dtype=torch.float64, device="cuda")
+    sus2sh.compute_batch(pot, sus2sh.box_array(cells), offs, t, x, e, f, v, param_grad=g)
+    torch.cuda.synchronize()
+    pot.close()
+    got = (e.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy())
+    ref = tuple(np.concatenate([r[i] for r in refs], axis=-1) for i in range(3))
+    _check_efw(ref, got, fp32)
+    g0 = np.sum([r[3] for r in refs], axis=0)
+    err = np.abs(g.cpu().numpy() - g0).max() / np.abs(g0).max()
+    print(f"  dE/dtheta: {len(g0)} parameters, max rel err {err:.2e}", flush=True)
+    assert err <= (1e-4 if fp32 else 1e-9)

@@ -363,9 +363,9 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot,
                       int atoms, int n, const double* basics, double* e_prime, double* grad,
                       int* info)
 {
-  if (nslot == 0) nslot = waves == 8 ? 2 : 4;
+  if (nslot == 0) nslot = waves >= 8 ? 2 : 4;
   if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
-      (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
+      (waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16) ||
       (nslot != 1 && nslot != 2 && nslot != 4 && nslot != 8) ||
       (real_bytes != 0 && real_bytes != 4 && real_bytes != 8) || (atoms != 1 && atoms != 2))
     return SUS2SH_ERR_ARGUMENT;

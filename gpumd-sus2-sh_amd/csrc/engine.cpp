@@ -89,12 +89,12 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.graph_variant = v;
     } else if (key == "sus2_sh_lane_waves") {
       const int w = std::atoi(val.c_str());
-      if (w != 1 && w != 2 && w != 4 && w != 8)
-        throw InputError("sus2_sh_lane_waves must be 1, 2, 4 or 8: " + val);
-      // fixes the lane program shape: 2 accumulators per wave with 8 waves, else 4; the
-      // smallest slot budget that holds the program, from 160 up
+      if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16)
+        throw InputError("sus2_sh_lane_waves must be 1, 2, 4, 8 or 16: " + val);
+      // fixes the lane program shape: 2 accumulators per wave with 8 or 16 waves, else 4;
+      // the smallest slot budget that holds the program, from 160 up
       o.lane_waves = w;
-      o.lane_nslot = w == 8 ? 2 : 4;
+      o.lane_nslot = w >= 8 ? 2 : 4;
       o.lane_min_slots = 160;
       o.lane_atoms = 1;
     } else if (key == "sus2_sh_deferred_check") {
@@ -296,12 +296,14 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
 // Shapes the autotuner tries (the first feasible one is the default): waves per tile x
 // accumulators per wave x minimum slot budget x atoms per lane.  Two atoms per lane share
 // each LDS address and each program record (half the record traffic per atom) at twice the
-// LDS per tile.
+// LDS per tile.  Sixteen waves per tile (1024-thread workgroups) keep four waves per SIMD
+// when one tile fills the CU's LDS (three-layer graphs need 384 FP32 slots, 96 KiB).
 const std::vector<Engine::LaneShape>& Engine::lane_shapes()
 {
   static const std::vector<LaneShape> shapes = {
     {4, 4, 160, 1}, {8, 2, 160, 1}, {4, 2, 160, 1}, {8, 2, 192, 1}, {8, 2, 256, 1},
-    {2, 4, 160, 1}, {8, 2, 208, 1}, {8, 2, 160, 2}, {4, 2, 160, 2}, {8, 2, 208, 2}};
+    {2, 4, 160, 1}, {8, 2, 208, 1}, {8, 2, 160, 2}, {4, 2, 160, 2}, {8, 2, 208, 2},
+    {16, 2, 192, 1}, {16, 2, 320, 1}};
   return shapes;
 }
 

@@ -320,8 +320,8 @@ bool lane_shape_supported(int waves, int nslot, int atoms)
 {
   if (atoms == 2) return nslot == 2 && (waves == 4 || waves == 8);
   if (atoms != 1) return false;
-  return (nslot == 2 && (waves == 4 || waves == 8)) || (nslot == 4 && (waves == 1 || waves == 2 || waves == 4)) ||
-         (nslot == 8 && waves == 1);
+  return (nslot == 2 && (waves == 4 || waves == 8 || waves == 16)) ||
+         (nslot == 4 && (waves == 1 || waves == 2 || waves == 4)) || (nslot == 8 && waves == 1);
 }
 
 template <typename Real>
@@ -338,6 +338,7 @@ bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneT
   }
   SUS2SH_LANE_CASE(2, 4, 1)
   SUS2SH_LANE_CASE(2, 8, 1)
+  SUS2SH_LANE_CASE(2, 16, 1)
   SUS2SH_LANE_CASE(4, 1, 1)
   SUS2SH_LANE_CASE(4, 2, 1)
   SUS2SH_LANE_CASE(4, 4, 1)

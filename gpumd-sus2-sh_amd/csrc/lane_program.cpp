@@ -166,19 +166,11 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
   p.waves = waves;
   const int A = m.moments_count, B = m.basic_count;
   p.basic_count = B;
-  if (m.layer_count > 2) {
-    p.reason = "product graph deeper than two layers";
+  if (m.layer_count > 3) {
+    p.reason = "product graph deeper than three layers";
     return p;
   }
-  max_slots = std::min(max_slots, kLaneMaxSlot + 1);
-  const int reserved = 2 * B + (with_trash ? 2 : 1);
-  if (max_slots - reserved < 2) {
-    p.reason = "too many basic moments for the per-lane slot budget";
-    return p;
-  }
-  p.ones_slot = 2 * B;
-  p.trash_slot = with_trash ? 2 * B + 1 : 2 * B;  // without a trash slot no bin may be empty
-  p.v_base = reserved;
+  const std::vector<int>& layer = m.node_layer;
 
   // direct weights d_n and the structural "mapped" flag
   std::vector<double> d(A, 0.0);
@@ -188,21 +180,66 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     mapped[m.moment_mapping[s]] = 1;
   }
   const int T = static_cast<int>(m.products.size());
-  std::vector<std::vector<int>> layer1_terms(A);  // products of each layer-1 target
-  std::vector<int> l2;                             // live layer-2 products
+  // Y: layer-2 nodes that are operands of live layer-3 products (three-layer graphs only)
+  std::vector<char> is_y(A, 0);
   for (int q = 0; q < T; ++q) {
     const Product& pr = m.products[q];
-    const int lt = m.node_layer[pr.target];
+    if (layer[pr.target] != 3 || !mapped[pr.target]) continue;
+    if (layer[pr.left] == 2) is_y[pr.left] = 1;
+    if (layer[pr.right] == 2) is_y[pr.right] = 1;
+  }
+  std::vector<std::vector<int>> layer1_terms(A);  // products of each layer-1 target
+  std::vector<std::vector<int>> y_terms(A);       // products of each Y node
+  std::vector<int> l2;  // products of the quadratic form: live layer-2 and layer-3 products
+  for (int q = 0; q < T; ++q) {
+    const Product& pr = m.products[q];
+    const int lt = layer[pr.target];
     if (lt == 1) {
       layer1_terms[pr.target].push_back(q);
+    } else if (is_y[pr.target]) {
+      y_terms[pr.target].push_back(q);
     } else if (mapped[pr.target]) {
       l2.push_back(q);
     }
   }
-  // component members: layer <= 1 nodes that are mapped or feed a live layer-2 product
+  // P: the layer-1 operands of the Y products.  Their values are computed once, before the
+  // first batch, into persistent slots (the Y rows of any batch read them), and their
+  // adjoints are collected in persistent slots: the quadratic-form part from their own
+  // component's batch, the part through Y after each batch that holds Y nodes; their reverse
+  // pass to the basic adjoints runs once, after the last batch.
+  std::vector<int> p1;
+  std::vector<int> pv_slot(A, -1), gp_slot(A, -1);
+  for (int n = 0; n < A; ++n) {
+    if (!is_y[n]) continue;
+    for (int q : y_terms[n]) {
+      for (int o : {m.products[q].left, m.products[q].right}) {
+        if (layer[o] != 1 || pv_slot[o] >= 0) continue;
+        pv_slot[o] = 0;
+        p1.push_back(o);
+      }
+    }
+  }
+  std::sort(p1.begin(), p1.end());
+  max_slots = std::min(max_slots, kLaneMaxSlot + 1);
+  const int reserved = 2 * B + (with_trash ? 2 : 1);
+  const int np = static_cast<int>(p1.size());
+  if (max_slots - reserved - 2 * np < 2) {
+    p.reason = "too many basic moments for the per-lane slot budget";
+    return p;
+  }
+  p.ones_slot = 2 * B;
+  p.trash_slot = with_trash ? 2 * B + 1 : 2 * B;  // without a trash slot no bin may be empty
+  for (int i = 0; i < np; ++i) {
+    pv_slot[p1[i]] = reserved + i;
+    gp_slot[p1[i]] = reserved + np + i;
+  }
+  p.v_base = reserved + 2 * np;
+
+  // component members: layer <= 1 and Y nodes that are mapped or feed a live product of the
+  // quadratic form
   std::vector<char> member(A, 0);
   for (int n = 0; n < A; ++n)
-    if (mapped[n] && m.node_layer[n] <= 1) member[n] = 1;
+    if (mapped[n] && (layer[n] <= 1 || is_y[n])) member[n] = 1;
   for (int q : l2) {
     member[m.products[q].left] = 1;
     member[m.products[q].right] = 1;
@@ -227,17 +264,18 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     }
   }
   p.components = static_cast<int>(comps.size());
-  // layer-2 products by member node
+  // quadratic-form products by member node
   std::vector<std::vector<int>> l2_of(A);
   for (int q : l2) {
     l2_of[m.products[q].left].push_back(q);
     if (m.products[q].right != m.products[q].left) l2_of[m.products[q].right].push_back(q);
   }
 
-  // region sizes: the free slots are split between values and adjoints
+  // region sizes: the free slots are split between values and adjoints (basic and P nodes
+  // keep their values in their own slots)
   auto v_need = [&](const std::vector<int>& c) {
     int k = 0;
-    for (int n : c) k += n >= B ? 1 : 0;
+    for (int n : c) k += n >= B && pv_slot[n] < 0 ? 1 : 0;
     return k;
   };
   int nv_max = 0, ng_max = 0;
@@ -287,24 +325,78 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
   }
   p.batches = static_cast<int>(batches.size());
 
+  // reverse terms of a layer-1 node n with adjoint slot gs into the basic adjoints
+  auto add_layer1_back = [&](std::vector<std::vector<Term>>& gb, int n, int gs) {
+    for (int q : layer1_terms[n]) {
+      const Product& pr = m.products[q];
+      if (pr.left == pr.right) {
+        gb[pr.left].push_back({2.0 * pr.coeff, gs, pr.left});
+      } else {
+        gb[pr.left].push_back({pr.coeff, gs, pr.right});
+        gb[pr.right].push_back({pr.coeff, gs, pr.left});
+      }
+    }
+  };
+  auto basic_rows = [&](std::vector<std::vector<Term>>& gb, std::vector<Row>& rows) {
+    for (int q = 0; q < B; ++q) {
+      if (gb[q].empty()) continue;
+      Row r;
+      r.target = B + q;
+      r.end = kLaneEndAdd;
+      r.terms = std::move(gb[q]);
+      rows.push_back(std::move(r));
+    }
+  };
+
+  // the operands of Y products are basic or P nodes, whose values sit in fixed slots
+  auto operand_slot = [&](int o) { return o < B ? o : pv_slot[o]; };
   std::vector<int> vslot(A, -1), gslot(A, -1);
+  for (int n : p1) vslot[n] = pv_slot[n];
+  if (np > 0) {
+    // P adjoints start at zero; P values from the basics (one epoch: disjoint slots)
+    std::vector<Row> pre;
+    for (int n : p1) {
+      Row z;
+      z.target = gp_slot[n];
+      z.end = kLaneEndStore;
+      z.terms.push_back({0.0, p.ones_slot, p.ones_slot});
+      pre.push_back(std::move(z));
+      Row r;
+      r.target = pv_slot[n];
+      r.end = kLaneEndStore;
+      for (int q : layer1_terms[n])
+        r.terms.push_back({m.products[q].coeff, m.products[q].left, m.products[q].right});
+      if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, p.ones_slot});
+      pre.push_back(std::move(r));
+    }
+    emit_segment(p, kLaneBinaryStore, pre);
+  }
   for (const std::vector<int>& batch : batches) {
     std::vector<int> nodes;
     for (int c : batch) nodes.insert(nodes.end(), comps[c].begin(), comps[c].end());
     int nv = 0, ng = 0;
+    bool has_y = false;
     for (int n : nodes) {
-      vslot[n] = n < B ? n : p.v_base + nv++;
+      if (n >= B && pv_slot[n] < 0) vslot[n] = p.v_base + nv++;
+      if (n < B) vslot[n] = n;
       gslot[n] = p.g_base + ng++;
+      has_y = has_y || is_y[n];
     }
-    // FWD: layer-1 values
+    // FWD: layer-1 values and Y values (Y rows read the basic and P slots)
     std::vector<Row> fwd;
     for (int n : nodes) {
-      if (n < B) continue;
+      if (n < B || pv_slot[n] >= 0) continue;
       Row r;
       r.target = vslot[n];
       r.end = kLaneEndStore;
-      for (int q : layer1_terms[n])
-        r.terms.push_back({m.products[q].coeff, m.products[q].left, m.products[q].right});
+      if (is_y[n]) {
+        for (int q : y_terms[n])
+          r.terms.push_back({m.products[q].coeff, operand_slot(m.products[q].left),
+                             operand_slot(m.products[q].right)});
+      } else {
+        for (int q : layer1_terms[n])
+          r.terms.push_back({m.products[q].coeff, m.products[q].left, m.products[q].right});
+      }
       if (r.terms.empty()) r.terms.push_back({0.0, p.ones_slot, p.ones_slot});
       fwd.push_back(std::move(r));
     }
@@ -354,29 +446,56 @@ LaneProgram compile_impl(const Model& m, int max_slots, int nslot, int waves, bo
     for (int n : nodes) {
       if (n < B) {
         gb[n].push_back({1.0, gslot[n], p.ones_slot});
-        continue;
+      } else if (pv_slot[n] >= 0) {
+        // a P node: its quadratic-form adjoint joins the persistent one
+        Row r;
+        r.target = gp_slot[n];
+        r.end = kLaneEndAdd;
+        r.terms.push_back({1.0, gslot[n], p.ones_slot});
+        eb.push_back(std::move(r));
+      } else if (!is_y[n]) {
+        add_layer1_back(gb, n, gslot[n]);
       }
-      for (int q : layer1_terms[n]) {
-        const Product& pr = m.products[q];
-        if (pr.left == pr.right) {
-          gb[pr.left].push_back({2.0 * pr.coeff, gslot[n], pr.left});
-        } else {
-          gb[pr.left].push_back({pr.coeff, gslot[n], pr.right});
-          gb[pr.right].push_back({pr.coeff, gslot[n], pr.left});
+    }
+    basic_rows(gb, eb);
+    // BWD reads what ENG reads (values, adjoints) and adds into the basic and P adjoints,
+    // which ENG does not read: both run in one epoch
+    emit_segment(p, kLaneBinaryAdd, eb, false);
+    if (has_y) {
+      // the Y adjoints of this batch into the adjoints of their operands (P and basic
+      // slots), in an epoch of its own (the BWD rows above add into the same slots)
+      std::vector<std::vector<Term>> into(A);
+      for (int n : nodes) {
+        if (!is_y[n]) continue;
+        for (int q : y_terms[n]) {
+          const Product& pr = m.products[q];
+          if (pr.left == pr.right) {
+            into[pr.left].push_back({2.0 * pr.coeff, gslot[n], operand_slot(pr.left)});
+          } else {
+            into[pr.left].push_back({pr.coeff, gslot[n], operand_slot(pr.right)});
+            into[pr.right].push_back({pr.coeff, gslot[n], operand_slot(pr.left)});
+          }
         }
       }
+      std::vector<Row> yb;
+      for (int o = 0; o < A; ++o) {
+        if (into[o].empty()) continue;
+        Row r;
+        r.target = o < B ? B + o : gp_slot[o];
+        r.end = kLaneEndAdd;
+        r.terms = std::move(into[o]);
+        yb.push_back(std::move(r));
+      }
+      emit_segment(p, kLaneBinaryAdd, yb);
     }
-    for (int q = 0; q < B; ++q) {
-      if (gb[q].empty()) continue;
-      Row r;
-      r.target = B + q;
-      r.end = kLaneEndAdd;
-      r.terms = std::move(gb[q]);
-      eb.push_back(std::move(r));
-    }
-    // BWD reads what ENG reads (values, adjoints) and adds into the basic adjoints, which
-    // ENG does not read: both run in one epoch
-    emit_segment(p, kLaneBinaryAdd, eb, false);
+  }
+  if (np > 0) {
+    // the P nodes' complete adjoints into the basic adjoints
+    std::vector<std::vector<Term>> gb(B);
+    for (int n : p1) add_layer1_back(gb, n, gp_slot[n]);
+    std::vector<Row> rows;
+    basic_rows(gb, rows);
+    emit_segment(p, kLaneBinaryAdd, rows);
   }
   if (p.needs_trash && !with_trash) {
     p.reason = "a store segment has an empty accumulator: needs a trash slot";

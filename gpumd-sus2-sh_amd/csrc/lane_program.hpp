@@ -1,4 +1,4 @@
-// Lane program: the product graph of a depth <= 2 model as one wave-uniform instruction
+// Lane program: the product graph of a depth <= 3 model as one wave-uniform instruction
 // stream executed by one atom per lane (kernel in lane_kernel.hip).
 //
 // Semantics are those of the flat loop of src/force/sus2_sh.cu:4799-4846: forward
@@ -21,9 +21,19 @@
 // 64-atom tile and split the rows, each wave's rows interleaved as groups of nslot
 // instructions so that one group issues nslot independent LDS reads per operand.
 //
+// Three-layer graphs (body order 6): the layer-2 nodes Y that feed live layer-3 products
+// join the quadratic form as nodes (a layer-3 product couples a Y node with a node of the
+// same (L, M) class), so the form is quadratic in the layer-<=1 and Y values.  A Y value is
+// computed in the FWD segment of its batch from its layer-1 operands P, whose values are
+// computed once, before the first batch, into persistent slots.  The total adjoint of a P
+// node is its quadratic-form adjoint (added to a persistent slot by the BWD segment of its
+// batch) plus the part through Y (a segment after each batch holding Y nodes: g_P += c g_Y
+// v_other, basic operands straight into the basic adjoints); one segment after the last
+// batch carries the P adjoints to the basic adjoints.
 // Per-lane slot map: [0, B) basics, [B, 2B) basic adjoints, 2B the constant 1, (2B + 1 a
-// write-only trash slot, only when a store segment has an accumulator without rows),
-// [v_base, v_base + cap_v) component values, [g_base, g_base + cap_g) component adjoints.
+// write-only trash slot, only when a store segment has an accumulator without rows), then
+// the P values and P adjoints (three-layer graphs), [v_base, v_base + cap_v) component
+// values, [g_base, g_base + cap_g) component adjoints.
 // Instruction word: x | y << 10 | t << 20 | end << 30 with end 0 (none), 1 (slot t = acc),
 // 2 (slot t += acc), 3 (site energy += acc).  Binary segments add c X[x] X[y], unary
 // segments add c X[x].  A row much longer than a segment's mean bin load is split between

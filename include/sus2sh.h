@@ -77,7 +77,7 @@ int sus2sh_probe(const char* path);
  * species_count element symbols, then key=value options (src/force/sus2_sh.cu:527-534).
  * Honoured options (like the reference, only key=value tokens are options; bare tokens are
  * ignored): sus2_float / sus2_sh_float (precision, env SUS2_GPUMD_FLOAT),
- * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8>,
+ * sus2_sh_graph_tile=<n>, sus2_sh_graph_variant=<0|1|2|3>, sus2_sh_lane_waves=<1|2|4|8|16>,
  * sus2_sh_lane_shape=<waves>x<accumulators>x<min slots>[x<atoms per lane>] (e.g. 8x2x208 or
  * 8x2x160x2: fixes the lane kernel's full shape), sus2_sh_autotune=<0|1>, sus2_sh_profile=<0|1>,
  * sus2_sh_deferred_check=<0|1> (steady calls make no host synchronisation and can be
@@ -219,7 +219,8 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
  * ms[v * 5 + log2(tile)] for variant v = 0..2 and tile 1..16, and ms[15 + c] for lane-program
  * (variant 3) shape c = 0..6: (waves per 64-atom tile, accumulators per wave, minimum
  * per-lane slots) = (4,4,160), (8,2,160), (4,2,160), (8,2,192), (8,2,256), (2,4,160),
- * (8,2,208), then with two atoms per lane (8,2,160), (4,2,160), (8,2,208) at c = 7..9.  When
+ * (8,2,208), then with two atoms per lane (8,2,160), (4,2,160), (8,2,208) at c = 7..9, then
+ * sixteen waves per tile (16,2,192), (16,2,320) at c = 10, 11.  When
  * the lane shape is fixed (sus2_sh_lane_waves / sus2_sh_lane_shape) only
  * that shape is timed and its time is in ms[15].  Returns the count written. */
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
@@ -267,7 +268,8 @@ int sus2sh_model_check(const char* mtp_path, int* dims);
 
 /* Host-only check of the lane program (variant 3 of the graph kernel; no reference
  * counterpart): compile it for the model with at most max_slots per-lane slots, `waves`
- * (1, 2, 4 or 8) waves per 64-atom tile and `nslot` (1, 2, 4 or 8; 0: 2 with 8 waves, else 4)
+ * (1, 2, 4, 8 or 16) waves per 64-atom tile and `nslot` (1, 2, 4 or 8; 0: 2 with 8 or 16
+ * waves, else 4)
  * accumulators per wave, and run it on the host for n basic vectors
  * basics[n][B] -> e_prime[n] = sum_s w_s M[map[s]] and grad[n][B] = de_prime/dbasic (without
  * the species factor).  real_bytes = 0 runs the compiled program; 4 or 8 runs its device

@@ -1,7 +1,7 @@
 """GPU parity of every lane-kernel shape on the benchmark model (through the C ABI).
 
 The FP32 benchmark runs the shape the autotuner picks for the l3333 graph (body order 5,
-sh_body_l_max = {3,3,3,3}: 1349 moments, 5475 products, 572 scalars) among the ten lane
+sh_body_l_max = {3,3,3,3}: 1349 moments, 5475 products, 572 scalars) among the twelve lane
 shapes (waves per tile, accumulators per wave, slot budget, atoms per lane).  Each shape is pinned
 with sus2_sh_lane_shape and compared with the FP64 oracle on a 12x12x8-cell fcc Cu-Zr piece
 of the benchmark structure (4,608 atoms, a = 3.85 A, jitter 0.08 A, rc 5.0 A, rb 8,
@@ -21,7 +21,8 @@ from oracle.binding import Oracle
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(4, 4, 160, 1), (8, 2, 160, 1), (4, 2, 160, 1), (8, 2, 192, 1), (8, 2, 256, 1),
-          (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2)]
+          (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2),
+          (16, 2, 192, 1), (16, 2, 320, 1)]
 
 
 @pytest.fixture(scope="module")

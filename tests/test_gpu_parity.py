@@ -228,10 +228,10 @@ def test_parameter_gradient_matches_oracle(make_model, fp32, model):
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("fp32", [False, True])
 def test_lane_program_matches_oracle(make_model, name, fp32):
-    """Graph variant 3 (one atom per lane, wave-uniform program) on every case; the
-    body-order-6 graph is three layers deep, and a graph whose largest component does not
-    fit the LDS of a 64-atom workgroup (l4k4 in FP64) has no lane program: both are
-    rejected cleanly."""
+    """Graph variant 3 (one atom per lane, wave-uniform program) on every case; a graph
+    whose per-lane state does not fit the LDS of a 64-atom workgroup (l4k4 in FP64, and the
+    full-l body-6 graph l2k3, whose 259 layer-1 operands of layer-2 tensors need persistent
+    slots) has no lane program and is rejected cleanly."""
     case = CASES[name]
     path = make_model(name, **case["model"])
     S = case["model"].get("species_count", 2)
@@ -240,7 +240,7 @@ def test_lane_program_matches_oracle(make_model, name, fp32):
                                           "sus2_sh_graph_variant=3"]
     # largest per-lane slot budget whose 64-atom workgroup fits 160 KiB of LDS
     feasible = sus2sh.lane_check(path, max_slots=640 if fp32 else 320, waves=4)[0]["ok"]
-    if case["model"]["body_order"] > 5 or not feasible:
+    if not feasible:
         with pytest.raises(sus2sh.Sus2shError) as err:
             _run(path, cell, types, pos, options=opts)
         assert "no lane program" in err.value.message

@@ -5,7 +5,9 @@ basic vectors.
 
 Bar: E' = sum_s w_s M[map[s]] and dE'/dbasic agree to 1e-11 relative to their scale (both
 in double; the lane program sums in another order: components of the layer-2 quadratic form
-and E' = 1/2 sum_n v_n (g_n + d_n)).  Graphs deeper than two layers have no lane program.
+and E' = 1/2 sum_n v_n (g_n + d_n)).  Three-layer (body-order 6) graphs add the layer-2 tensor
+nodes Y that feed layer-3 products to the quadratic form, with their layer-1 operands P kept in
+persistent slots.
 """
 import dataclasses
 
@@ -73,7 +75,8 @@ def test_l3333_structure(make_model):
 # every lane shape the engine's autotuner tries (waves, accumulators, slot budget, atoms per
 # lane)
 LANE_SHAPES = [(4, 4, 160, 1), (8, 2, 160, 1), (4, 2, 160, 1), (8, 2, 192, 1), (8, 2, 256, 1),
-               (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2)]
+               (2, 4, 160, 1), (8, 2, 208, 1), (8, 2, 160, 2), (4, 2, 160, 2), (8, 2, 208, 2),
+               (16, 2, 192, 1), (16, 2, 320, 1)]
 
 
 @pytest.mark.parametrize("waves,nslot,slots,atoms", LANE_SHAPES)
@@ -101,11 +104,53 @@ def test_l3333_every_tuned_shape_and_encoding(make_model, waves, nslot, slots, a
         assert np.abs(g - g0).max() <= tol * scale
 
 
-def test_body6_has_no_lane_program(make_model):
-    path = make_model("lane_body6", l_max=2, k_max=2, body_order=6, rb_size=4, max_dist=4.0,
-                      seed=5)
+BODY6 = [
+    # (l_max, k_max, body_l_max, slot budget): full-l graphs (many Y / P nodes) and the
+    # production shape with a last intermediate of l <= 1 ({.., 1}: 73 Y, 45 P nodes)
+    (2, 2, None, 1024),
+    (1, 3, None, 1024),
+    (3, 3, (3, 3, 3, 3, 1), 384),
+    (4, 3, (4, 4, 4, 4, 1), 384),
+]
+
+
+@pytest.mark.parametrize("L,K,blm,slots", BODY6)
+def test_body6_graphs(make_model, L, K, blm, slots):
+    """Three-layer graphs: every wave split against the oracle's flat graph, and the FP32
+    device encoding of the 8-wave program at 2e-6 of the scale."""
+    path = make_model(f"lane_body6_l{L}k{K}_{blm}", l_max=L, k_max=K, body_order=6,
+                      body_l_max=blm, rb_size=4, max_dist=4.0, seed=5 + L + K, species_count=2)
     assert sus2sh.model_check(path)["layer_count"] == 3
-    _check(path, expect_ok=False)
+    info = _check(path, max_slots=slots, n=3)
+    assert info["instructions"] > 0
+    orc = Oracle(path)
+    basics = np.random.default_rng(11).normal(0.0, 0.4, size=(3, orc.dims["B"]))
+    e0, g0 = orc.graph(basics)
+    scale = max(1.0, np.abs(e0).max(), np.abs(g0).max())
+    info, e, g = sus2sh.lane_check(path, basics, max_slots=slots, waves=8, nslot=2, real_bytes=4)
+    assert info["ok"] == 1
+    assert np.abs(e - e0).max() <= 2e-6 * scale
+    assert np.abs(g - g0).max() <= 2e-6 * scale
+
+
+def test_non_standard_depth3_graph(model_dir):
+    """A three-layer graph that is not the standard one: perturbed coefficients, scalars
+    mapped to random nodes of every layer (so some layer-2 operands of layer-3 products are
+    mapped themselves, and some layer-3 products are dead), one node mapped twice."""
+    rng = np.random.default_rng(7)
+    spec = sus2sh.synthetic_model(2, 2, 6, species_count=2, rb_size=4, max_dist=4.0, seed=17)
+    g = spec.graph
+    coeffs = g.coeffs * rng.uniform(0.5, 1.5, size=g.coeffs.shape)
+    targets = np.unique(g.products[:, 2])
+    scalars = rng.choice(np.concatenate([targets, [0, 1]]), size=g.scalar_count).astype(np.int32)
+    scalars[1] = scalars[0]
+    g2 = dataclasses.replace(g, coeffs=coeffs, scalars=scalars)
+    spec2 = dataclasses.replace(spec, graph=g2,
+                                moment_coeffs=rng.normal(0.0, 0.3, size=g.scalar_count))
+    path = sus2sh.write_mtp(spec2, str(model_dir / "lane_nonstandard_b6.mtp"))
+    info = sus2sh.model_check(path)
+    assert info["standard_graph"] == 0 and info["layer_count"] == 3
+    _check(path, max_slots=1024, n=3)
 
 
 def test_budget_too_small(make_model):

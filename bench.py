@@ -65,11 +65,11 @@ def kernel_flops_per_atom(info, nbar):
     return {"basic": basic, "graph": graph, "force": force}
 
 
-def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
+def run_engine(path, cell, types, pos, precision, steps, warmup, profile, extra=()):
     import torch
     import sus2sh
     n = len(types)
-    opts = ["Cu", "Zr", "sus2_float=%d" % (1 if precision == "fp32" else 0)]
+    opts = ["Cu", "Zr", "sus2_float=%d" % (1 if precision == "fp32" else 0)] + list(extra)
     if profile:
         opts.append("sus2_sh_profile=1")
     pot = sus2sh.SUS2_SH(path, n, opts)
@@ -104,8 +104,10 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile):
     info = pot.info()
     tuning = {(f"v{v}_w{w}" if v < 3 else f"lane{w}"): round(ms, 4)
               for (v, w), ms in pot.tuning().items()}
+    shapes, _ = pot.edge_shapes()
     out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
                info=info, nbar=nbar, energy=float(e.sum().item()), tuning=tuning,
+               edge_shapes={k: f"{t}x{npl}" for k, (t, npl) in shapes.items()},
                force=f.cpu().numpy(), potential=e.cpu().numpy(), virial=v.cpu().numpy(),
                stream=stream.cuda_stream)
     if stage_sum is not None:
@@ -245,6 +247,26 @@ def c3ref(nx, ny, nz, steps):
             "fp32_vs_reference_a100_l3333": r32["atom_steps_per_s"] / REFERENCE_A100_L3333}
 
 
+def c2_small(steps):
+    """A 32,000-atom structure (the 20x20x20-cell Cu64Zr36 alloy, a = 3.85 A, jitter 0.08 A)
+    with the benchmark's l3333 model, FP32 fixed-cell force loop: once with the edge-kernel
+    shapes the autotuner keeps, once pinned to one lane per atom in 256-thread workgroups.  At
+    this size one lane per atom gives 500 waves for 1,024 SIMDs."""
+    import sus2sh
+    from sus2sh import configs
+    path = sus2sh.write_mtp(configs.model_c3(), os.path.join(
+        tempfile.mkdtemp(prefix="sus2sh_c2s_"), "l3333.mtp"))
+    pos, types, cell = configs.structure_c2()
+    tuned = run_engine(path, cell, types, pos, "fp32", steps, 3, profile=True)
+    fixed = run_engine(path, cell, types, pos, "fp32", steps, 3, profile=True,
+                       extra=["sus2_sh_basic_shape=256x1", "sus2_sh_force_shape=256x1"])
+    return {"atoms": len(types), "model": "l3333 body5 rb8 rc5.0 seed3003",
+            "tuned_atom_steps_per_s": tuned["atom_steps_per_s"],
+            "tuned_stage_ms": tuned["stage_ms"], "tuned_edge_shapes": tuned["edge_shapes"],
+            "one_lane_per_atom_atom_steps_per_s": fixed["atom_steps_per_s"],
+            "one_lane_per_atom_stage_ms": fixed["stage_ms"]}
+
+
 def c5_batch(evals):
     """The batched-structure workload: 4,096 triclinic Cu-Zr cells of 8..256 atoms (543,189
     atoms), l3k3 body 5, one sus2sh_compute_batch call per evaluation with and without
@@ -284,6 +306,8 @@ def c5_batch(evals):
             out[key + "_ms"] = dt * 1e3
             out[key + "_atom_evals_per_s"] = n / dt
             out[key + "_structures_per_s"] = ns / dt
+        shapes, _ = pot.edge_shapes()
+        out[prec + "_edge_shapes"] = {k: f"{t}x{npl}" for k, (t, npl) in shapes.items()}
         pot.close()
     return out
 
@@ -302,6 +326,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c3ref-steps", type=int, default=10,
                     help="FP32 steps of the reference-shaped model run (rc 7.5, rb 10; 0: skip)")
+    ap.add_argument("--c2-steps", type=int, default=20,
+                    help="FP32 steps of the 32,000-atom structure with tuned and with "
+                         "one-lane-per-atom edge kernels (0: skip)")
     ap.add_argument("--c5-evals", type=int, default=5,
                     help="timed batched evaluations of the 4,096-cell workload (0: skip)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
@@ -370,6 +397,12 @@ def main():
         log(f"[bench] reference-shaped model (rc 7.5, rb 10): {args.c3ref_steps} FP32 steps")
         ref_shape = c3ref(nx, ny, nz, args.c3ref_steps)
         log(f"[bench] c3ref {ref_shape}")
+    c2s = None
+    if args.c2_steps > 0:
+        log(f"[bench] 32,000-atom structure: {args.c2_steps} FP32 steps, tuned and one lane "
+            "per atom")
+        c2s = c2_small(args.c2_steps)
+        log(f"[bench] c2 {c2s}")
     c5 = None
     if args.c5_evals > 0:
         log(f"[bench] batched 4,096-cell workload: {args.c5_evals} evaluations per mode")
@@ -417,6 +450,8 @@ def main():
         "graph_variant_fp64": r64["info"]["graph_variant"],
         "graph_tuning_ms_fp32": r32["tuning"],
         "graph_tuning_ms_fp64": r64["tuning"],
+        "edge_shapes_fp32": r32["edge_shapes"],
+        "edge_shapes_fp64": r64["edge_shapes"],
         "step_tflops_fp32": total_flops / (r32["ms_per_step"] * 1e-3) / 1e12,
         # the dominant kernel is FP32 vector-ALU / LDS work: it issues no matrix-core (MFMA)
         # instruction, so it is bound by VALU issue and priced against the published FP32
@@ -434,6 +469,7 @@ def main():
         "errors_vs_fp64_oracle": oracle_err,
         "oracle_force_scale": oracle_forces,
         "reference_shape_c3ref": ref_shape,
+        "small_32k": c2s,
         "batch_c5": c5,
         "graph_replay_fp32": graph32,
         "cpu_baseline": cpu,

@@ -279,6 +279,23 @@ int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n)
   return k;
 }
 
+int sus2sh_get_edge_tuning(const sus2sh_handle* handle, int* shape, float* ms, int n)
+{
+  if (!handle) return 0;
+  const sus2sh::Engine& e = *handle->engine;
+  if (shape) {
+    shape[0] = e.basic_shape().threads;
+    shape[1] = e.basic_shape().npl;
+    shape[2] = e.force_shape().threads;
+    shape[3] = e.force_shape().npl;
+  }
+  if (!ms || n < 0) return 0;
+  const std::vector<float>& t = e.edge_tune_ms();
+  const int k = std::min<int>(n, static_cast<int>(t.size()));
+  for (int i = 0; i < k; ++i) ms[i] = t[i];
+  return k;
+}
+
 double sus2sh_mean_neighbors(const sus2sh_handle* handle)
 {
   return handle ? handle->engine->mean_neighbors() : 0.0;

@@ -1,6 +1,8 @@
-// Per-edge kernels: basic moments and forces/virial.  One lane per atom in cell-sorted
-// order, one wavefront per 64-atom tile; each lane walks its atom's neighbours and keeps
-// the full (k, l, m) accumulator grid in VGPRs, so nothing per-edge is ever stored.
+// Per-edge kernels: basic moments and forces/virial.  Atoms in cell-sorted order, npl
+// consecutive lanes per atom (1 by default; 2-8 give small structures more lanes than atoms):
+// each lane walks its share of the atom's neighbours and keeps the full (k, l, m) accumulator
+// grid in VGPRs, so nothing per-edge is ever stored; the npl partial grids are summed with
+// lane shuffles.
 //
 //   basic moments   B_i[k,l,m] = sum_j R_{k(L+1)+l}(r_ij) Y_lm(r_ij)
 //                   (src/force/sus2_sh.cu:3486-3550, 3650-3724)
@@ -41,6 +43,15 @@ __host__ __device__ constexpr int coeff_stride(int NB, int n)
   return (w + ((4 - w % 32) + 32) % 32) / wpr;
 }
 
+// Sum of v over the npl consecutive lanes of one atom (npl a power of two dividing 64):
+// butterfly of xor shuffles, so every lane of the group ends with the same value.
+template <typename Real>
+__device__ __forceinline__ Real group_sum(Real v, int npl)
+{
+  for (int off = 1; off < npl; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 template <typename Real>
 __device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, int row, int NB,
                                              Real* s_coeff, Real* s_scal)
@@ -75,21 +86,25 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
   stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);
 
   const int N = a.N;
-  const int p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= N) return;
+  // npl lanes per atom: lane `sub` of atom p takes the edges k = sub, sub + npl, ...
+  const int npl = a.npl;
+  const int p = xcd_block() * (static_cast<int>(blockDim.x) / npl) + static_cast<int>(threadIdx.x) / npl;
+  const int sub = static_cast<int>(threadIdx.x) & (npl - 1);
+  const bool valid = p < N;
+  if (npl == 1 && !valid) return;
   Real acc[FG];
 #pragma unroll
   for (int f = 0; f < FG; ++f) acc[f] = Real(0);
-  const int ti = a.type_s[p];
-  const int cnt = a.count[p];
+  const int ti = valid ? a.type_s[p] : 0;
+  const int cnt = valid ? a.count[p] : 0;
   const int S = a.rt.S;
   const int rb = a.rt.rb;
   const Real rc = a.rt.rc;
   // the next edge is loaded while the current one is evaluated
-  Edge<Real> next = cnt > 0 ? a.nl[p] : Edge<Real>{};
-  for (int k = 0; k < cnt; ++k) {
+  Edge<Real> next = sub < cnt ? a.nl[static_cast<size_t>(sub) * N + p] : Edge<Real>{};
+  for (int k = sub; k < cnt; k += npl) {
     const Edge<Real> e = next;
-    if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
+    if (k + npl < cnt) next = a.nl[static_cast<size_t>(k + npl) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
     const Real inv = dev_rsqrt<Real>(r2);
     const Real r = r2 * inv;
@@ -115,10 +130,15 @@ __global__ __launch_bounds__(256) void k_basics(EdgeLaunch<Real> a, Real* __rest
       }
     }
   }
+  if (npl > 1) {
+#pragma unroll
+    for (int f = 0; f < FG; ++f) acc[f] = group_sum(acc[f], npl);
+  }
+  if (!valid) return;
 #pragma unroll
   for (int f = 0; f < FG; ++f) {
     const int b = a.fmap[f];
-    if (b >= 0) out[static_cast<size_t>(b) * N + p] = acc[f];
+    if (b >= 0 && (f & (npl - 1)) == sub) out[static_cast<size_t>(b) * N + p] = acc[f];
   }
 }
 
@@ -236,9 +256,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   unsigned long long* acc_q = reinterpret_cast<unsigned long long*>(acc_w);  // FIX
   unsigned long long* f_q = reinterpret_cast<unsigned long long*>(f_s);      // FIX
   int w_count = -1, w_total = 0;
-  const int block = xcd_block();  // the force tile: kWinTile atoms of the cell order
+  // npl lanes per atom (edge k to lane k mod npl); a workgroup covers `per_block` atoms of
+  // the cell order, inside one window tile (kWinTile atoms, a multiple of per_block)
+  const int npl = a.npl;
+  const int per_block = static_cast<int>(blockDim.x) / npl;
+  const int block = xcd_block();
   if (WIN) {
-    const int* tw = win + static_cast<size_t>(block) * kWinStride;
+    const int* tw = win + static_cast<size_t>(block * per_block / kWinTile) * kWinStride;
     w_count = tw[0];
     w_total = tw[1];
     if (w_count >= 0) {
@@ -262,9 +286,10 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);  // ends with a barrier
 
   const int N = a.N;
-  const int p = block * blockDim.x + threadIdx.x;
+  const int p = block * per_block + static_cast<int>(threadIdx.x) / npl;
+  const int sub = static_cast<int>(threadIdx.x) & (npl - 1);
   const bool valid = p < N;
-  if (!PG && !WIN && !valid) return;
+  if (!PG && !WIN && npl == 1 && !valid) return;
   Real G[FG];
 #pragma unroll
   for (int f = 0; f < FG; ++f) {
@@ -276,10 +301,10 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   const Real rc = a.rt.rc;
   Real fx = 0, fy = 0, fz = 0;
   Real w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  Edge<Real> next = cnt > 0 ? a.nl[p] : Edge<Real>{};
-  for (int k = 0; k < cnt; ++k) {
+  Edge<Real> next = sub < cnt ? a.nl[static_cast<size_t>(sub) * N + p] : Edge<Real>{};
+  for (int k = sub; k < cnt; k += npl) {
     const Edge<Real> e = next;
-    if (k + 1 < cnt) next = a.nl[static_cast<size_t>(k + 1) * N + p];
+    if (k + npl < cnt) next = a.nl[static_cast<size_t>(k + npl) * N + p];
     const Real r2 = e.dx * e.dx + e.dy * e.dy + e.dz * e.dz;
     const Real inv = dev_rsqrt<Real>(r2);
     const Real r = r2 * inv;
@@ -393,7 +418,14 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     w[7] -= e.dz * g0;
     w[8] -= e.dz * g1;
   }
-  if (valid) {
+  if (npl > 1) {
+    fx = group_sum(fx, npl);
+    fy = group_sum(fy, npl);
+    fz = group_sum(fz, npl);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) w[c] = group_sum(w[c], npl);
+  }
+  if (valid && sub == 0) {
     if constexpr (FIX) {
       atomicAdd(f_q + p, fix_encode(static_cast<double>(fx), kFixSumMax, a.fix_flag));
       atomicAdd(f_q + N + p, fix_encode(static_cast<double>(fy), kFixSumMax, a.fix_flag));
@@ -469,21 +501,24 @@ int uniform_nb(const EdgeLaunch<Real>& a)
 template <typename Real, int LL>
 bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
 {
-  const int blocks = (a.N + 255) / 256;
+  if (!edge_shape_valid(a.threads, a.npl)) return false;
+  const int per_block = a.threads / a.npl;
+  const int blocks = (a.N + per_block - 1) / per_block;
+  const int T = a.threads;
   const int nb = a.gate_m ? 0 : uniform_nb(a);  // the gated kernel uses the per-mu path
   const size_t lds = table_bytes(a, nb);
 #define SUS2SH_BASICS_CASE(KK)                                                    \
   if (a.K == KK) {                                                                \
     if (a.gate_m)                                                                 \
-      k_basics<Real, LL, KK, 0, true><<<blocks, 256, lds, s>>>(a, basics);        \
+      k_basics<Real, LL, KK, 0, true><<<blocks, T, lds, s>>>(a, basics);        \
     else if (nb == 8)                                                             \
-      k_basics<Real, LL, KK, 8, false><<<blocks, 256, lds, s>>>(a, basics);       \
+      k_basics<Real, LL, KK, 8, false><<<blocks, T, lds, s>>>(a, basics);       \
     else if (nb == 12)                                                            \
-      k_basics<Real, LL, KK, 12, false><<<blocks, 256, lds, s>>>(a, basics);      \
+      k_basics<Real, LL, KK, 12, false><<<blocks, T, lds, s>>>(a, basics);      \
     else if (nb == 16)                                                            \
-      k_basics<Real, LL, KK, 16, false><<<blocks, 256, lds, s>>>(a, basics);      \
+      k_basics<Real, LL, KK, 16, false><<<blocks, T, lds, s>>>(a, basics);      \
     else                                                                          \
-      k_basics<Real, LL, KK, 0, false><<<blocks, 256, lds, s>>>(a, basics);       \
+      k_basics<Real, LL, KK, 0, false><<<blocks, T, lds, s>>>(a, basics);       \
     return true;                                                                  \
   }
   SUS2SH_FOR_EACH_K(SUS2SH_BASICS_CASE)
@@ -495,8 +530,11 @@ template <typename Real, int LL>
 bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,
                      double* pg_rad, const int* win, int win_cap, hipStream_t s)
 {
-  const int blocks = (a.N + 255) / 256;
-  static_assert(kWinTile == 256, "force tiles are the 256-thread workgroups");
+  if (!edge_shape_valid(a.threads, a.npl)) return false;
+  const int per_block = a.threads / a.npl;
+  const int blocks = (a.N + per_block - 1) / per_block;
+  const int T = a.threads;
+  static_assert(kWinTile % 256 == 0, "a window tile holds whole force workgroups");
   // the parameter gradient, the gated kernel and deterministic summation use the per-mu path
   const bool fix = a.fix_flag != nullptr;
   const int nb = (pg_rad || a.gate_m || fix) ? 0 : uniform_nb(a);
@@ -514,7 +552,7 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   {                                                                                            \
     auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV, GV, FV>;                               \
     if (lds > 65536) allow_large_lds<&k_forces<Real, LL, KK, PGV, NBV, WINV, GV, FV>>();       \
-    kern<<<blocks, 256, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
+    kern<<<blocks, T, lds, s>>>(a, grads, f_s, v_s, pg_rad, win);                            \
   }
 #define SUS2SH_FORCES_CASE(KK)                                                               \
   if (a.K == KK) {                                                                           \

@@ -103,6 +103,17 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.md_graph = parse_bool(val, key);
     } else if (key == "sus2_sh_deterministic") {
       o.deterministic = parse_bool(val, key);
+    } else if (key == "sus2_sh_basic_shape" || key == "sus2_sh_force_shape") {
+      // threads per workgroup x lanes per atom of an edge kernel, e.g. 256x1 or 128x4
+      int t = 0, n = 0;
+      char x = 0;
+      std::istringstream in(val);
+      in >> t >> x >> n;
+      if (!in || x != 'x' || !in.eof() || !edge_shape_valid(t, n))
+        throw InputError(key + " must be <threads>x<lanes per atom> with threads 64, 128 or 256 "
+                         "and 1, 2, 4 or 8 lanes per atom: " + val);
+      (key == "sus2_sh_basic_shape" ? o.basic_threads : o.force_threads) = t;
+      (key == "sus2_sh_basic_shape" ? o.basic_npl : o.force_npl) = n;
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -240,6 +251,11 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     tuned_ = true;
     if (opt_.graph_variant < 0) variant_ = lane_.ok ? 3 : 1;
   }
+  // edge-kernel shapes: fixed ones stay; deterministic mode keeps one lane per atom (one
+  // summation order for every atom count), else the autotuner or the size rule decides
+  if (opt_.basic_threads > 0) basic_shape_ = EdgeShape{opt_.basic_threads, opt_.basic_npl};
+  if (opt_.force_threads > 0) force_shape_ = EdgeShape{opt_.force_threads, opt_.force_npl};
+  if (opt_.deterministic || (opt_.basic_threads > 0 && opt_.force_threads > 0)) edge_tuned_ = true;
   if (opt_.use_float)
     upload_tables<float>();
   else
@@ -291,6 +307,25 @@ LaneProgram Engine::build_lane(const LaneShape& shape) const
     if (p.reason.find("slot budget") == std::string::npos) break;  // not a budget limit
   }
   return best;
+}
+
+// Edge-kernel shapes the autotuner tries: threads per workgroup x lanes per atom.
+const std::vector<EdgeShape>& Engine::edge_candidates()
+{
+  static const std::vector<EdgeShape> c = {{256, 1}, {256, 2}, {256, 4}, {256, 8},
+                                           {128, 1}, {128, 2}, {128, 4}, {128, 8},
+                                           {64, 1},  {64, 2},  {64, 4},  {64, 8}};
+  return c;
+}
+
+// Shape of the edge kernels before (or without) the autotuner: as many lanes per atom as
+// keep the lane count N * npl at or below 2^17 (two waves per SIMD of a 256-CU device), so
+// that small structures still fill the device; one lane per atom from 131,072 atoms on.
+EdgeShape Engine::default_edge_shape(int N)
+{
+  EdgeShape e;
+  while (e.npl < 8 && static_cast<long long>(N) * e.npl * 2 <= (1LL << 17)) e.npl *= 2;
+  return e;
 }
 
 // Shapes the autotuner tries (the first feasible one is the default): waves per tile x
@@ -1012,6 +1047,49 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       throw UnsupportedError("SUS2_SH: gate graph tile does not fit in LDS.");
     el.gate_m = static_cast<const Real*>(ws_.gate_m);
   }
+  // edge-kernel shapes: the autotuner times every candidate once per handle on the first
+  // call that can be timed (one k chunk, no gate, no parameter gradient, not captured, from
+  // 512 atoms: below that the launches are too short to rank); until then the size rule
+  const bool edge_tune = opt_.autotune && !edge_tuned_ && !param_grad && !capturing && !noalloc &&
+                         !gate && kchunks_ == 1 && N >= 512;
+  if (!edge_tuned_ && !edge_tune) {
+    if (opt_.basic_threads <= 0) basic_shape_ = default_edge_shape(N);
+    if (opt_.force_threads <= 0) force_shape_ = default_edge_shape(N);
+  }
+  const std::vector<EdgeShape>& edge_cand = edge_candidates();
+  hipEvent_t et0 = nullptr, et1 = nullptr;
+  if (edge_tune) {
+    hip_check(hipEventCreate(&et0), "hipEventCreate");
+    hip_check(hipEventCreate(&et1), "hipEventCreate");
+    edge_tune_ms_.assign(2 * edge_cand.size(), -1.0f);
+  }
+  // mean time of two launches after one untimed one (-1: the shape does not launch)
+  const auto time_edge = [&](const auto& launch) {
+    if (!launch()) return -1.0f;
+    float ms = 0.0f;
+    hip_check(hipEventRecord(et0, s), "hipEventRecord");
+    for (int rep = 0; rep < 2; ++rep) launch();
+    hip_check(hipEventRecord(et1, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(et1), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, et0, et1), "hipEventElapsedTime");
+    return ms / 2;
+  };
+  if (edge_tune && opt_.basic_threads <= 0) {
+    float best = 1e30f;
+    for (size_t c = 0; c < edge_cand.size(); ++c) {
+      EdgeLaunch<Real> et = el;
+      et.threads = edge_cand[c].threads;
+      et.npl = edge_cand[c].npl;
+      const float ms = time_edge([&] { return launch_basics<Real>(et, static_cast<Real*>(ws_.basics), s); });
+      edge_tune_ms_[c] = ms;
+      if (ms >= 0.0f && ms < best) {
+        best = ms;
+        basic_shape_ = edge_cand[c];
+      }
+    }
+  }
+  el.threads = basic_shape_.threads;
+  el.npl = basic_shape_.npl;
   bool basics_ok = launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s);
   el.gate_m = nullptr;
   for (int c = 1; c < kchunks_ && basics_ok; ++c)
@@ -1141,6 +1219,37 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * acc_bytes));
     launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
     win = ws_.win;
+  }
+  if (edge_tune && opt_.force_threads <= 0) {
+    // trial launches accumulate into f_s and overwrite v_s: both are reset below
+    float best = 1e30f;
+    for (size_t c = 0; c < edge_cand.size(); ++c) {
+      EdgeLaunch<Real> et = el;
+      et.threads = edge_cand[c].threads;
+      et.npl = edge_cand[c].npl;
+      const float ms = time_edge([&] {
+        return launch_forces<Real>(et, static_cast<const Real*>(ws_.grads), ws_.f_s, ws_.v_s,
+                                   nullptr, win, win_cap, s);
+      });
+      edge_tune_ms_[edge_cand.size() + c] = ms;
+      if (ms >= 0.0f && ms < best) {
+        best = ms;
+        force_shape_ = edge_cand[c];
+      }
+    }
+  }
+  if (edge_tune) {
+    (void)hipEventDestroy(et0);
+    (void)hipEventDestroy(et1);
+    edge_tuned_ = true;
+  }
+  el.threads = force_shape_.threads;
+  el.npl = force_shape_.npl;
+  if (param_grad && opt_.force_threads <= 0) {
+    // the parameter-gradient kernel zeroes, fills and flushes per-pair accumulators in every
+    // workgroup: it keeps 256-thread workgroups (the timed shapes do not cover it)
+    el.threads = 256;
+    el.npl = default_edge_shape(N).npl;
   }
   launch_fill_double(ws_.f_s, 0.0, 3 * static_cast<size_t>(N), s);
   if (gate) {

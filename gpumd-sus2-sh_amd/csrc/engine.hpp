@@ -73,6 +73,16 @@ struct EngineOptions {
   // bit-reproducible energies, forces and virials: fixed-point force sums, no autotuner,
   // graph programs without atomic back-chunk merges
   bool deterministic = false;
+  // edge-kernel launch shapes (threads per workgroup x lanes per atom) of the basic-moment
+  // and force kernels; 0: the autotuner's choice (or the size rule when it does not run)
+  int basic_threads = 0, basic_npl = 0;
+  int force_threads = 0, force_npl = 0;
+};
+
+// Edge-kernel launch shape: threads per workgroup and lanes per atom.
+struct EdgeShape {
+  int threads = 256;
+  int npl = 1;
 };
 
 struct CapacityError : public std::runtime_error {
@@ -153,6 +163,12 @@ class Engine {
   int graph_tile() const { return tile_; }
   int graph_variant() const { return variant_; }
   const std::vector<float>& tune_ms() const { return tune_ms_; }
+  // edge-kernel shapes in use and the autotuner's times per candidate
+  // (edge_candidates()[c] at edge_tune_ms()[c] for the basic kernel, [C + c] for forces)
+  EdgeShape basic_shape() const { return basic_shape_; }
+  EdgeShape force_shape() const { return force_shape_; }
+  static const std::vector<EdgeShape>& edge_candidates();
+  const std::vector<float>& edge_tune_ms() const { return edge_tune_ms_; }
   int neighbor_capacity() const { return cap_; }
   int param_count() const;
   const std::vector<float>& stage_ms() const { return stage_ms_; }
@@ -204,6 +220,12 @@ class Engine {
   int tile_ = 1;
   int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane
   bool tuned_ = false;
+  // edge-kernel shapes: fixed by options, else tuned once on the first eligible call (the
+  // size rule default_edge_shape() until then)
+  EdgeShape basic_shape_, force_shape_;
+  bool edge_tuned_ = false;
+  std::vector<float> edge_tune_ms_;
+  static EdgeShape default_edge_shape(int N);
   // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2 and
   // [15 + c] for lane-program shape c
   std::vector<float> tune_ms_;

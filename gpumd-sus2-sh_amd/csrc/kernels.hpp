@@ -177,7 +177,19 @@ struct EdgeLaunch {
   int* fix_flag = nullptr;
   // parameter gradient: interleaved copies of the per-pair LDS accumulators (power of two)
   int pg_copies = 1;
+  // launch shape: `threads` per workgroup (64, 128 or 256) and `npl` consecutive lanes per
+  // atom (1, 2, 4 or 8) that split its neighbour list (edge k goes to lane k mod npl) and
+  // sum their partial results with lane shuffles; a workgroup covers threads / npl atoms
+  int threads = 256;
+  int npl = 1;
 };
+
+// Feasible edge-kernel launch shapes (threads per workgroup, lanes per atom).
+inline bool edge_shape_valid(int threads, int npl)
+{
+  return (threads == 64 || threads == 128 || threads == 256) &&
+         (npl == 1 || npl == 2 || npl == 4 || npl == 8);
+}
 
 template <typename Real>
 bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);

@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_get_model_info",
     "sus2sh_get_stage_times",
     "sus2sh_get_tuning",
+    "sus2sh_get_edge_tuning",
     "sus2sh_mean_neighbors",
     "sus2sh_clebsch_gordan",
     "sus2sh_real_cg",
@@ -164,6 +165,8 @@ def lib():
     L.sus2sh_get_stage_times.restype = c_int
     L.sus2sh_get_tuning.argtypes = [c_void_p, P(ctypes.c_float), c_int]
     L.sus2sh_get_tuning.restype = c_int
+    L.sus2sh_get_edge_tuning.argtypes = [c_void_p, P(c_int), P(ctypes.c_float), c_int]
+    L.sus2sh_get_edge_tuning.restype = c_int
     L.sus2sh_mean_neighbors.argtypes = [c_void_p]
     L.sus2sh_mean_neighbors.restype = c_double
     L.sus2sh_clebsch_gordan.argtypes = [c_int] * 6

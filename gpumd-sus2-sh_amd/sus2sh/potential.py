@@ -99,6 +99,20 @@ class SUS2_SH:
                 out[(3, k - 15)] = buf[k]
         return out
 
+    def edge_shapes(self):
+        """Edge-kernel launch shapes of the last compute and the autotuner's times:
+        ({"basic": (threads, lanes per atom), "force": (...)},
+         {("basic" | "force", threads, lanes per atom): ms})."""
+        shape = (ctypes.c_int * 4)()
+        buf = (ctypes.c_float * 24)()
+        n = lib().sus2sh_get_edge_tuning(self._h, shape, buf, 24)
+        cand = [(t, npl) for t in (256, 128, 64) for npl in (1, 2, 4, 8)]
+        times = {}
+        for k in range(n):
+            if buf[k] >= 0:
+                times[("basic" if k < 12 else "force",) + cand[k % 12]] = buf[k]
+        return {"basic": (shape[0], shape[1]), "force": (shape[2], shape[3])}, times
+
     def mean_neighbors(self):
         return lib().sus2sh_mean_neighbors(self._h)
 

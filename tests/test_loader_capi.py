@@ -142,3 +142,15 @@ def test_create_reports_errors_without_exiting(tmp_path):
     with pytest.raises(sus2sh.Sus2shError) as err:
         sus2sh.model_check(bad)
     assert "Failed to open SUS2-SH model file" in err.value.message
+
+
+@pytest.mark.parametrize("opt", ["sus2_sh_basic_shape=96x2", "sus2_sh_force_shape=256x3",
+                                 "sus2_sh_force_shape=256", "sus2_sh_basic_shape=512x1"])
+def test_invalid_edge_shape_option_is_an_input_error(good, opt):
+    """Edge-kernel shapes outside threads {64, 128, 256} x lanes per atom {1, 2, 4, 8} are
+    rejected while the options are parsed (before any device call) with a message naming the
+    option."""
+    with pytest.raises(sus2sh.Sus2shError) as err:
+        sus2sh.SUS2_SH(good, 16, ["Cu", "Zr", opt])
+    assert err.value.code == _capi.SUS2SH_ERR_INPUT
+    assert opt.split("=")[0] in err.value.message

@@ -105,9 +105,11 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile, extra=
     tuning = {(f"v{v}_w{w}" if v < 3 else f"lane{w}"): round(ms, 4)
               for (v, w), ms in pot.tuning().items()}
     shapes, _ = pot.edge_shapes()
+    mfma = ("", "+fp32-mfma", "+bf16-mfma")[shapes.pop("basic_mfma")]
     out = dict(seconds=dt, atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
                info=info, nbar=nbar, energy=float(e.sum().item()), tuning=tuning,
-               edge_shapes={k: f"{t}x{npl}" for k, (t, npl) in shapes.items()},
+               edge_shapes={k: f"{t}x{npl}" + (mfma if k == "basic" else "")
+                            for k, (t, npl) in shapes.items()},
                force=f.cpu().numpy(), potential=e.cpu().numpy(), virial=v.cpu().numpy(),
                stream=stream.cuda_stream)
     if stage_sum is not None:
@@ -307,7 +309,9 @@ def c5_batch(evals):
             out[key + "_atom_evals_per_s"] = n / dt
             out[key + "_structures_per_s"] = ns / dt
         shapes, _ = pot.edge_shapes()
-        out[prec + "_edge_shapes"] = {k: f"{t}x{npl}" for k, (t, npl) in shapes.items()}
+        mfma = ("", "+fp32-mfma", "+bf16-mfma")[shapes.pop("basic_mfma")]
+        out[prec + "_edge_shapes"] = {k: f"{t}x{npl}" + (mfma if k == "basic" else "")
+                                      for k, (t, npl) in shapes.items()}
         pot.close()
     return out
 
@@ -326,6 +330,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--c3ref-steps", type=int, default=10,
                     help="FP32 steps of the reference-shaped model run (rc 7.5, rb 10; 0: skip)")
+    ap.add_argument("--mfma-steps", type=int, default=10,
+                    help="FP32 steps with matrix-core basic moments, FP32 and BF16 operands "
+                         "(0: skip)")
     ap.add_argument("--c2-steps", type=int, default=20,
                     help="FP32 steps of the 32,000-atom structure with tuned and with "
                          "one-lane-per-atom edge kernels (0: skip)")
@@ -377,6 +384,22 @@ def main():
     log(f"[bench] fp64 {r64['atom_steps_per_s']:.4e} atom-steps/s, stages {r64.get('stage_ms')}")
     ferr = float(np.abs(r32["force"] - r64["force"]).max())
     frms = float(np.sqrt(((r32["force"] - r64["force"]) ** 2).mean()))
+    # the matrix-core basic moments: FP32 operands and the optional BF16 operands (FP32
+    # accumulation), FP32 engine otherwise; same timed-step count as the FP64 run
+    mfma_legs = {}
+    gpu_runs = {"fp32": r32, "fp64": r64}
+    if args.mfma_steps > 0:
+        for mode in ("fp32", "bf16"):
+            log(f"[bench] FP32 run with {mode} matrix-core basic moments: {args.mfma_steps} steps")
+            rm = run_engine(path, cell, types, pos, "fp32", args.mfma_steps, 2, profile=True,
+                            extra=[f"sus2_sh_basic_mfma={mode}"])
+            gpu_runs[f"fp32_{mode}_mfma_basics"] = rm
+            mfma_legs[mode] = {
+                "atom_steps_per_s": rm["atom_steps_per_s"], "ms_per_step": rm["ms_per_step"],
+                "basic_ms": rm["stage_ms"]["basic"],
+                "max_force_err_vs_fp64_engine": float(np.abs(rm["force"] - r64["force"]).max()),
+                "max_energy_err_vs_fp64_engine": float(np.abs(rm["potential"] - r64["potential"]).max())}
+            log(f"[bench] {mode} mfma basics {mfma_legs[mode]}")
 
     info, nbar = r32["info"], r32["nbar"]
     flops = kernel_flops_per_atom(info, nbar)
@@ -412,7 +435,7 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline: FP64 oracle on the full structure (also the error reference)")
         cpu, oracle_err, oracle_forces = cpu_baseline(
-            path, cell, types, pos, {"fp32": r32, "fp64": r64}, args.cpu_seconds)
+            path, cell, types, pos, gpu_runs, args.cpu_seconds)
         log(f"[bench] cpu {cpu}; errors vs oracle {oracle_err}")
     per_kernel = {}
     for k in ("basic", "graph", "force"):
@@ -469,6 +492,7 @@ def main():
         "errors_vs_fp64_oracle": oracle_err,
         "oracle_force_scale": oracle_forces,
         "reference_shape_c3ref": ref_shape,
+        "mfma_basic_moments": mfma_legs,
         "small_32k": c2s,
         "batch_c5": c5,
         "graph_replay_fp32": graph32,

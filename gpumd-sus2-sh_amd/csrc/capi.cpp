@@ -288,6 +288,7 @@ int sus2sh_get_edge_tuning(const sus2sh_handle* handle, int* shape, float* ms, i
     shape[1] = e.basic_shape().npl;
     shape[2] = e.force_shape().threads;
     shape[3] = e.force_shape().npl;
+    shape[4] = e.basic_shape().mfma;
   }
   if (!ms || n < 0) return 0;
   const std::vector<float>& t = e.edge_tune_ms();

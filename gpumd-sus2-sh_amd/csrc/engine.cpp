@@ -114,6 +114,13 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
                          "and 1, 2, 4 or 8 lanes per atom: " + val);
       (key == "sus2_sh_basic_shape" ? o.basic_threads : o.force_threads) = t;
       (key == "sus2_sh_basic_shape" ? o.basic_npl : o.force_npl) = n;
+    } else if (key == "sus2_sh_basic_mfma") {
+      if (val == "0" || val == "off") o.basic_mfma = 0;
+      else if (val == "1" || val == "fp32") o.basic_mfma = 1;
+      else if (val == "2" || val == "bf16") o.basic_mfma = 2;
+      else if (val == "auto") o.basic_mfma = -1;
+      else
+        throw InputError("sus2_sh_basic_mfma must be auto, 0 (off), 1 (fp32) or 2 (bf16): " + val);
     } else if (key == "sus2_sh_graph_tile") {
       const int t = std::atoi(val.c_str());
       if (t != 1 && t != 2 && t != 4 && t != 8 && t != 16)
@@ -260,6 +267,12 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     upload_tables<float>();
   else
     upload_tables<double>();
+  if (opt_.basic_mfma > 0 && !mfma_ok_)
+    throw InputError("sus2_sh_basic_mfma: the matrix-core basic moments need the FP32 engine, "
+                     "one radial scaling (a, s) per species pair, no gate, l_max <= 3, "
+                     "k_max (l_max + 1) <= 16 and radial_basis_size <= 16.");
+  if (opt_.basic_mfma >= 0) basic_shape_.mfma = opt_.basic_mfma;
+  if (opt_.basic_mfma > 0 && opt_.force_threads > 0) edge_tuned_ = true;
   if (model_.zbl_enabled) {
     zbl_z_.upload(zbl_.atomic_number.data(), zbl_.atomic_number.size() * sizeof(int));
     zbl_inner_.upload(zbl_.inner.data(), zbl_.inner.size() * sizeof(double));
@@ -402,6 +415,19 @@ void Engine::upload_tables()
         radial_uniform_ = false;
         break;
       }
+  // matrix-core basic moments: c_p[mu][n] = sigma_p c[mu][n] (build_radial_tables)
+  mfma_ok_ = sizeof(Real) == sizeof(float) && kchunks_ == 1 &&
+             mfma_basics_supported(L, K, rb, m.species_count, radial_uniform_, m.gate_enabled);
+  if (mfma_ok_) {
+    std::vector<Real> rcm(static_cast<size_t>(Mr) * rb), sig(pairs);
+    for (size_t i = 0; i < rcm.size(); ++i) rcm[i] = static_cast<Real>(m.radial_coeffs[i]);
+    for (int ti = 0; ti < m.species_count; ++ti)
+      for (int tj = 0; tj < m.species_count; ++tj)
+        sig[ti * m.species_count + tj] = static_cast<Real>(
+          m.scaling * m.radial_type_coeffs[ti] * m.radial_type_coeffs[tj]);
+    mfma_rc_.upload(rcm.data(), rcm.size() * sizeof(Real));
+    pair_sigma_.upload(sig.data(), sig.size() * sizeof(Real));
+  }
 
   // each chunk's full (k,l,m) grid -> basic index (the kernels' full_index<L, Kc> with the
   // chunk-local k); a basic belongs to the chunk that holds its k
@@ -1053,7 +1079,11 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   const bool edge_tune = opt_.autotune && !edge_tuned_ && !param_grad && !capturing && !noalloc &&
                          !gate && kchunks_ == 1 && N >= 512;
   if (!edge_tuned_ && !edge_tune) {
-    if (opt_.basic_threads <= 0) basic_shape_ = default_edge_shape(N);
+    if (opt_.basic_threads <= 0) {
+      const int mfma = basic_shape_.mfma;  // the kernel kind is not the rule's to change
+      basic_shape_ = default_edge_shape(N);
+      basic_shape_.mfma = mfma;
+    }
     if (opt_.force_threads <= 0) force_shape_ = default_edge_shape(N);
   }
   const std::vector<EdgeShape>& edge_cand = edge_candidates();
@@ -1061,8 +1091,22 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (edge_tune) {
     hip_check(hipEventCreate(&et0), "hipEventCreate");
     hip_check(hipEventCreate(&et1), "hipEventCreate");
-    edge_tune_ms_.assign(2 * edge_cand.size(), -1.0f);
+    // [0, C) vector basic kernel, [C, 2C) forces, [2C] the FP32 matrix-core basic kernel
+    edge_tune_ms_.assign(2 * edge_cand.size() + 1, -1.0f);
   }
+  el.mfma_rc = mfma_ok_ ? mfma_rc_.as<Real>() : nullptr;
+  el.pair_sigma = mfma_ok_ ? pair_sigma_.as<Real>() : nullptr;
+  if (basic_shape_.mfma > 0 && !mfma_ok_)
+    throw UnsupportedError("SUS2_SH: the model no longer fits the matrix-core basic moments "
+                           "(sus2_sh_basic_mfma): one radial scaling per species pair is needed.");
+  // the basic-moment launch of the current shape (matrix cores or the vector kernel)
+  const auto basics_launch = [&](const EdgeLaunch<Real>& e, const EdgeShape& sh) {
+    if constexpr (sizeof(Real) == sizeof(float)) {
+      if (sh.mfma > 0)
+        return launch_basics_mfma(e, static_cast<float*>(ws_.basics), sh.mfma == 2, s);
+    }
+    return launch_basics<Real>(e, static_cast<Real*>(ws_.basics), s);
+  };
   // mean time of two launches after one untimed one (-1: the shape does not launch)
   const auto time_edge = [&](const auto& launch) {
     if (!launch()) return -1.0f;
@@ -1074,23 +1118,32 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     hip_check(hipEventElapsedTime(&ms, et0, et1), "hipEventElapsedTime");
     return ms / 2;
   };
-  if (edge_tune && opt_.basic_threads <= 0) {
+  if (edge_tune && opt_.basic_mfma <= 0) {
     float best = 1e30f;
-    for (size_t c = 0; c < edge_cand.size(); ++c) {
-      EdgeLaunch<Real> et = el;
-      et.threads = edge_cand[c].threads;
-      et.npl = edge_cand[c].npl;
-      const float ms = time_edge([&] { return launch_basics<Real>(et, static_cast<Real*>(ws_.basics), s); });
-      edge_tune_ms_[c] = ms;
-      if (ms >= 0.0f && ms < best) {
-        best = ms;
-        basic_shape_ = edge_cand[c];
+    if (opt_.basic_threads <= 0) {
+      for (size_t c = 0; c < edge_cand.size(); ++c) {
+        EdgeLaunch<Real> et = el;
+        et.threads = edge_cand[c].threads;
+        et.npl = edge_cand[c].npl;
+        const float ms = time_edge([&] { return basics_launch(et, edge_cand[c]); });
+        edge_tune_ms_[c] = ms;
+        if (ms >= 0.0f && ms < best) {
+          best = ms;
+          basic_shape_ = edge_cand[c];
+        }
       }
+    }
+    if (opt_.basic_mfma < 0 && mfma_ok_) {
+      // the FP32 matrix-core kernel (same precision class as the vector kernel)
+      const EdgeShape mf{256, 1, 1};
+      const float ms = time_edge([&] { return basics_launch(el, mf); });
+      edge_tune_ms_[2 * edge_cand.size()] = ms;
+      if (ms >= 0.0f && ms < best) basic_shape_ = mf;
     }
   }
   el.threads = basic_shape_.threads;
   el.npl = basic_shape_.npl;
-  bool basics_ok = launch_basics<Real>(el, static_cast<Real*>(ws_.basics), s);
+  bool basics_ok = basics_launch(el, basic_shape_);
   el.gate_m = nullptr;
   for (int c = 1; c < kchunks_ && basics_ok; ++c)
     basics_ok = launch_basics<Real>(chunk_launch(c), static_cast<Real*>(ws_.basics), s);

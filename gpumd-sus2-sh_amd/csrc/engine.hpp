@@ -77,12 +77,16 @@ struct EngineOptions {
   // and force kernels; 0: the autotuner's choice (or the size rule when it does not run)
   int basic_threads = 0, basic_npl = 0;
   int force_threads = 0, force_npl = 0;
+  // basic moments on the matrix cores: -1 the autotuner's choice between the vector kernel
+  // and FP32 MFMAs, 0 never, 1 FP32 MFMAs, 2 BF16 operands with FP32 accumulation
+  int basic_mfma = -1;
 };
 
 // Edge-kernel launch shape: threads per workgroup and lanes per atom.
 struct EdgeShape {
   int threads = 256;
   int npl = 1;
+  int mfma = 0;  // basic moments only: 0 vector kernel, 1 FP32 MFMA, 2 BF16 MFMA
 };
 
 struct CapacityError : public std::runtime_error {
@@ -248,6 +252,9 @@ class Engine {
   int kchunk_k0_[kMaxKChunks] = {};
   int kchunk_K_[kMaxKChunks] = {};
   DeviceBuffer radial_coeff_[kMaxKChunks], radial_scal_[kMaxKChunks], fmap_[kMaxKChunks];
+  // matrix-core basic moments: shared radial coefficients [Mr][rb] and pair factors [S * S]
+  DeviceBuffer mfma_rc_, pair_sigma_;
+  bool mfma_ok_ = false;  // the model fits the matrix-core basic-moment kernel
   DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
   DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;
   DeviceBuffer e_slot_, e_w_, seed_, shift_, sigma_;

@@ -182,7 +182,18 @@ struct EdgeLaunch {
   // sum their partial results with lane shuffles; a workgroup covers threads / npl atoms
   int threads = 256;
   int npl = 1;
+  // matrix-core basic moments (FP32 engine): the shared radial coefficient matrix c[mu][n]
+  // ([Mr][rb]) and the pair factors sigma_p ([S * S]) with c_p = sigma_p c
+  const Real* mfma_rc = nullptr;
+  const Real* pair_sigma = nullptr;
 };
+
+// Basic moments as two dense matrix-core products per atom (T^T Y over the edges, then the
+// radial coefficients): FP32 operands, or BF16 operands with FP32 accumulation.  For FP32
+// engines with one radial scaling per pair (uniform), no gate and l_max <= 3,
+// k_max (l_max + 1) <= 16, rb <= 16 (mfma_basics_supported).
+bool mfma_basics_supported(int l_max, int k_max, int rb, int S, bool uniform, bool gate);
+bool launch_basics_mfma(const EdgeLaunch<float>& a, float* basics, bool bf16, hipStream_t s);
 
 // Feasible edge-kernel launch shapes (threads per workgroup, lanes per atom).
 inline bool edge_shape_valid(int threads, int npl)

@@ -101,17 +101,20 @@ class SUS2_SH:
 
     def edge_shapes(self):
         """Edge-kernel launch shapes of the last compute and the autotuner's times:
-        ({"basic": (threads, lanes per atom), "force": (...)},
-         {("basic" | "force", threads, lanes per atom): ms})."""
-        shape = (ctypes.c_int * 4)()
-        buf = (ctypes.c_float * 24)()
-        n = lib().sus2sh_get_edge_tuning(self._h, shape, buf, 24)
+        ({"basic": (threads, lanes per atom), "force": (...), "basic_mfma": 0 | 1 | 2},
+         {("basic" | "force", threads, lanes per atom): ms, ("basic_mfma_fp32",): ms})."""
+        shape = (ctypes.c_int * 5)()
+        buf = (ctypes.c_float * 25)()
+        n = lib().sus2sh_get_edge_tuning(self._h, shape, buf, 25)
         cand = [(t, npl) for t in (256, 128, 64) for npl in (1, 2, 4, 8)]
         times = {}
-        for k in range(n):
+        for k in range(min(n, 24)):
             if buf[k] >= 0:
                 times[("basic" if k < 12 else "force",) + cand[k % 12]] = buf[k]
-        return {"basic": (shape[0], shape[1]), "force": (shape[2], shape[3])}, times
+        if n > 24 and buf[24] >= 0:
+            times[("basic_mfma_fp32",)] = buf[24]
+        return ({"basic": (shape[0], shape[1]), "force": (shape[2], shape[3]),
+                 "basic_mfma": shape[4]}, times)
 
     def mean_neighbors(self):
         return lib().sus2sh_mean_neighbors(self._h)

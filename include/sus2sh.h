@@ -226,12 +226,13 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
 /* Launch shapes of the edge kernels (no reference counterpart: the reference launches one
- * fixed shape).  shape[0..4) = basic-moment kernel threads per workgroup and lanes per atom,
- * force kernel threads and lanes per atom (the shapes of the last compute).  ms[c] = the
- * autotuner's basic-kernel time of candidate c and ms[12 + c] its force-kernel time (-1:
- * not tried), candidates c = 0..11 being threads 256, 128, 64 (c / 4) x lanes per atom
- * 1, 2, 4, 8 (c mod 4).  Returns the number of ms entries written (0 before the autotuner
- * ran or without it). */
+ * fixed shape).  shape[0..5) = basic-moment kernel threads per workgroup and lanes per atom,
+ * force kernel threads and lanes per atom, and the basic-moment kernel kind (0 vector,
+ * 1 FP32 matrix cores, 2 BF16 matrix cores; sus2_sh_basic_mfma).  ms[c] = the autotuner's
+ * vector basic-kernel time of candidate c and ms[12 + c] its force-kernel time, candidates
+ * c = 0..11 being threads 256, 128, 64 (c / 4) x lanes per atom 1, 2, 4, 8 (c mod 4);
+ * ms[24] = the FP32 matrix-core basic kernel (-1: not tried).  Returns the number of ms
+ * entries written (0 before the autotuner ran or without it). */
 int sus2sh_get_edge_tuning(const sus2sh_handle* handle, int* shape, float* ms, int n);
 
 /* Mean number of neighbours per atom found by the last compute. */

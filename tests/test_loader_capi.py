@@ -145,7 +145,8 @@ def test_create_reports_errors_without_exiting(tmp_path):
 
 
 @pytest.mark.parametrize("opt", ["sus2_sh_basic_shape=96x2", "sus2_sh_force_shape=256x3",
-                                 "sus2_sh_force_shape=256", "sus2_sh_basic_shape=512x1"])
+                                 "sus2_sh_force_shape=256", "sus2_sh_basic_shape=512x1",
+                                 "sus2_sh_basic_mfma=fp16"])
 def test_invalid_edge_shape_option_is_an_input_error(good, opt):
     """Edge-kernel shapes outside threads {64, 128, 256} x lanes per atom {1, 2, 4, 8} are
     rejected while the options are parsed (before any device call) with a message naming the

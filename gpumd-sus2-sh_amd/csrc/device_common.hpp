@@ -86,6 +86,11 @@ struct RadialTable {
   Real rc;
   int kind;     // 0: xi = tanh(z), 1: xi = z / sqrt(1 + z^2); z = a (r - s) / 2
   int uniform;  // 1 when (a, s) does not depend on mu for every pair
+  // uniform models: coeff[pair][mu][n] = sigma[pair] shared[mu][n], the shared matrix with
+  // rows zero-padded to the kernel's row length NB ([Mr][NB]; null: use coeff)
+  const Real* shared = nullptr;
+  const Real* sigma = nullptr;  // [S*S]
+  int shared_nb = 0;            // row length of `shared`
 };
 
 struct CellGrid {
@@ -281,6 +286,44 @@ __device__ __forceinline__ void eval_radial(const Real* __restrict__ coeff,
       t1 = t2;
       d0 = d1;
       d1 = d2;
+    }
+    R[mu] = env * acc;
+    if (DERIV) dR[mu] = denv * acc + env * dxi * accd;
+  }
+}
+
+// The uniform case of eval_radial with the coefficient rows shared by all pairs: every lane
+// reads the same row (`c`, constant address space: scalar loads, SGPR operands) and the pair
+// enters through sigma_p and (a, s):
+//   R_mu = sigma_p (r - rc)^2 sum_n c[mu][n] T_n(xi_p(r))   (src/force/sus2_sh.cu:3171-3220).
+template <typename Real, int MR, bool DERIV, int NB>
+__device__ __forceinline__ void eval_radial_shared(const __attribute__((address_space(4))) Real* c,
+                                                   Real sigma, Real a, Real s, Real rc, int kind,
+                                                   Real r, Real (&R)[MR], Real (&dR)[MR])
+{
+  const Real dr = r - rc;
+  const Real env = sigma * dr * dr;
+  const Real denv = Real(2) * sigma * dr;
+  Real xi = 0, dxi = 0;
+  radial_map<Real>(kind, a, s, r, xi, dxi);
+  Real T[NB], dT[NB];
+  T[0] = Real(1);
+  dT[0] = Real(0);
+  T[1] = xi;
+  dT[1] = Real(1);
+#pragma unroll
+  for (int n = 2; n < NB; ++n) {
+    T[n] = Real(2) * xi * T[n - 1] - T[n - 2];
+    if (DERIV) dT[n] = Real(2) * (T[n - 1] + xi * dT[n - 1]) - dT[n - 2];
+  }
+#pragma unroll
+  for (int mu = 0; mu < MR; ++mu) {
+    Real acc = 0, accd = 0;
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const Real cn = c[mu * NB + n];
+      acc += cn * T[n];
+      if (DERIV) accd += cn * dT[n];
     }
     R[mu] = env * acc;
     if (DERIV) dR[mu] = denv * acc + env * dxi * accd;

@@ -52,6 +52,42 @@ __device__ __forceinline__ Real group_sum(Real v, int npl)
   return v;
 }
 
+// The FP32 uniform force kernel reads its coefficient rows from the rows shared by all pairs
+// (scalar loads, SGPR operands: no per-edge LDS reads, and the kernel fits its 3-waves-per-SIMD
+// register budget without spilling); the other kernels stage every pair's rows in LDS.
+template <typename Real, int NB, bool FORCE>
+constexpr bool shared_rows()
+{
+  return FORCE && NB > 0 && sizeof(Real) == sizeof(float);
+}
+
+// Bytes of the radial-table region at the start of an edge kernel's LDS: (a, s, sigma) per
+// pair with shared rows, else every pair's coefficient rows (padded to the kernel's row
+// length) and (a, s) per radial function.
+template <typename Real>
+__host__ __device__ constexpr size_t table_region_bytes(bool shared, int NB, int S, int Mr, int rb)
+{
+  return shared ? (3 * static_cast<size_t>(S) * S * sizeof(Real) + 15) & ~size_t(15)
+                : ((static_cast<size_t>(S) * S *
+                    (coeff_stride<Real>(NB, Mr * coeff_row(NB, rb)) + padded_stride(Mr * 2)) *
+                    sizeof(Real)) + 15) & ~size_t(15);
+}
+
+// (a, s, sigma) of every pair for the shared-row kernels; ends with a barrier
+template <typename Real>
+__device__ __forceinline__ void stage_pairs(const RadialTable<Real>& rt, int Mr, Real* s_pair)
+{
+  for (int i = threadIdx.x; i < rt.S * rt.S; i += blockDim.x) {
+    s_pair[3 * i] = rt.scal[static_cast<size_t>(i) * Mr * 2];
+    s_pair[3 * i + 1] = rt.scal[static_cast<size_t>(i) * Mr * 2 + 1];
+    s_pair[3 * i + 2] = rt.sigma[i];
+  }
+  __syncthreads();
+}
+
+template <typename Real>
+using const_rows = const __attribute__((address_space(4))) Real;
+
 template <typename Real>
 __device__ __forceinline__ void stage_tables(const RadialTable<Real>& rt, int Mr, int row, int NB,
                                              Real* s_coeff, Real* s_scal)
@@ -242,7 +278,8 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   const int cs = coeff_stride<Real>(NB, MR * row), ss = padded_stride(MR * 2);
   Real* s_scal = s_coeff + S * S * cs;
   // PG: per-pair adjoint accumulators [S*S][MR*rb + 2*MR] (double), 16-byte aligned
-  const size_t pg_off = ((static_cast<size_t>(S) * S * (cs + ss) * sizeof(Real)) + 15) & ~size_t(15);
+  constexpr bool SHARED = shared_rows<Real, NB, true>();
+  const size_t pg_off = table_region_bytes<Real>(SHARED, NB, S, MR, rb);
   double* s_pg = reinterpret_cast<double*>(smem_raw + pg_off);
   const int pg_stride = MR * rb + 2 * MR;
   const int pg_cp = a.pg_copies;  // interleaved copies of the accumulators (power of two)
@@ -283,7 +320,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
       }
     }
   }
-  stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);  // ends with a barrier
+  if constexpr (SHARED)
+    stage_pairs(a.rt, MR, s_coeff);  // (a, s, sigma) per pair; ends with a barrier
+  else
+    stage_tables(a.rt, MR, row, NB, s_coeff, s_scal);  // ends with a barrier
+  const_rows<Real>* shared = (const_rows<Real>*)a.rt.shared;
 
   const int N = a.N;
   const int p = block * per_block + static_cast<int>(threadIdx.x) / npl;
@@ -312,7 +353,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     const int pair = ti * S + (e.j >> kEdgeTypeShift);
     const int jn = e.j & kEdgeIndexMask;
     Real R[MR], dR[MR];
-    eval_radial<Real, MR, true, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
+    if constexpr (SHARED)
+      eval_radial_shared<Real, MR, true, NB>(shared, s_coeff[3 * pair + 2], s_coeff[3 * pair],
+                                             s_coeff[3 * pair + 1], rc, a.rt.kind, r, R, dR);
+    else
+      eval_radial<Real, MR, true, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                     a.rt.kind, r, R, dR);
     Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
     solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
@@ -478,20 +523,24 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
 }
 
 template <typename Real>
-size_t table_bytes(const EdgeLaunch<Real>& a, int NB)
+size_t table_bytes(const EdgeLaunch<Real>& a, int NB, bool force)
 {
-  const int Mr = a.K * (a.L + 1);
-  return static_cast<size_t>(a.rt.S) * a.rt.S *
-         (coeff_stride<Real>(NB, Mr * coeff_row(NB, a.rt.rb)) + padded_stride(Mr * 2)) * sizeof(Real);
+  const bool shared = force && NB > 0 && sizeof(Real) == sizeof(float);
+  return table_region_bytes<Real>(shared, NB, a.rt.S, a.K * (a.L + 1), a.rt.rb);
 }
 
 // NB of the uniform kernels: coefficient rows padded to 8, 12 or 16 (rb = 10, the common
-// production size, pads to 12); 0 selects the per-mu path.
+// production size, pads to 12); 0 selects the per-mu path.  The FP32 force kernel reads the
+// shared rows: without shared rows of that length it takes the per-mu path.
 template <typename Real>
-int uniform_nb(const EdgeLaunch<Real>& a)
+int uniform_nb(const EdgeLaunch<Real>& a, bool force)
 {
   if (!a.rt.uniform) return 0;
-  return a.rt.rb <= 8 ? 8 : (a.rt.rb <= 12 ? 12 : 16);
+  const int nb = a.rt.rb <= 8 ? 8 : (a.rt.rb <= 12 ? 12 : 16);
+  if (force && sizeof(Real) == sizeof(float) &&
+      !(a.rt.shared && a.rt.sigma && a.rt.shared_nb == nb))
+    return 0;
+  return nb;
 }
 
 #define SUS2SH_FOR_EACH_K(X) X(1) X(2) X(3) X(4) X(5) X(6)
@@ -505,8 +554,8 @@ bool launch_basics_l(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s)
   const int per_block = a.threads / a.npl;
   const int blocks = (a.N + per_block - 1) / per_block;
   const int T = a.threads;
-  const int nb = a.gate_m ? 0 : uniform_nb(a);  // the gated kernel uses the per-mu path
-  const size_t lds = table_bytes(a, nb);
+  const int nb = a.gate_m ? 0 : uniform_nb(a, false);  // the gated kernel uses the per-mu path
+  const size_t lds = table_bytes(a, nb, false);
 #define SUS2SH_BASICS_CASE(KK)                                                    \
   if (a.K == KK) {                                                                \
     if (a.gate_m)                                                                 \
@@ -537,9 +586,9 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   static_assert(kWinTile % 256 == 0, "a window tile holds whole force workgroups");
   // the parameter gradient, the gated kernel and deterministic summation use the per-mu path
   const bool fix = a.fix_flag != nullptr;
-  const int nb = (pg_rad || a.gate_m || fix) ? 0 : uniform_nb(a);
+  const int nb = (pg_rad || a.gate_m || fix) ? 0 : uniform_nb(a, true);
   const bool use_win = win && !pg_rad && !a.gate_m;
-  size_t lds = table_bytes(a, nb);
+  size_t lds = table_bytes(a, nb, true);
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
     lds = ((lds + 15) & ~size_t(15)) +

@@ -415,18 +415,34 @@ void Engine::upload_tables()
         radial_uniform_ = false;
         break;
       }
-  // matrix-core basic moments: c_p[mu][n] = sigma_p c[mu][n] (build_radial_tables)
-  mfma_ok_ = sizeof(Real) == sizeof(float) && kchunks_ == 1 &&
-             mfma_basics_supported(L, K, rb, m.species_count, radial_uniform_, m.gate_enabled);
-  if (mfma_ok_) {
-    std::vector<Real> rcm(static_cast<size_t>(Mr) * rb), sig(pairs);
-    for (size_t i = 0; i < rcm.size(); ++i) rcm[i] = static_cast<Real>(m.radial_coeffs[i]);
+  // every pair's coefficient rows are one shared matrix times the pair factor sigma_p
+  // (build_radial_tables): c_p[mu][n] = sigma_p c[mu][n]
+  {
+    std::vector<Real> sig(pairs);
     for (int ti = 0; ti < m.species_count; ++ti)
       for (int tj = 0; tj < m.species_count; ++tj)
         sig[ti * m.species_count + tj] = static_cast<Real>(
           m.scaling * m.radial_type_coeffs[ti] * m.radial_type_coeffs[tj]);
-    mfma_rc_.upload(rcm.data(), rcm.size() * sizeof(Real));
     pair_sigma_.upload(sig.data(), sig.size() * sizeof(Real));
+    // the uniform edge kernels' rows, zero-padded to their row length (8, 12 or 16)
+    shared_nb_ = rb <= 8 ? 8 : (rb <= 12 ? 12 : 16);
+    for (int c = 0; c < kchunks_; ++c) {
+      const int mu0 = kchunk_k0_[c] * (L + 1), mrc = kchunk_K_[c] * (L + 1);
+      std::vector<Real> sh(static_cast<size_t>(mrc) * shared_nb_, Real(0));
+      for (int mu = 0; mu < mrc; ++mu)
+        for (int n = 0; n < rb; ++n)
+          sh[static_cast<size_t>(mu) * shared_nb_ + n] =
+            static_cast<Real>(m.radial_coeffs[static_cast<size_t>(mu0 + mu) * rb + n]);
+      shared_rc_[c].upload(sh.data(), sh.size() * sizeof(Real));
+    }
+  }
+  // matrix-core basic moments: the shared matrix unpadded ([Mr][rb])
+  mfma_ok_ = sizeof(Real) == sizeof(float) && kchunks_ == 1 &&
+             mfma_basics_supported(L, K, rb, m.species_count, radial_uniform_, m.gate_enabled);
+  if (mfma_ok_) {
+    std::vector<Real> rcm(static_cast<size_t>(Mr) * rb);
+    for (size_t i = 0; i < rcm.size(); ++i) rcm[i] = static_cast<Real>(m.radial_coeffs[i]);
+    mfma_rc_.upload(rcm.data(), rcm.size() * sizeof(Real));
   }
 
   // each chunk's full (k,l,m) grid -> basic index (the kernels' full_index<L, Kc> with the
@@ -524,6 +540,17 @@ void Engine::upload_gate_tables()
   build_radial_tables(m, coeff, scal, &m.gate_radial_coeffs);
   std::vector<Real> cR(coeff.begin(), coeff.end());
   gate_coeff_.upload(cR.data(), cR.size() * sizeof(Real));
+  {
+    // shared rows of the gate coefficients (gate models have one k chunk)
+    const int Mr0 = m.radial_funcs_count, rb0 = m.rb_size;
+    const int nb = rb0 <= 8 ? 8 : (rb0 <= 12 ? 12 : 16);
+    std::vector<Real> sh(static_cast<size_t>(Mr0) * nb, Real(0));
+    for (int mu = 0; mu < Mr0; ++mu)
+      for (int n = 0; n < rb0; ++n)
+        sh[static_cast<size_t>(mu) * nb + n] =
+          static_cast<Real>(m.gate_radial_coeffs[static_cast<size_t>(mu) * rb0 + n]);
+    gate_shared_.upload(sh.data(), sh.size() * sizeof(Real));
+  }
 
   Model gm = m;  // same graph, the gate scalars as its scalar list
   const int Qg = static_cast<int>(m.gate_moment_index.size());
@@ -1026,6 +1053,9 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   el.type_s = ws_.type_s;
   el.rt.coeff = radial_coeff_[0].as<Real>();
   el.rt.scal = radial_scal_[0].as<Real>();
+  el.rt.shared = shared_rc_[0].as<Real>();
+  el.rt.sigma = pair_sigma_.as<Real>();
+  el.rt.shared_nb = shared_nb_;
   el.rt.S = model_.species_count;
   el.rt.rb = model_.rb_size;
   el.rt.rc = static_cast<Real>(model_.max_dist);
@@ -1038,6 +1068,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     ec.K = kchunk_K_[c];
     ec.rt.coeff = radial_coeff_[c].as<Real>();
     ec.rt.scal = radial_scal_[c].as<Real>();
+    ec.rt.shared = shared_rc_[c].as<Real>();
     ec.fmap = fmap_[c].as<int>();
     return ec;
   };
@@ -1050,6 +1081,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (gate) {
     EdgeLaunch<Real> eg = el;
     eg.rt.coeff = gate_coeff_.as<Real>();
+    eg.rt.shared = gate_shared_.as<Real>();
     if (!launch_basics<Real>(eg, static_cast<Real*>(ws_.gbasics), s))
       throw UnsupportedError("SUS2_SH: no basic-moment kernel for this (l_max, k_max).");
     gg.fwd_rows = g_fwd_rows_.as<int4>();
@@ -1324,6 +1356,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       throw UnsupportedError("SUS2_SH: gate graph tile does not fit in LDS.");
     EdgeLaunch<Real> e2 = el;
     e2.rt.coeff = gate_coeff_.as<Real>();
+    e2.rt.shared = gate_shared_.as<Real>();
     e2.v_accumulate = 1;
     if (!launch_forces<Real>(e2, static_cast<const Real*>(ws_.ggrads), ws_.f_s, ws_.v_s, nullptr,
                              win, win_cap, s))

@@ -254,6 +254,10 @@ class Engine {
   DeviceBuffer radial_coeff_[kMaxKChunks], radial_scal_[kMaxKChunks], fmap_[kMaxKChunks];
   // matrix-core basic moments: shared radial coefficients [Mr][rb] and pair factors [S * S]
   DeviceBuffer mfma_rc_, pair_sigma_;
+  // the uniform edge kernels' shared radial rows ([Mr_chunk][shared_nb_] per k chunk, and the
+  // gate layer's): coeff_p = sigma_p shared
+  DeviceBuffer shared_rc_[kMaxKChunks], gate_shared_;
+  int shared_nb_ = 8;
   bool mfma_ok_ = false;  // the model fits the matrix-core basic-moment kernel
   DeviceBuffer fwd_rows_, fwd_w_, fwd_terms_, fwd_layer_;
   DeviceBuffer bwd_chunks_, bwd_terms_, bwd_layer_;

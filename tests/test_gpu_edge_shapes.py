@@ -72,7 +72,7 @@ def test_edge_shape_matches_oracle(case, threads, npl, fp32):
             f"sus2_sh_force_shape={threads}x{npl}"]
     print(f"[case] edge shape {threads}x{npl} fp32={fp32} N={len(types)}", flush=True)
     got, (shape, times) = _evaluate(path, cell, types, pos, opts)
-    assert shape == {"basic": (threads, npl), "force": (threads, npl)}
+    assert shape == {"basic": (threads, npl), "force": (threads, npl), "basic_mfma": 0}
     assert times == {}  # fixed shapes: the edge autotuner does not run
     _compare(ref, got, fp32)
 
@@ -86,10 +86,15 @@ def test_edge_autotuner_and_batch(case, fp32):
     print(f"[case] edge autotuner fp32={fp32} N={len(types)}", flush=True)
     got, (shape, times) = _evaluate(path, cell, types, pos, ["Cu", "Zr", f"sus2_float={int(fp32)}"])
     print(f"  kept {shape}; times {times}", flush=True)
-    assert len(times) == 24 and all(ms > 0 for ms in times.values())
-    best_basic = min((ms, k) for k, ms in times.items() if k[0] == "basic")[1]
+    # 12 vector basic-kernel shapes, 12 force shapes and, in FP32, the matrix-core basic kernel
+    assert len(times) == (25 if fp32 else 24) and all(ms > 0 for ms in times.values())
+    best_basic = min((ms, k) for k, ms in times.items() if k[0] in ("basic", "basic_mfma_fp32"))[1]
     best_force = min((ms, k) for k, ms in times.items() if k[0] == "force")[1]
-    assert shape["basic"] == best_basic[1:] and shape["force"] == best_force[1:]
+    if best_basic[0] == "basic_mfma_fp32":
+        assert shape["basic_mfma"] == 1
+    else:
+        assert shape["basic_mfma"] == 0 and shape["basic"] == best_basic[1:]
+    assert shape["force"] == best_force[1:]
     _compare(ref, got, fp32)
 
     import torch

@@ -8,6 +8,7 @@
   python profile_summary.py steady <kernel_trace.csv> [--last 4]
   python profile_summary.py counters <counter_collection.csv> [--last 4] [--simds 1024]
          [--xcds 8]
+  python profile_summary.py waits <counter_collection.csv> [--last 4]
 
 `stats` prints the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (calls,
 average / total duration, share of GPU time).  `traffic` reads one `--pmc FETCH_SIZE` and one
@@ -22,7 +23,10 @@ GRBM_GUI_ACTIVE: the SQ counters are totals over the device (wave-instructions),
 GRBM_GUI_ACTIVE is summed over the XCDs, so one XCD's busy cycles are GRBM / --xcds.  It
 reports per dispatch the instructions per wave and the issue rates per SIMD-cycle (VALU) and
 per CU-cycle (LDS, SALU, SMEM); "VALU x4" is the fraction of SIMD cycles that the VALU
-instructions would fill at 4 cycles per wave64 instruction.
+instructions would fill at 4 cycles per wave64 instruction.  `waits` prints every other SQ
+counter of a run that also collected SQ_WAVES and SQ_WAVE_CYCLES as a fraction of the wave-cycles
+(e.g. SQ_WAIT_ANY: waiting on a dependency or barrier, SQ_WAIT_INST_ANY: ready but not issued,
+SQ_ACTIVE_INST_ANY: issuing).
 """
 import argparse
 import collections
@@ -141,6 +145,29 @@ def counters(path, last, simds, xcds):
               f"{m.get('SQ_INSTS_SALU', 0) / (cus * cyc):8.3f}")
 
 
+def waits(path, last):
+    """Per kernel: every SQ counter of the run except SQ_WAVES / SQ_WAVE_CYCLES as a fraction
+    of SQ_WAVE_CYCLES (wave-cycles), means of the last `last` dispatches."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        per[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    rows = []
+    for name, d in per.items():
+        if "SQ_WAVE_CYCLES" not in d or "SQ_WAVES" not in d:
+            continue
+        m = {c: sum(v[-last:]) / len(v[-last:]) for c, v in d.items()}
+        if m["SQ_WAVE_CYCLES"] <= 0 or m["SQ_WAVE_CYCLES"] / max(m["SQ_WAVES"], 1.0) < 5e3:
+            continue
+        rows.append((name, m))
+    cols = sorted({c for _, m in rows for c in m
+                   if c.startswith("SQ_") and c not in ("SQ_WAVES", "SQ_WAVE_CYCLES")})
+    print(f"{'kernel':72s} {'cyc/wave':>9s} " + " ".join(f"{c[3:][:14]:>14s}" for c in cols))
+    for name, m in sorted(rows, key=lambda r: -r[1]["SQ_WAVE_CYCLES"]):
+        wc = m["SQ_WAVE_CYCLES"]
+        print(f"{name[:72]:72s} {wc / max(m['SQ_WAVES'], 1.0):9.0f} " +
+              " ".join(f"{m.get(c, 0.0) / wc:14.3f}" for c in cols))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -162,8 +189,13 @@ def main():
     c.add_argument("--last", type=int, default=4)
     c.add_argument("--simds", type=int, default=1024)
     c.add_argument("--xcds", type=int, default=8)
+    w = sub.add_parser("waits")
+    w.add_argument("csv")
+    w.add_argument("--last", type=int, default=4)
     a = ap.parse_args()
-    if a.cmd == "stats":
+    if a.cmd == "waits":
+        waits(a.csv, a.last)
+    elif a.cmd == "stats":
         stats(a.csv, a.top)
     elif a.cmd == "steady":
         steady(a.csv, a.last)

@@ -307,7 +307,10 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
 // coalesced loads, then every lane scans its three cells there.  A band with more than
 // kStageCap atoms, and a wave over more than two rows, read the candidates from global
 // memory as k_neighbors does.
-constexpr int kStageCap = 512;  // staged candidate atoms per band (16 KiB)
+// a band of the benchmark's fcc alloy holds ~80-160 atoms (one or two rows of ~10 cells): 256
+// staged atoms (8 KiB) cover it and keep the workgroup's LDS small enough for ~16 waves per CU
+// (512: 0.87 ms, 256: 0.82 ms, 128: 0.83 ms at 1,024,000 atoms, FP32)
+constexpr int kStageCap = 256;  // staged candidate atoms per band (8 KiB)
 constexpr int kStageTab = 128;  // staged cells per band
 
 template <typename Real>

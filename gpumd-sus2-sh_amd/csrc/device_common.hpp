@@ -430,6 +430,60 @@ __device__ __forceinline__ void solid_values_grads(Real x, Real y, Real z,
   }
 }
 
+// w grad S_lm added to (gx, gy, gz) from the values of degree l - 1: the gradient of a
+// Racah-normalised real regular solid harmonic is an exact combination of at most two
+// harmonics of the next lower degree (with A = sqrt((l+|m|)(l+|m|-1)) / 2,
+// B = sqrt((l-|m|)(l-|m|-1)) / 2):
+//   dz S_lm = sqrt((l+m)(l-m)) S_{l-1,m}
+//   m = 0:  dx S_l0 = -sqrt(l(l-1)/2) S_{l-1,1},  dy S_l0 = -sqrt(l(l-1)/2) S_{l-1,-1}
+//   m > 0:  dx S_lm = A S_{l-1,m-1} - B S_{l-1,m+1},  dy S_lm = -A S_{l-1,-(m-1)} - B S_{l-1,-(m+1)}
+//   m < 0:  dx S_lm = A S_{l-1,m+1} - B S_{l-1,m-1},  dy S_lm =  A S_{l-1,-m-1} + B S_{l-1,-m+1}
+// where a term whose harmonic does not exist (|m'| > l - 1) is absent, and a term on
+// S_{l-1,0} reached from |m| = 1 carries A sqrt(2) in dx (m = 1) or dy (m = -1) only.
+// Checked against central differences of solid_values for l <= 6.
+template <typename Real, int L>
+__device__ __forceinline__ void add_solid_grad(int l, int m, Real w,
+                                               const Real (&S)[(L + 1) * (L + 1)], Real& gx,
+                                               Real& gy, Real& gz)
+{
+  if (l == 0) return;
+  const int mu = m < 0 ? -m : m;
+  if (mu <= l - 1) gz += w * Real(__builtin_sqrt(double((l + m) * (l - m)))) * S[sh_index(l - 1, m)];
+  if (m == 0) {
+    if (l >= 2) {
+      const Real c = Real(-__builtin_sqrt(double(l * (l - 1)) / 2.0));
+      gx += w * c * S[sh_index(l - 1, 1)];
+      gy += w * c * S[sh_index(l - 1, -1)];
+    }
+    return;
+  }
+  const double A = 0.5 * __builtin_sqrt(double((l + mu) * (l + mu - 1)));
+  const double B = l - mu - 1 >= 0 ? 0.5 * __builtin_sqrt(double((l - mu) * (l - mu - 1))) : 0.0;
+  if (m > 0) {
+    if (mu == 1) {
+      gx += w * Real(A * 1.4142135623730951) * S[sh_index(l - 1, 0)];
+    } else {
+      gx += w * Real(A) * S[sh_index(l - 1, mu - 1)];
+      gy -= w * Real(A) * S[sh_index(l - 1, -(mu - 1))];
+    }
+    if (mu + 1 <= l - 1) {
+      gx -= w * Real(B) * S[sh_index(l - 1, mu + 1)];
+      gy -= w * Real(B) * S[sh_index(l - 1, -(mu + 1))];
+    }
+  } else {
+    if (mu == 1) {
+      gy += w * Real(A * 1.4142135623730951) * S[sh_index(l - 1, 0)];
+    } else {
+      gx += w * Real(A) * S[sh_index(l - 1, -(mu - 1))];
+      gy += w * Real(A) * S[sh_index(l - 1, mu - 1)];
+    }
+    if (mu + 1 <= l - 1) {
+      gx -= w * Real(B) * S[sh_index(l - 1, -(mu + 1))];
+      gy += w * Real(B) * S[sh_index(l - 1, mu + 1)];
+    }
+  }
+}
+
 template <int L>
 __device__ __forceinline__ constexpr double ylm_norm()
 {

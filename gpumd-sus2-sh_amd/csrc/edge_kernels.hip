@@ -359,8 +359,8 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
     else
       eval_radial<Real, MR, true, NB>(s_coeff + pair * cs, s_scal + pair * ss, rb, rc,
                                     a.rt.kind, r, R, dR);
-    Real Sv[NC], Gx[NC], Gy[NC], Gz[NC];
-    solid_values_grads<Real, L>(ux, uy, uz, Sv, Gx, Gy, Gz);
+    Real Sv[NC];
+    solid_values<Real, L>(ux, uy, uz, Sv);
     if constexpr (GATE) {
       const Real* gm = a.gate_m + static_cast<size_t>(jn) * MR;
       const Real* gd = a.gate_d + static_cast<size_t>(jn) * MR;
@@ -400,10 +400,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
         }
         const int y = sh_index(l, m);
         sv += (V - Real(l) * inv * U) * nl * Sv[y];
-        const Real un = U * nl;
-        gx += un * Gx[y];
-        gy += un * Gy[y];
-        gz += un * Gz[y];
+        add_solid_grad<Real, L>(l, m, U * nl, Sv, gx, gy, gz);
       }
     }
     if (PG) {

@@ -601,51 +601,100 @@ __global__ __launch_bounds__(64) void k_windows(int N, CellGrid g, const int* __
     ce[c] = s1;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  // insertion sort by start, then merge overlapping / touching ranges
-  for (int i = 1; i < total; ++i) {
-    const int a = cs[i], b = ce[i];
-    int j = i - 1;
-    while (j >= 0 && cs[j] > a) {
-      cs[j + 1] = cs[j];
-      ce[j + 1] = ce[j];
-      --j;
-    }
-    cs[j + 1] = a;
-    ce[j + 1] = b;
+  // Sort the ranges by start (rank of each among the non-empty ones, ties by index) and merge
+  // overlapping or touching ranges -- the whole wave at once: every lane ranks its two ranges
+  // against all of them (broadcast LDS reads), then scans the sorted list.  Empty ranges rank
+  // last and are dropped.
+  __shared__ int ss[kWinCand], se[kWinCand];
+  constexpr int kNone = 0x7fffffff;
+  const int lane = threadIdx.x;
+  int a0 = kNone, b0 = 0, a1 = kNone, b1 = 0;
+  if (lane < total && ce[lane] > cs[lane]) {
+    a0 = cs[lane];
+    b0 = ce[lane];
   }
-  int count = 0, size = 0, cur_s = 0, cur_e = -1;
-  for (int i = 0; i < total; ++i) {
-    if (ce[i] <= cs[i]) continue;  // empty
-    if (cur_e >= cs[i]) {
-      cur_e = max(cur_e, ce[i]);
-      continue;
-    }
-    if (cur_e > cur_s) {
-      if (count == kWinMax) {
-        out[0] = -1;
-        return;
-      }
-      out[2 + 2 * count] = cur_s;
-      out[3 + 2 * count] = cur_e - cur_s;
-      size += cur_e - cur_s;
-      ++count;
-    }
-    cur_s = cs[i];
-    cur_e = ce[i];
+  if (lane + 64 < total && ce[lane + 64] > cs[lane + 64]) {
+    a1 = cs[lane + 64];
+    b1 = ce[lane + 64];
   }
-  if (cur_e > cur_s) {
-    if (count == kWinMax) {
-      out[0] = -1;
-      return;
-    }
-    out[2 + 2 * count] = cur_s;
-    out[3 + 2 * count] = cur_e - cur_s;
-    size += cur_e - cur_s;
-    ++count;
+  int r0 = 0, r1 = 0;
+  for (int j = 0; j < total; ++j) {
+    const int sj = ce[j] > cs[j] ? cs[j] : kNone;
+    r0 += (sj < a0 || (sj == a0 && j < lane)) ? 1 : 0;
+    r1 += (sj < a1 || (sj == a1 && j < lane + 64)) ? 1 : 0;
   }
-  out[0] = size <= cap ? count : -1;
-  out[1] = size;
+  if (a0 != kNone) {
+    ss[r0] = a0;
+    se[r0] = b0;
+  }
+  if (a1 != kNone) {
+    ss[r1] = a1;
+    se[r1] = b1;
+  }
+  int nvalid = (a0 != kNone ? 1 : 0) + (a1 != kNone ? 1 : 0);
+  for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_xor(nvalid, off, 64);
+  __syncthreads();
+  // sorted element k = 2 lane + v (v = 0, 1): it starts a merged window when its start lies
+  // beyond the end of every earlier range (the exclusive prefix max of the ends); a window's
+  // end is the prefix max of the ends at its last element
+  const int k1 = 2 * lane + 1;
+  const bool v0 = 2 * lane < nvalid, v1 = k1 < nvalid;
+  const int s0 = v0 ? ss[2 * lane] : 0, e0 = v0 ? se[2 * lane] : -1;
+  const int s1 = v1 ? ss[k1] : 0, e1 = v1 ? se[k1] : -1;
+  int incl = max(e0, e1);
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl = max(incl, t);
+  }
+  int excl = __shfl_up(incl, 1, 64);
+  if (lane == 0) excl = -1;
+  const int end0 = max(excl, e0), end1 = max(end0, e1);
+  const bool st0 = v0 && s0 > excl;
+  const bool st1 = v1 && s1 > end0;
+  // windows started up to and including element k1 (inclusive prefix count)
+  int cst = (st0 ? 1 : 0) + (st1 ? 1 : 0);
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(cst, off, 64);
+    if (lane >= off) cst += t;
+  }
+  const int count = __shfl(cst, 63, 64);
+  const int win1 = cst - 1;                  // window of element k1
+  const int win0 = cst - (st1 ? 1 : 0) - 1;  // window of element k0
+  // start of the window of element k1: starts increase along the sorted list, so it is the
+  // prefix max of the starts seen so far
+  int gs = st1 ? s1 : (st0 ? s0 : -1);
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(gs, off, 64);
+    if (lane >= off) gs = max(gs, t);
+  }
+  int gs_prev = __shfl_up(gs, 1, 64);
+  if (lane == 0) gs_prev = -1;
+  const int gs0 = st0 ? s0 : gs_prev;  // start of the window of element k0
+  const int gs1 = st1 ? s1 : gs0;
+  // element k closes its window when element k + 1 starts one or k is the last
+  const int next_st = __shfl_down(st0 ? 1 : 0, 1, 64);  // st0 of the next lane
+  const bool last0 = v0 && (!v1 || st1);
+  const bool last1 = v1 && (k1 + 1 >= nvalid || next_st != 0);
+  if (count > kWinMax) {
+    if (lane == 0) out[0] = -1;
+    return;
+  }
+  int sz = 0;
+  if (last0) {
+    out[2 + 2 * win0] = gs0;
+    out[3 + 2 * win0] = end0 - gs0;
+    sz += end0 - gs0;
+  }
+  if (last1) {
+    out[2 + 2 * win1] = gs1;
+    out[3 + 2 * win1] = end1 - gs1;
+    sz += end1 - gs1;
+  }
+  for (int off = 32; off > 0; off >>= 1) sz += __shfl_xor(sz, off, 64);
+  if (lane == 0) {
+    out[0] = sz <= cap ? count : -1;
+    out[1] = sz;
+  }
 }
 
 inline int blocks_for(size_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }

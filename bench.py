@@ -102,7 +102,7 @@ def run_engine(path, cell, types, pos, precision, steps, warmup, profile, extra=
     dt = time.perf_counter() - t0
     nbar = pot.mean_neighbors()
     info = pot.info()
-    tuning = {(f"v{v}_w{w}" if v < 3 else f"lane{w}"): round(ms, 4)
+    tuning = {(f"v{v}_w{w}" if v < 3 else f"lane{w}" if v == 3 else f"block{w}"): round(ms, 4)
               for (v, w), ms in pot.tuning().items()}
     shapes, _ = pot.edge_shapes()
     mfma = ("", "+fp32-mfma", "+bf16-mfma")[shapes.pop("basic_mfma")]
@@ -408,7 +408,8 @@ def main():
     achieved = flops[dominant] * N / (stages[dominant] * 1e-3) / 1e12
     total_flops = (flops["basic"] + flops["graph"] + flops["force"]) * N
     kernel_name = {"basic": "k_basics", "force": "k_forces",
-                   "graph": "k_lane_graph" if info["graph_variant"] == 3 else "k_graph"}[dominant]
+                   "graph": {3: "k_lane_graph", 4: "k_block_graph"}.get(info["graph_variant"],
+                                                                       "k_graph")}[dominant]
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as fh:

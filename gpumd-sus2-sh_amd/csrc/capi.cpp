@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/sus2sh.h"
+#include "block_program.hpp"
 #include "engine.hpp"
 #include "lane_program.hpp"
 #include "model.hpp"
@@ -410,6 +411,34 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot,
     for (int i = 0; i < n; ++i)
       e_prime[i] = sus2sh::run_lane_records_host(p, rec, basics + static_cast<size_t>(i) * B,
                                                  grad + static_cast<size_t>(i) * B);
+  });
+}
+
+int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_bytes, int n,
+                       const double* basics, double* e_prime, double* grad, int* info)
+{
+  if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
+      (waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16) ||
+      (real_bytes != 4 && real_bytes != 8))
+    return SUS2SH_ERR_ARGUMENT;
+  return guarded(&g_create_error, [&] {
+    const sus2sh::Model m = sus2sh::load_model(mtp_path);
+    const sus2sh::BlockProgram p = sus2sh::compile_block_program(m, waves, real_bytes, max_slots);
+    if (info) {
+      info[0] = p.ok ? 1 : 0;
+      info[1] = p.slots;
+      info[2] = p.batches;
+      info[3] = p.tensors;
+      info[4] = p.fwd_blocks;
+      info[5] = static_cast<int>(p.fwd_terms);
+      info[6] = static_cast<int>(p.adj_terms);
+      info[7] = static_cast<int>(p.bwd_terms);
+    }
+    if (!p.ok) return;
+    const int B = m.basic_count;
+    for (int i = 0; i < n; ++i)
+      e_prime[i] = sus2sh::run_block_program_host(p, basics + static_cast<size_t>(i) * B,
+                                                  grad + static_cast<size_t>(i) * B);
   });
 }
 

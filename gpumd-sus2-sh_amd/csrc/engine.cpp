@@ -85,8 +85,14 @@ EngineOptions parse_options(const Model& m, int nopts, const char** opts)
       o.autotune = parse_bool(val, key);
     } else if (key == "sus2_sh_graph_variant") {
       const int v = std::atoi(val.c_str());
-      if (v < 0 || v > 3) throw InputError("sus2_sh_graph_variant must be 0, 1, 2 or 3: " + val);
+      if (v < 0 || v > 4 || val.empty() || val.find_first_not_of("0123456789") != std::string::npos)
+        throw InputError("sus2_sh_graph_variant must be 0, 1, 2, 3 or 4: " + val);
       o.graph_variant = v;
+    } else if (key == "sus2_sh_block_waves") {
+      const int w = std::atoi(val.c_str());
+      if ((w != 4 && w != 8 && w != 16) || val.find_first_not_of("0123456789") != std::string::npos)
+        throw InputError("sus2_sh_block_waves must be 4, 8 or 16: " + val);
+      o.block_waves = w;
     } else if (key == "sus2_sh_lane_waves") {
       const int w = std::atoi(val.c_str());
       if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16)
@@ -246,6 +252,16 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
   }
 
   compile_lane();
+  block_ = build_block(opt_.block_waves > 0 ? opt_.block_waves : block_waves_list()[0]);
+  if (opt_.graph_variant == 4) {
+    if (!block_.ok)
+      throw InputError("sus2_sh_graph_variant=4: no block program for this model (" + block_.reason +
+                       ").");
+    if (opt_.deterministic)
+      throw InputError("sus2_sh_graph_variant=4 sums basic adjoints with atomics; it is not "
+                       "available with sus2_sh_deterministic=1.");
+    tuned_ = true;  // a fixed variant disables the autotuner
+  }
   if (opt_.graph_variant == 3) {
     if (!lane_.ok)
       throw InputError("sus2_sh_graph_variant=3: no lane program for this model (" + lane_.reason +
@@ -257,6 +273,10 @@ Engine::Engine(const std::string& path, int num_atoms, int nopts, const char** o
     // lane program when the model has one, else the balanced-stream row kernel
     tuned_ = true;
     if (opt_.graph_variant < 0) variant_ = lane_.ok ? 3 : 1;
+  } else if (opt_.graph_variant < 0) {
+    // until the autotuner has timed the candidates (or when it does not run): the block
+    // program, else the lane program, else the balanced-stream row kernel
+    variant_ = block_.ok ? 4 : (lane_.ok ? 3 : 1);
   }
   // edge-kernel shapes: fixed ones stay; deterministic mode keeps one lane per atom (one
   // summation order for every atom count), else the autotuner or the size rule decides
@@ -304,6 +324,37 @@ int Engine::param_count() const
 
 // Lane program of one shape: the smallest per-lane slot budget >= min_slots that holds every
 // graph component (160 slots are 40 KiB of LDS per 64-atom FP32 tile).
+const std::vector<int>& Engine::block_waves_list()
+{
+  static const std::vector<int> waves = {8, 16, 4};
+  return waves;
+}
+
+BlockProgram Engine::build_block(int waves) const
+{
+  const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
+  const int max_slots = static_cast<int>(max_lds_ / (64 * rsz));
+  BlockProgram p = compile_block_program(model_, waves, static_cast<int>(rsz), max_slots);
+  if (p.ok) {
+    BlockTable bt{};
+    bt.basics = model_.basic_count;
+    bt.slots = p.slots;
+    bt.v_base = p.v_base;
+    bt.waves = waves;
+    if (block_graph_lds(bt, static_cast<int>(rsz)) > max_lds_) {
+      p.ok = false;
+      p.reason = "block program exceeds the LDS of a workgroup";
+    }
+  }
+  return p;
+}
+
+void Engine::upload_block()
+{
+  block_seg_.upload(block_.seg.data(), block_.seg.size() * sizeof(int));
+  block_words_.upload(block_.words.data(), std::max<size_t>(block_.words.size(), 1) * sizeof(uint32_t));
+}
+
 LaneProgram Engine::build_lane(const LaneShape& shape) const
 {
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
@@ -525,6 +576,7 @@ void Engine::upload_tables()
   stored_index_.upload(g.stored_scalar_index.data(), g.stored_scalar_index.size() * sizeof(int));
   stored_slot_.upload(g.stored_scalar_slot.data(), g.stored_scalar_slot.size() * sizeof(int));
   if (lane_.ok) upload_lane();
+  if (block_.ok) upload_block();
   if (m.gate_enabled) upload_gate_tables<Real>();
 }
 
@@ -842,6 +894,21 @@ template <typename Real>
 bool Engine::launch_graph_variant(int variant, int tile, int N, const GraphTable<Real>& gt,
                                   hipStream_t s)
 {
+  if (variant == 4) {
+    if (!block_.ok) return false;
+    BlockTable bt{};
+    bt.seg = block_seg_.as<int>();
+    bt.words = block_words_.as<uint32_t>();
+    bt.shift = shift_.as<double>();
+    bt.sigma = sigma_.as<double>();
+    bt.batches = block_.batches;
+    bt.basics = model_.basic_count;
+    bt.v_base = block_.v_base;
+    bt.slots = block_.slots;
+    bt.waves = block_.waves;
+    return launch_block_graph<Real>(N, static_cast<const Real*>(ws_.basics), ws_.type_s, bt,
+                                    ws_.e_s, static_cast<Real*>(ws_.grads), s, max_lds_);
+  }
   if (variant == 3) {
     if (!lane_.ok) return false;
     LaneTable<Real> lt{};
@@ -1231,8 +1298,11 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     int best_tile = tile_, best_variant = variant_;
     const std::vector<LaneShape>& shapes = lane_shapes();
     const int n_lane = opt_.lane_nslot > 0 ? 1 : static_cast<int>(shapes.size());
-    tune_ms_.assign(15 + shapes.size(), -1.0f);
+    const std::vector<int>& bwaves = block_waves_list();
+    const int n_block = opt_.block_waves > 0 ? 1 : static_cast<int>(bwaves.size());
+    tune_ms_.assign(15 + shapes.size() + bwaves.size(), -1.0f);
     LaneProgram best_lane = lane_;
+    BlockProgram best_block = block_;
     auto time_it = [&](int v, int t) {
       float ms = 0.0f;
       if (!launch_graph_variant<Real>(v, t, N, gt, s)) return -1.0f;
@@ -1243,11 +1313,20 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
       hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
       return ms / 2;
     };
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < 5; ++v) {
       if (opt_.graph_variant >= 0 && v != opt_.graph_variant) continue;  // fixed variant
-      for (int li = 0; li < (v == 3 ? n_lane : 5); ++li) {
+      for (int li = 0; li < (v == 4 ? n_block : v == 3 ? n_lane : 5); ++li) {
         const int t = 1 << li;
         if (v < 3 && graph_lds_bytes<Real>(t, prog_.slot_count) > max_lds_) continue;
+        if (v == 4) {
+          if (opt_.block_waves <= 0) {
+            block_ = build_block(bwaves[li]);
+            if (!block_.ok) continue;
+            upload_block();
+          } else if (!block_.ok) {
+            continue;
+          }
+        }
         if (v == 3) {
           if (opt_.lane_nslot <= 0) {
             lane_ = build_lane(shapes[li]);
@@ -1259,12 +1338,13 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
         }
         const float ms = time_it(v, t);
         if (ms < 0.0f) continue;
-        tune_ms_[v == 3 ? 15 + li : v * 5 + li] = ms;
+        tune_ms_[v == 4 ? 15 + static_cast<int>(shapes.size()) + li : v == 3 ? 15 + li : v * 5 + li] = ms;
         if (ms < best) {
           best = ms;
           best_tile = t;
           best_variant = v;
           if (v == 3) best_lane = lane_;
+          if (v == 4) best_block = block_;
         }
       }
     }
@@ -1272,6 +1352,10 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
     (void)hipEventDestroy(t1);
     if (best_variant < 3) tile_ = best_tile;
     variant_ = best_variant;
+    if (best_block.ok && (best_block.waves != block_.waves || !block_.ok)) {
+      block_ = std::move(best_block);
+      upload_block();
+    }
     if (best_lane.ok && (best_lane.waves != lane_.waves || best_lane.nslot != lane_.nslot ||
                          best_lane.slots != lane_.slots || best_lane.atoms != lane_.atoms)) {
       lane_ = std::move(best_lane);

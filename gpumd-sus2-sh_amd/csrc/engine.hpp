@@ -16,6 +16,7 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "block_program.hpp"
 #include "lane_program.hpp"
 #include "program.hpp"
 #include "zbl.hpp"
@@ -59,7 +60,8 @@ struct EngineOptions {
   bool use_float = true;
   int graph_tile = 0;  // 0: automatic
   int graph_variant = -1;  // -1: automatic; 0 rows, 1 balanced streams, 2 round-robin,
-                          // 3 lane program
+                          // 3 lane program, 4 block program
+  int block_waves = 0;   // block kernel: waves per 64-atom tile (0: autotuned, 4, 8 or 16)
   bool profile = false;
   bool autotune = true;  // time every feasible graph tile on the first call
   int lane_waves = 0;    // lane kernel: waves per 64-atom tile (0: autotuned shape)
@@ -162,6 +164,7 @@ class Engine {
   const Model& model() const { return model_; }
   const GraphProgram& program() const { return prog_; }
   const LaneProgram& lane_program() const { return lane_; }
+  const BlockProgram& block_program() const { return block_; }
   const ZblTables& zbl() const { return zbl_; }
   const EngineOptions& options() const { return opt_; }
   int graph_tile() const { return tile_; }
@@ -200,6 +203,10 @@ class Engine {
   LaneProgram build_lane(const LaneShape& shape) const;
   void compile_lane();
   void upload_lane();
+  // block program: waves per tile the autotuner tries (the first is the default)
+  static const std::vector<int>& block_waves_list();
+  BlockProgram build_block(int waves) const;
+  void upload_block();
   template <typename Real>
   bool launch_graph_variant(int variant, int tile, int N, const GraphTable<Real>& gt,
                             hipStream_t s);
@@ -222,7 +229,8 @@ class Engine {
   int gate_tile_ = 1;
   EngineOptions opt_;
   int tile_ = 1;
-  int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane
+  int variant_ = 1;  // 0: row mode, 1: balanced streams, 2: round-robin streams, 3: lane,
+                     // 4: block
   bool tuned_ = false;
   // edge-kernel shapes: fixed by options, else tuned once on the first eligible call (the
   // size rule default_edge_shape() until then)
@@ -230,10 +238,11 @@ class Engine {
   bool edge_tuned_ = false;
   std::vector<float> edge_tune_ms_;
   static EdgeShape default_edge_shape(int N);
-  // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2 and
-  // [15 + c] for lane-program shape c
+  // graph-kernel ms per candidate, [variant * 5 + log2(tile)] for variants 0..2,
+  // [15 + c] for lane-program shape c and [27 + c] for block-program shape c
   std::vector<float> tune_ms_;
   LaneProgram lane_;
+  BlockProgram block_;
   int cap_ = 0;
   bool cap_known_ = false;  // cap_ was validated by a synchronised call with cap_atoms_ atoms
   int cap_atoms_ = -1;
@@ -266,6 +275,7 @@ class Engine {
     bwd_slen_[2];
   DeviceBuffer row_sbegin_, row_slist_, stored_index_, stored_slot_;
   DeviceBuffer lane_segs_, lane_ins_;
+  DeviceBuffer block_seg_, block_words_;
   DeviceBuffer win_;  // neighbour windows of the force tiles
   ZblTables zbl_;     // model.zbl_enabled: atomic numbers and pair cutoffs
   DeviceBuffer zbl_z_, zbl_inner_, zbl_outer_;

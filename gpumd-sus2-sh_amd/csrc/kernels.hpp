@@ -258,6 +258,27 @@ template <typename Real>
 bool launch_lane_graph(int N, const Real* basics, const int* type_s, const LaneTable<Real>& lt,
                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);
 
+// Block graph kernel (block_kernel.hip, program block_program.hpp): one 64-atom tile per
+// workgroup of `waves` (4, 8 or 16) waves.
+struct BlockTable {
+  const int* seg;          // [batches * waves * 4] word ranges of the FWD / TENSOR epochs
+  const uint32_t* words;   // run-encoded records
+  const double* shift;     // [S]
+  const double* sigma;     // [S]
+  int batches;
+  int basics;
+  int v_base;              // first tensor-value slot (reused for the per-wave energies)
+  int slots;               // per-lane slots of the program
+  int waves;
+};
+// LDS bytes of one workgroup: the basic adjoints (basics x 64 doubles) and the program's Real
+// slots (at least v_base + waves)
+size_t block_graph_lds(const BlockTable& bt, int real_bytes);
+// false when the tile's LDS would exceed max_lds or `waves` has no kernel
+template <typename Real>
+bool launch_block_graph(int N, const Real* basics, const int* type_s, const BlockTable& bt,
+                        double* e_s, Real* grads, hipStream_t s, size_t max_lds);
+
 // --- two-layer gate (gate_kernels.hip) ---
 // Gate values h_{a,mu} = sum_q weff[mu][q] s_q(a) over the gate scalars s_q of the gate
 // program (stored slots), G = 1 + A tanh(a_t h).

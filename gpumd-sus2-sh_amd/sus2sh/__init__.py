@@ -66,6 +66,33 @@ def lane_check(path, basics=None, max_slots=160, waves=1, nslot=0, real_bytes=0,
     return d, e, g
 
 
+def block_check(path, basics=None, max_slots=1024, waves=8, real_bytes=4):
+    """Host-only run of the block program (the block graph kernel's device words, graph
+    variant 4) for basic vectors basics [n, B].  Returns (info dict, e_prime [n], grad [n, B]);
+    e_prime / grad are None when the model has no block program."""
+    import ctypes
+    import numpy as np
+    from ._capi import check
+    P = ctypes.POINTER(ctypes.c_double)
+    info = (ctypes.c_int * 8)()
+    args = (str(path).encode(), int(max_slots), int(waves), int(real_bytes))
+    if basics is None:
+        check(lib().sus2sh_block_check(*args, 0, None, None, None, info))
+        e = g = None
+    else:
+        b = np.ascontiguousarray(np.atleast_2d(basics), dtype=np.float64)
+        e = np.zeros(b.shape[0])
+        g = np.zeros_like(b)
+        check(lib().sus2sh_block_check(*args, b.shape[0], b.ctypes.data_as(P),
+                                       e.ctypes.data_as(P), g.ctypes.data_as(P), info))
+    keys = ("ok", "slots", "batches", "tensors", "fwd_blocks", "fwd_terms", "adj_terms",
+            "bwd_terms")
+    d = dict(zip(keys, list(info)))
+    if not d["ok"]:
+        e = g = None
+    return d, e, g
+
+
 def zbl_tables(path, symbols=()):
     """Host-only ZBL tables of a zbl_enabled model (sus2sh_zbl_tables): (atomic numbers [S],
     inner [S, S], outer [S, S]); `symbols` are the element symbols that follow the model file

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "sus2sh_graph_free",
     "sus2sh_model_check",
     "sus2sh_lane_check",
+    "sus2sh_block_check",
     "sus2sh_zbl_tables",
 )
 
@@ -186,6 +187,9 @@ def lib():
     L.sus2sh_lane_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
                                     P(c_double), P(c_double), P(c_double), P(c_int)]
     L.sus2sh_lane_check.restype = c_int
+    L.sus2sh_block_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, P(c_double),
+                                     P(c_double), P(c_double), P(c_int)]
+    L.sus2sh_block_check.restype = c_int
     L.sus2sh_zbl_tables.argtypes = [c_char_p, c_int, P(c_char_p), P(c_int), P(c_double),
                                     P(c_double)]
     L.sus2sh_zbl_tables.restype = c_int

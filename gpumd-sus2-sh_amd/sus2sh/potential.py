@@ -90,13 +90,14 @@ class SUS2_SH:
 
     def tuning(self):
         """Autotuner graph-kernel times {(variant, tile): ms} (variant 0 rows, 1 balanced
-        streams, 2 round-robin streams; variant 3 is the lane program, keyed (3, shape))."""
+        streams, 2 round-robin streams; variant 3 is the lane program, keyed (3, shape);
+        variant 4 the block program, keyed (4, shape) with shapes 0, 1, 2 = 8, 16, 4 waves)."""
         buf = (ctypes.c_float * 32)()
         n = lib().sus2sh_get_tuning(self._h, buf, 32)
         out = {(k // 5, 1 << (k % 5)): buf[k] for k in range(min(n, 15)) if buf[k] >= 0}
         for k in range(15, n):
             if buf[k] >= 0:
-                out[(3, k - 15)] = buf[k]
+                out[(3, k - 15) if k < 27 else (4, k - 27)] = buf[k]
         return out
 
     def edge_shapes(self):

@@ -222,7 +222,9 @@ int sus2sh_get_stage_times(const sus2sh_handle* handle, float* ms, int n);
  * (8,2,208), then with two atoms per lane (8,2,160), (4,2,160), (8,2,208) at c = 7..9, then
  * sixteen waves per tile (16,2,192), (16,2,320) at c = 10, 11.  When
  * the lane shape is fixed (sus2_sh_lane_waves / sus2_sh_lane_shape) only
- * that shape is timed and its time is in ms[15].  Returns the count written. */
+ * that shape is timed and its time is in ms[15].  ms[27 + c] for the block program
+ * (variant 4) with 8, 16, 4 waves per 64-atom tile at c = 0, 1, 2 (ms[27] only when
+ * sus2_sh_block_waves fixes the shape).  Returns the count written. */
 int sus2sh_get_tuning(const sus2sh_handle* handle, float* ms, int n);
 
 /* Launch shapes of the edge kernels (no reference counterpart: the reference launches one
@@ -290,6 +292,16 @@ int sus2sh_model_check(const char* mtp_path, int* dims);
 int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot, int real_bytes,
                       int atoms, int n, const double* basics, double* e_prime, double* grad,
                       int* info);
+
+/* Host check of the block graph kernel's program (sus2_sh_graph_variant=4; no reference
+ * counterpart): compile it for the model with `waves` (1, 2, 4, 8 or 16) waves per 64-atom
+ * tile, coefficients of real_bytes (4 or 8) and at most max_slots per-lane slots, and run its
+ * device words on the host for n basic vectors basics[n][B] -> e_prime[n] and grad[n][B] as
+ * sus2sh_lane_check does.  info[0..8) = ok, slots, batches, quadratic-form tensors, layer-1
+ * blocks, forward terms, adjoint terms, reverse terms (per atom).  When the model has no
+ * block program (info[0] = 0) nothing is evaluated.  Returns a status code. */
+int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_bytes, int n,
+                       const double* basics, double* e_prime, double* grad, int* info);
 
 #ifdef __cplusplus
 }

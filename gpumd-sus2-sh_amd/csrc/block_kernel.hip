@@ -15,6 +15,7 @@
 // and a record costs two to four scalar words plus its partners.
 #include <utility>
 
+#include "block_program.hpp"
 #include "cg_table.hpp"
 #include "kernels.hpp"
 
@@ -45,21 +46,23 @@ __device__ __forceinline__ void lds_add(double* p, double v)
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Compile-time flags of the entries of block type T: the first entry of each output
-// component M, and the first entry of each (m1, m2) pair.
-template <int T>
+// Compile-time flags of the entries of block type T with output component M in [M0, M1): the
+// first such entry of each M, and the first of each (m1, m2) pair.
+template <int T, int M0, int M1>
 struct CgFirst {
   struct F {
     bool m[CgTable<T>::n];
     bool p[CgTable<T>::n];
   };
+  static constexpr bool in(int i) { return CgTable<T>::e[i].M >= M0 && CgTable<T>::e[i].M < M1; }
   static constexpr F make()
   {
     F f{};
     for (int i = 0; i < CgTable<T>::n; ++i) {
-      f.m[i] = true;
-      f.p[i] = true;
+      f.m[i] = in(i);
+      f.p[i] = in(i);
       for (int j = 0; j < i; ++j) {
+        if (!in(j)) continue;
         if (CgTable<T>::e[j].M == CgTable<T>::e[i].M) f.m[i] = false;
         if (CgTable<T>::e[j].m1 == CgTable<T>::e[i].m1 && CgTable<T>::e[j].m2 == CgTable<T>::e[i].m2)
           f.p[i] = false;
@@ -70,64 +73,68 @@ struct CgFirst {
   static constexpr F v = make();
 };
 
-// o[M] (+)= c a[m1] b[m2] for entry I of block type T
-template <typename Real, int T, int I, int NA, int NB, int NO>
+// o[M - M0] (+)= c a[m1] b[m2] for entry I of block type T (entries outside [M0, M1) skipped)
+template <typename Real, int T, int M0, int M1, int I, int NA, int NB, int NO>
 __device__ __forceinline__ void fwd_term(const Real (&a)[NA], const Real (&b)[NB], Real (&o)[NO])
 {
   constexpr CgEntry e = CgTable<T>::e[I];
-  constexpr Real c = static_cast<Real>(e.c);
-  const Real t = c * (a[e.m1] * b[e.m2]);
-  if constexpr (CgFirst<T>::v.m[I])
-    o[e.M] = t;
-  else
-    o[e.M] += t;
+  if constexpr (e.M >= M0 && e.M < M1) {
+    constexpr Real c = static_cast<Real>(e.c);
+    const Real t = c * (a[e.m1] * b[e.m2]);
+    if constexpr (CgFirst<T, M0, M1>::v.m[I])
+      o[e.M - M0] = t;
+    else
+      o[e.M - M0] += t;
+  }
 }
 
-template <typename Real, int T, int NA, int NB, int NO, int... I>
+template <typename Real, int T, int M0, int M1, int NA, int NB, int NO, int... I>
 __device__ __forceinline__ void fwd_terms(const Real (&a)[NA], const Real (&b)[NB], Real (&o)[NO],
                                           std::integer_sequence<int, I...>)
 {
-  (fwd_term<Real, T, I>(a, b, o), ...);
+  (fwd_term<Real, T, M0, M1, I>(a, b, o), ...);
 }
 
-// W[m1][m2] (+)= c g[M] for entry I
-template <typename Real, int T, int I, int NA, int NB, int NO>
+// W[m1][m2] (+)= c g[M - M0] for entry I
+template <typename Real, int T, int M0, int M1, int I, int NA, int NB, int NO>
 __device__ __forceinline__ void w_term(const Real (&g)[NO], Real (&W)[NA][NB])
 {
   constexpr CgEntry e = CgTable<T>::e[I];
-  constexpr Real c = static_cast<Real>(e.c);
-  if constexpr (CgFirst<T>::v.p[I])
-    W[e.m1][e.m2] = c * g[e.M];
-  else
-    W[e.m1][e.m2] += c * g[e.M];
+  if constexpr (e.M >= M0 && e.M < M1) {
+    constexpr Real c = static_cast<Real>(e.c);
+    if constexpr (CgFirst<T, M0, M1>::v.p[I])
+      W[e.m1][e.m2] = c * g[e.M - M0];
+    else
+      W[e.m1][e.m2] += c * g[e.M - M0];
+  }
 }
 
 // for the first entry of each (m1, m2): ga[m1] += W b[m2], gb[m2] += W a[m1]
-template <typename Real, int T, int I, int NA, int NB>
+template <typename Real, int T, int M0, int M1, int I, int NA, int NB>
 __device__ __forceinline__ void wab_term(const Real (&W)[NA][NB], const Real (&a)[NA],
                                          const Real (&b)[NB], Real (&ga)[NA], Real (&gb)[NB])
 {
-  if constexpr (CgFirst<T>::v.p[I]) {
+  if constexpr (CgFirst<T, M0, M1>::v.p[I]) {
     constexpr CgEntry e = CgTable<T>::e[I];
     ga[e.m1] += W[e.m1][e.m2] * b[e.m2];
     gb[e.m2] += W[e.m1][e.m2] * a[e.m1];
   }
 }
 
-template <typename Real, int T, int NA, int NB, int NO, int... I>
+template <typename Real, int T, int M0, int M1, int NA, int NB, int NO, int... I>
 __device__ __forceinline__ void bwd_terms(const Real (&g)[NO], const Real (&a)[NA], const Real (&b)[NB],
                                           Real (&ga)[NA], Real (&gb)[NB], std::integer_sequence<int, I...>)
 {
   Real W[NA][NB];
-  (w_term<Real, T, I>(g, W), ...);
-  (wab_term<Real, T, I>(W, a, b, ga, gb), ...);
+  (w_term<Real, T, M0, M1, I>(g, W), ...);
+  (wab_term<Real, T, M0, M1, I>(W, a, b, ga, gb), ...);
 }
 
-// one layer-1 tensor: v = (b_A (x) b_B)_L
-template <typename Real, int T>
+// one layer-1 tensor (components [M0, M1)): v = (b_A (x) b_B)_L
+template <typename Real, int T, int M0, int M1>
 __device__ __forceinline__ void fwd_block(Real* X, uint32_t vo, uint32_t ao, uint32_t bo)
 {
-  constexpr int NA = 2 * kCgL1[T] + 1, NB = 2 * kCgL2[T] + 1, NO = 2 * kCgLL[T] + 1;
+  constexpr int NA = 2 * kCgL1[T] + 1, NB = 2 * kCgL2[T] + 1, NO = M1 - M0;
   const Real* xa = X + ao;
   const Real* xb = X + bo;
   Real a[NA], b[NB], o[NO];
@@ -135,22 +142,22 @@ __device__ __forceinline__ void fwd_block(Real* X, uint32_t vo, uint32_t ao, uin
   for (int m = 0; m < NA; ++m) a[m] = xa[m * kCol];
 #pragma unroll
   for (int m = 0; m < NB; ++m) b[m] = xb[m * kCol];
-  fwd_terms<Real, T>(a, b, o, std::make_integer_sequence<int, CgTable<T>::n>{});
+  fwd_terms<Real, T, M0, M1>(a, b, o, std::make_integer_sequence<int, CgTable<T>::n>{});
   Real* xo = X + vo;
 #pragma unroll
   for (int M = 0; M < NO; ++M) xo[M * kCol] = o[M];
 }
 
-// g = d [M = 0] + sum over partners K' v_partner; advances p past the partner list
-template <typename Real, int L>
-__device__ __forceinline__ void adjoint(cu32*& p, uint32_t np, const Real* X, Real d,
-                                        Real (&g)[2 * L + 1])
+// g = d [M = 0 of a degree-0 tensor] + sum over partners K' v_partner (NL components);
+// advances p past the partner list
+template <typename Real, int NL, bool D>
+__device__ __forceinline__ void adjoint(cu32*& p, uint32_t np, const Real* X, Real d, Real (&g)[NL])
 {
-  constexpr int NL = 2 * L + 1, PW = 1 + static_cast<int>(sizeof(Real)) / 4;
-  constexpr int U = L <= 2 ? 4 : 2;  // partners per scalar batch
+  constexpr int PW = 1 + static_cast<int>(sizeof(Real)) / 4;
+  constexpr int U = NL <= 5 ? 4 : 2;  // partners per scalar batch
 #pragma unroll
   for (int M = 0; M < NL; ++M) g[M] = Real(0);
-  if (L == 0) g[0] = d;
+  if (D) g[0] = d;
   uint32_t j = 0;
   for (; j + U <= np; j += U) {
     uint32_t off[U];
@@ -184,31 +191,31 @@ __device__ __forceinline__ void adjoint(cu32*& p, uint32_t np, const Real* X, Re
 }
 
 // 1/2 v . (g + d [M = 0]) into the lane's energy
-template <typename Real, int L>
-__device__ __forceinline__ void energy(const Real* xv, Real d, const Real (&g)[2 * L + 1], Real& e)
+template <typename Real, int NL, bool D>
+__device__ __forceinline__ void energy(const Real* xv, Real d, const Real (&g)[NL], Real& e)
 {
-  constexpr int NL = 2 * L + 1;
   Real v[NL];
 #pragma unroll
   for (int M = 0; M < NL; ++M) v[M] = xv[M * kCol];
-  Real s = L == 0 ? d * v[0] : Real(0);
+  Real s = D ? d * v[0] : Real(0);
 #pragma unroll
   for (int M = 0; M < NL; ++M) s += v[M] * g[M];
   e += Real(0.5) * s;
 }
 
-// quadratic-form record of a layer-1 tensor of block type T
-template <typename Real, int T>
+// quadratic-form record of a layer-1 tensor of block type T (components [M0, M1))
+template <typename Real, int T, int M0, int M1>
 __device__ __forceinline__ void tensor_pair(cu32*& p, Real* X, double* G, Real& e)
 {
-  constexpr int L = kCgLL[T], NA = 2 * kCgL1[T] + 1, NB = 2 * kCgL2[T] + 1;
+  constexpr int NA = 2 * kCgL1[T] + 1, NB = 2 * kCgL2[T] + 1, NL = M1 - M0;
+  constexpr bool D = kCgLL[T] == 0;
   const uint32_t w0 = p[0], w1 = p[1];
   const Real d = read_real<Real>(p + 2);
   p += 2 + static_cast<int>(sizeof(Real)) / 4;
   const uint32_t vo = w0 & 0xFFFFu, np = w0 >> 16, ao = w1 & 0xFFFFu, bo = w1 >> 16;
-  Real g[2 * L + 1];
-  adjoint<Real, L>(p, np, X, d, g);
-  energy<Real, L>(X + vo, d, g, e);
+  Real g[NL];
+  adjoint<Real, NL, D>(p, np, X, d, g);
+  energy<Real, NL, D>(X + vo, d, g, e);
   const Real* xa = X + ao;
   const Real* xb = X + bo;
   Real a[NA], b[NB], ga[NA], gb[NB];
@@ -222,7 +229,7 @@ __device__ __forceinline__ void tensor_pair(cu32*& p, Real* X, double* G, Real& 
     b[m] = xb[m * kCol];
     gb[m] = Real(0);
   }
-  bwd_terms<Real, T>(g, a, b, ga, gb, std::make_integer_sequence<int, CgTable<T>::n>{});
+  bwd_terms<Real, T, M0, M1>(g, a, b, ga, gb, std::make_integer_sequence<int, CgTable<T>::n>{});
   double* Ga = G + ao;
   double* Gb = G + bo;
 #pragma unroll
@@ -231,20 +238,75 @@ __device__ __forceinline__ void tensor_pair(cu32*& p, Real* X, double* G, Real& 
   for (int m = 0; m < NB; ++m) lds_add(Gb + m * kCol, static_cast<double>(gb[m]));
 }
 
-// quadratic-form record of a basic tensor of degree L: its adjoint goes to G directly
-template <typename Real, int L>
+// quadratic-form record of a basic tensor of degree L (components [M0, M1), the record's
+// offset is that of component M0): its adjoint goes to G directly
+template <typename Real, int L, int M0, int M1>
 __device__ __forceinline__ void tensor_basic(cu32*& p, Real* X, double* G, Real& e)
 {
+  constexpr int NL = M1 - M0;
   const uint32_t w0 = p[0];
   const Real d = read_real<Real>(p + 2);
   p += 2 + static_cast<int>(sizeof(Real)) / 4;
   const uint32_t vo = w0 & 0xFFFFu, np = w0 >> 16;
-  Real g[2 * L + 1];
-  adjoint<Real, L>(p, np, X, d, g);
-  energy<Real, L>(X + vo, d, g, e);
+  Real g[NL];
+  adjoint<Real, NL, L == 0>(p, np, X, d, g);
+  energy<Real, NL, L == 0>(X + vo, d, g, e);
   double* Gv = G + vo;
 #pragma unroll
-  for (int M = 0; M <= 2 * L; ++M) lds_add(Gv + M * kCol, static_cast<double>(g[M]));
+  for (int M = 0; M < NL; ++M) lds_add(Gv + M * kCol, static_cast<double>(g[M]));
+}
+
+// a run of `cnt` FWD records of block type T, part `part` (0 whole, 1 lower half, 2 upper;
+// halves are compiled for FP32 only: an FP64 tile does not reach two workgroups per CU)
+template <typename Real, int T>
+__device__ __forceinline__ void fwd_run(uint32_t part, int cnt, cu32*& p, Real* X)
+{
+  constexpr int L = kCgLL[T];
+  if constexpr (L >= 2 && sizeof(Real) == 4) {
+    if (part == 1) {
+      for (int r = 0; r < cnt; ++r, p += 2) fwd_block<Real, T, 0, L + 1>(X, p[0] & 0xFFFFu, p[0] >> 16, p[1]);
+      return;
+    }
+    if (part == 2) {
+      for (int r = 0; r < cnt; ++r, p += 2)
+        fwd_block<Real, T, L + 1, 2 * L + 1>(X, p[0] & 0xFFFFu, p[0] >> 16, p[1]);
+      return;
+    }
+  }
+  for (int r = 0; r < cnt; ++r, p += 2) fwd_block<Real, T, 0, 2 * L + 1>(X, p[0] & 0xFFFFu, p[0] >> 16, p[1]);
+}
+
+template <typename Real, int T>
+__device__ __forceinline__ void pair_run(uint32_t part, int cnt, cu32*& p, Real* X, double* G, Real& e)
+{
+  constexpr int L = kCgLL[T];
+  if constexpr (L >= 2 && sizeof(Real) == 4) {
+    if (part == 1) {
+      for (int r = 0; r < cnt; ++r) tensor_pair<Real, T, 0, L + 1>(p, X, G, e);
+      return;
+    }
+    if (part == 2) {
+      for (int r = 0; r < cnt; ++r) tensor_pair<Real, T, L + 1, 2 * L + 1>(p, X, G, e);
+      return;
+    }
+  }
+  for (int r = 0; r < cnt; ++r) tensor_pair<Real, T, 0, 2 * L + 1>(p, X, G, e);
+}
+
+template <typename Real, int L>
+__device__ __forceinline__ void basic_run(uint32_t part, int cnt, cu32*& p, Real* X, double* G, Real& e)
+{
+  if constexpr (L >= 2 && sizeof(Real) == 4) {
+    if (part == 1) {
+      for (int r = 0; r < cnt; ++r) tensor_basic<Real, L, 0, L + 1>(p, X, G, e);
+      return;
+    }
+    if (part == 2) {
+      for (int r = 0; r < cnt; ++r) tensor_basic<Real, L, L + 1, 2 * L + 1>(p, X, G, e);
+      return;
+    }
+  }
+  for (int r = 0; r < cnt; ++r) tensor_basic<Real, L, 0, 2 * L + 1>(p, X, G, e);
 }
 
 template <typename Real>
@@ -253,11 +315,12 @@ __device__ __forceinline__ void run_fwd(cu32* p, cu32* end, Real* X)
   while (p < end) {
     const uint32_t h = p[0];
     const int cnt = static_cast<int>(h >> 8);
+    const uint32_t kind = h & 255u;
     ++p;
-    switch (h & 255u) {
-#define SUS2SH_FWD_CASE(T, l1, l2, L)                                        \
-  case T:                                                                    \
-    for (int r = 0; r < cnt; ++r, p += 2) fwd_block<Real, T>(X, p[0] & 0xFFFFu, p[0] >> 16, p[1]); \
+    switch (kind & 63u) {
+#define SUS2SH_FWD_CASE(T, l1, l2, L) \
+  case T:                             \
+    fwd_run<Real, T>(kind >> 6, cnt, p, X); \
     break;
       SUS2SH_CG_TYPES(SUS2SH_FWD_CASE)
 #undef SUS2SH_FWD_CASE
@@ -273,26 +336,25 @@ __device__ __forceinline__ void run_tensors(cu32* p, cu32* end, Real* X, double*
   while (p < end) {
     const uint32_t h = p[0];
     const int cnt = static_cast<int>(h >> 8);
+    const uint32_t kind = h & 255u;
     ++p;
-    switch (h & 255u) {
-#define SUS2SH_TEN_CASE(T, l1, l2, L)                                       \
-  case T:                                                                   \
-    for (int r = 0; r < cnt; ++r) tensor_pair<Real, T>(p, X, G, e);      \
+    if (kind >= static_cast<uint32_t>(kBlockBasicKind)) {
+      const uint32_t k = kind - kBlockBasicKind, part = k >> 2;
+      switch (k & 3u) {
+        case 0: basic_run<Real, 0>(part, cnt, p, X, G, e); break;
+        case 1: basic_run<Real, 1>(part, cnt, p, X, G, e); break;
+        case 2: basic_run<Real, 2>(part, cnt, p, X, G, e); break;
+        default: basic_run<Real, 3>(part, cnt, p, X, G, e); break;
+      }
+      continue;
+    }
+    switch (kind & 63u) {
+#define SUS2SH_TEN_CASE(T, l1, l2, L) \
+  case T:                             \
+    pair_run<Real, T>(kind >> 6, cnt, p, X, G, e); \
     break;
       SUS2SH_CG_TYPES(SUS2SH_TEN_CASE)
 #undef SUS2SH_TEN_CASE
-      case kCgTypes + 0:
-        for (int r = 0; r < cnt; ++r) tensor_basic<Real, 0>(p, X, G, e);
-        break;
-      case kCgTypes + 1:
-        for (int r = 0; r < cnt; ++r) tensor_basic<Real, 1>(p, X, G, e);
-        break;
-      case kCgTypes + 2:
-        for (int r = 0; r < cnt; ++r) tensor_basic<Real, 2>(p, X, G, e);
-        break;
-      case kCgTypes + 3:
-        for (int r = 0; r < cnt; ++r) tensor_basic<Real, 3>(p, X, G, e);
-        break;
       default:
         break;
     }

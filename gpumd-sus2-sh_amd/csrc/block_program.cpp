@@ -102,7 +102,8 @@ std::vector<std::vector<int>> balance(const std::vector<std::pair<double, int>>&
 
 }  // namespace
 
-BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, int max_slots)
+BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, int max_slots,
+                                   int target_slots)
 {
   BlockProgram p;
   p.waves = waves;
@@ -258,7 +259,7 @@ BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, in
     }
   }
 
-  // live tensors, degree classes and batches
+  // live tensors and degree classes
   std::vector<int> live;
   for (int i = 0; i < static_cast<int>(ten.size()); ++i) {
     for (auto it = ten[i].partners.begin(); it != ten[i].partners.end();)
@@ -266,84 +267,140 @@ BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, in
     if (ten[i].d != 0.0 || !ten[i].partners.empty()) live.push_back(i);
   }
   if (live.empty()) return fail("the product graph has no live tensor");
-  std::vector<int> class_size(2 * kCgLMax + 1, 0);
-  std::vector<char> class_used(2 * kCgLMax + 1, 0);
+  for (const Tensor& t : ten)
+    if (t.partners.size() >= (1u << 16)) return fail("a tensor has too many partners");
+  constexpr int NC = 2 * kCgLMax + 1;
+  std::vector<int> class_size(NC, 0);
+  std::vector<char> class_used(NC, 0);
   for (int i : live) {
     class_used[ten[i].L] = 1;
     if (!ten[i].basic) class_size[ten[i].L] += 2 * ten[i].L + 1;
   }
+
+  // Units: a degree class, or one of its two component halves [0, L + 1), [L + 1, 2L + 1)
+  // (the quadratic form couples equal components only, so a half is evaluated on its own).
+  // Whole classes when they fit target_slots; else the classes of degree >= 2 that exceed it
+  // are halved, when that makes every unit fit; else whole classes up to max_slots.
+  struct Unit {
+    int L, part, M0, M1, size, batch;
+  };
+  auto make_units = [&](int cap_limit) {
+    std::vector<Unit> u;
+    for (int L = 0; L < NC; ++L) {
+      if (!class_used[L]) continue;
+      if (cap_limit > 0 && class_size[L] > cap_limit && L >= 2) {
+        const int w0 = L + 1, w1 = L;
+        u.push_back(Unit{L, 1, 0, w0, class_size[L] / (2 * L + 1) * w0, -1});
+        u.push_back(Unit{L, 2, w0, 2 * L + 1, class_size[L] / (2 * L + 1) * w1, -1});
+      } else {
+        u.push_back(Unit{L, 0, 0, 2 * L + 1, class_size[L], -1});
+      }
+    }
+    return u;
+  };
   int cap = 1;
-  for (int s : class_size) cap = std::max(cap, s);
-  std::vector<int> classes;
-  for (int L = 0; L <= 2 * kCgLMax; ++L)
-    if (class_used[L]) classes.push_back(L);
-  std::stable_sort(classes.begin(), classes.end(),
-                   [&](int a, int b) { return class_size[a] > class_size[b]; });
-  std::vector<int> batch_of(2 * kCgLMax + 1, -1), batch_fill;
-  for (int L : classes) {
+  for (int sz : class_size) cap = std::max(cap, sz);
+  std::vector<Unit> units = make_units(0);
+  if (target_slots > 0 && B + cap > target_slots) {
+    std::vector<Unit> halves = make_units(target_slots - B);
+    int hcap = 1;
+    for (const Unit& un : halves) hcap = std::max(hcap, un.size);
+    if (B + hcap <= target_slots) {
+      units = halves;
+      cap = hcap;
+    }
+  }
+  // first-fit decreasing of the units into batches of `cap` values
+  std::vector<int> order(units.size());
+  for (size_t k = 0; k < units.size(); ++k) order[k] = static_cast<int>(k);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return units[a].size > units[b].size; });
+  std::vector<int> batch_fill;
+  for (int k : order) {
     int b = -1;
-    for (int k = 0; k < static_cast<int>(batch_fill.size()) && b < 0; ++k)
-      if (batch_fill[k] + class_size[L] <= cap) b = k;
+    for (int j = 0; j < static_cast<int>(batch_fill.size()) && b < 0; ++j)
+      if (batch_fill[j] + units[k].size <= cap) b = j;
     if (b < 0) {
       b = static_cast<int>(batch_fill.size());
       batch_fill.push_back(0);
     }
-    batch_of[L] = b;
-    batch_fill[b] += class_size[L];
+    units[k].batch = b;
+    batch_fill[b] += units[k].size;
   }
   p.batches = static_cast<int>(batch_fill.size());
   p.cap = cap;
   p.v_base = B;
   p.slots = B + cap;
+  p.units = static_cast<int>(units.size());
   if (p.slots > max_slots || p.slots > 1024) return fail("block program exceeds the per-lane slot budget");
+  // value slot of (tensor, unit): layer-1 tensors get the unit's width in its batch; a basic
+  // tensor's components M0.. sit at its basic slot + M0
   std::vector<int> fill(p.batches, 0);
-  for (int i : live) {
-    Tensor& t = ten[i];
-    t.batch = batch_of[t.L];
-    if (!t.basic) {
-      t.slot = p.v_base + fill[t.batch];
-      fill[t.batch] += 2 * t.L + 1;
+  std::map<std::pair<int, int>, int> slot_of;
+  for (size_t k = 0; k < units.size(); ++k) {
+    const Unit& un = units[k];
+    for (int i : live) {
+      const Tensor& t = ten[i];
+      if (t.L != un.L) continue;
+      if (t.basic) {
+        slot_of[{i, static_cast<int>(k)}] = t.slot + un.M0;
+      } else {
+        slot_of[{i, static_cast<int>(k)}] = p.v_base + fill[un.batch];
+        fill[un.batch] += un.M1 - un.M0;
+      }
     }
   }
 
-  // records: per batch, the FWD blocks and the quadratic-form tensors, balanced over waves
+  // records: per batch, the FWD blocks and the quadratic-form tensors of its units, balanced
+  // over the waves
   const int col = 64;
   p.seg.assign(static_cast<size_t>(4) * p.batches * waves, 0);
   for (int b = 0; b < p.batches; ++b) {
+    // items: (cost, (tensor, unit))
     std::vector<std::pair<double, int>> fwd_items, ten_items;
-    for (int i : live) {
-      const Tensor& t = ten[i];
-      if (t.batch != b) continue;
-      const int nl = 2 * t.L + 1;
-      if (!t.basic) {
-        const int e = kCgEntries[t.type];
-        fwd_items.emplace_back(2.0 * e + 2 * ten[t.ta].L + 2 * ten[t.tb].L + 2 + nl, i);
-        p.fwd_terms += e;
-        p.bwd_terms += 2 * e;
-        ++p.fwd_blocks;
+    std::vector<std::pair<int, int>> item;
+    for (size_t k = 0; k < units.size(); ++k) {
+      const Unit& un = units[k];
+      if (un.batch != b) continue;
+      const double frac = static_cast<double>(un.M1 - un.M0) / (2 * un.L + 1);
+      for (int i : live) {
+        const Tensor& t = ten[i];
+        if (t.L != un.L) continue;
+        const int nl = un.M1 - un.M0;
+        const int id = static_cast<int>(item.size());
+        item.emplace_back(i, static_cast<int>(k));
+        if (!t.basic) {
+          const double e = kCgEntries[t.type] * frac;
+          fwd_items.emplace_back(2.0 * e + 2 * ten[t.ta].L + 2 * ten[t.tb].L + 2 + nl, id);
+          p.fwd_terms += static_cast<long long>(std::lround(e));
+          p.bwd_terms += static_cast<long long>(std::lround(2 * e));
+          ++p.fwd_blocks;
+        }
+        const double bwd = t.basic ? nl : 3.0 * kCgEntries[t.type] * frac + 2 * ten[t.ta].L + 2 * ten[t.tb].L + 2;
+        ten_items.emplace_back(2.0 * nl * static_cast<double>(t.partners.size()) + 2.0 * nl + bwd, id);
+        p.adj_terms += static_cast<long long>(nl) * static_cast<long long>(t.partners.size());
+        ++p.tensors;
       }
-      const double bwd = t.basic ? nl : 3.0 * kCgEntries[t.type] + 2 * ten[t.ta].L + 2 * ten[t.tb].L + 2;
-      ten_items.emplace_back(2.0 * nl * static_cast<double>(t.partners.size()) + 2.0 * nl + bwd, i);
-      p.adj_terms += static_cast<long long>(nl) * static_cast<long long>(t.partners.size());
-      ++p.tensors;
     }
     const auto fwd_w = balance(fwd_items, waves), ten_w = balance(ten_items, waves);
+    auto kind_of = [&](int id) {
+      const Tensor& t = ten[item[id].first];
+      const int part = units[item[id].second].part;
+      return t.basic ? kBlockBasicKind + 4 * part + t.L : t.type | part << 6;
+    };
     for (int wv = 0; wv < waves; ++wv) {
       int* sg = &p.seg[(static_cast<size_t>(b) * waves + wv) * 4];
-      auto kind_fwd = [&](int i) { return ten[i].type; };
-      auto kind_ten = [&](int i) { return ten[i].basic ? kCgTypes + ten[i].L : ten[i].type; };
       // FWD runs
       std::vector<int> list = fwd_w[wv];
-      std::stable_sort(list.begin(), list.end(), [&](int x, int y) { return kind_fwd(x) < kind_fwd(y); });
+      std::stable_sort(list.begin(), list.end(), [&](int x, int y) { return kind_of(x) < kind_of(y); });
       sg[0] = static_cast<int>(p.words.size());
       for (size_t k = 0; k < list.size();) {
         size_t e = k;
-        while (e < list.size() && kind_fwd(list[e]) == kind_fwd(list[k])) ++e;
-        p.words.push_back(static_cast<uint32_t>(kind_fwd(list[k])) | static_cast<uint32_t>(e - k) << 8);
+        while (e < list.size() && kind_of(list[e]) == kind_of(list[k])) ++e;
+        p.words.push_back(static_cast<uint32_t>(kind_of(list[k])) | static_cast<uint32_t>(e - k) << 8);
         for (size_t r = k; r < e; ++r) {
-          const Tensor& t = ten[list[r]];
-          p.words.push_back(static_cast<uint32_t>(t.slot * col) |
-                            static_cast<uint32_t>(ten[t.ta].slot * col) << 16);
+          const Tensor& t = ten[item[list[r]].first];
+          const int vs = slot_of[item[list[r]]];
+          p.words.push_back(static_cast<uint32_t>(vs * col) | static_cast<uint32_t>(ten[t.ta].slot * col) << 16);
           p.words.push_back(static_cast<uint32_t>(ten[t.tb].slot * col));
         }
         k = e;
@@ -351,21 +408,22 @@ BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, in
       sg[1] = static_cast<int>(p.words.size());
       // TENSOR runs
       list = ten_w[wv];
-      std::stable_sort(list.begin(), list.end(), [&](int x, int y) { return kind_ten(x) < kind_ten(y); });
+      std::stable_sort(list.begin(), list.end(), [&](int x, int y) { return kind_of(x) < kind_of(y); });
       sg[2] = static_cast<int>(p.words.size());
       for (size_t k = 0; k < list.size();) {
         size_t e = k;
-        while (e < list.size() && kind_ten(list[e]) == kind_ten(list[k])) ++e;
-        p.words.push_back(static_cast<uint32_t>(kind_ten(list[k])) | static_cast<uint32_t>(e - k) << 8);
+        while (e < list.size() && kind_of(list[e]) == kind_of(list[k])) ++e;
+        p.words.push_back(static_cast<uint32_t>(kind_of(list[k])) | static_cast<uint32_t>(e - k) << 8);
         for (size_t r = k; r < e; ++r) {
-          const Tensor& t = ten[list[r]];
-          p.words.push_back(static_cast<uint32_t>(t.slot * col) |
-                            static_cast<uint32_t>(t.partners.size()) << 16);
+          const int ti = item[list[r]].first, uk = item[list[r]].second;
+          const Tensor& t = ten[ti];
+          const int vs = slot_of[item[list[r]]];
+          p.words.push_back(static_cast<uint32_t>(vs * col) | static_cast<uint32_t>(t.partners.size()) << 16);
           const int ao = t.basic ? t.slot : ten[t.ta].slot, bo = t.basic ? t.slot : ten[t.tb].slot;
           p.words.push_back(static_cast<uint32_t>(ao * col) | static_cast<uint32_t>(bo * col) << 16);
           put_real(p.words, t.d, real_bytes);
           for (const auto& pr : t.partners) {
-            p.words.push_back(static_cast<uint32_t>(ten[pr.first].slot * col));
+            p.words.push_back(static_cast<uint32_t>(slot_of[{pr.first, uk}] * col));
             put_real(p.words, pr.second, real_bytes);
           }
         }
@@ -374,8 +432,7 @@ BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, in
       sg[3] = static_cast<int>(p.words.size());
     }
   }
-  for (const Tensor& t : ten)
-    if (t.partners.size() >= (1u << 16)) return fail("a tensor has too many partners");
+
 
   // self-check against the flat graph on two seeded basic vectors (FP32 coefficients: 1e-5)
   uint64_t seed = 0xb10cb10cULL;
@@ -403,18 +460,28 @@ double run_block_program_host(const BlockProgram& p, const double* basics, doubl
   for (int i = 0; i < B; ++i) X[i] = basics[i];
   double e = 0.0;
   const uint32_t* W = p.words.data();
+  // component range of a part of degree L
+  auto range = [](int L, int part, int& M0, int& M1) {
+    M0 = part == 2 ? L + 1 : 0;
+    M1 = part == 1 ? L + 1 : 2 * L + 1;
+  };
   for (int b = 0; b < p.batches; ++b) {
     for (int wv = 0; wv < p.waves; ++wv) {
       const int* sg = &p.seg[(static_cast<size_t>(b) * p.waves + wv) * 4];
       for (int k = sg[0]; k < sg[1];) {
-        const int t = static_cast<int>(W[k] & 255u), cnt = static_cast<int>(W[k] >> 8);
+        const int kind = static_cast<int>(W[k] & 255u), cnt = static_cast<int>(W[k] >> 8);
         ++k;
+        const int t = kind & 63;
+        int M0, M1;
+        range(kCgLL[t], kind >> 6, M0, M1);
         const CgView& v = cg_view(t);
         for (int r = 0; r < cnt; ++r, k += 2) {
           const int vo = static_cast<int>(W[k] & 0xFFFFu) / col, ao = static_cast<int>(W[k] >> 16) / col;
           const int bo = static_cast<int>(W[k + 1]) / col;
-          for (int M = 0; M <= 2 * kCgLL[t]; ++M) X[vo + M] = 0.0;
-          for (int i = 0; i < v.n; ++i) X[vo + v.e[i].M] += v.e[i].c * X[ao + v.e[i].m1] * X[bo + v.e[i].m2];
+          for (int M = M0; M < M1; ++M) X[vo + M - M0] = 0.0;
+          for (int i = 0; i < v.n; ++i)
+            if (v.e[i].M >= M0 && v.e[i].M < M1)
+              X[vo + v.e[i].M - M0] += v.e[i].c * X[ao + v.e[i].m1] * X[bo + v.e[i].m2];
         }
       }
     }
@@ -423,8 +490,11 @@ double run_block_program_host(const BlockProgram& p, const double* basics, doubl
       for (int k = sg[2]; k < sg[3];) {
         const int kind = static_cast<int>(W[k] & 255u), cnt = static_cast<int>(W[k] >> 8);
         ++k;
-        const bool basic = kind >= kCgTypes;
-        const int L = basic ? kind - kCgTypes : kCgLL[kind];
+        const bool basic = kind >= kBlockBasicKind;
+        const int L = basic ? (kind - kBlockBasicKind) & 3 : kCgLL[kind & 63];
+        int M0, M1;
+        range(L, basic ? (kind - kBlockBasicKind) >> 2 : kind >> 6, M0, M1);
+        const int nl = M1 - M0;
         for (int r = 0; r < cnt; ++r) {
           const int vo = static_cast<int>(W[k] & 0xFFFFu) / col, np = static_cast<int>(W[k] >> 16);
           const int ao = static_cast<int>(W[k + 1] & 0xFFFFu) / col, bo = static_cast<int>(W[k + 1] >> 16) / col;
@@ -435,17 +505,18 @@ double run_block_program_host(const BlockProgram& p, const double* basics, doubl
           for (int j = 0; j < np; ++j, k += 1 + rw) {
             const int po = static_cast<int>(W[k]) / col;
             const double c = get_real(W + k + 1, p.real_bytes);
-            for (int M = 0; M <= 2 * L; ++M) g[M] += c * X[po + M];
+            for (int M = 0; M < nl; ++M) g[M] += c * X[po + M];
           }
           double en = L == 0 ? d * X[vo] : 0.0;
-          for (int M = 0; M <= 2 * L; ++M) en += X[vo + M] * g[M];
+          for (int M = 0; M < nl; ++M) en += X[vo + M] * g[M];
           e += 0.5 * en;
           if (basic) {
-            for (int M = 0; M <= 2 * L; ++M) G[vo + M] += g[M];
+            for (int M = 0; M < nl; ++M) G[vo + M] += g[M];
           } else {
-            const CgView& v = cg_view(kind);
+            const CgView& v = cg_view(kind & 63);
             for (int i = 0; i < v.n; ++i) {
-              const double cg = v.e[i].c * g[v.e[i].M];
+              if (v.e[i].M < M0 || v.e[i].M >= M1) continue;
+              const double cg = v.e[i].c * g[v.e[i].M - M0];
               G[ao + v.e[i].m1] += cg * X[bo + v.e[i].m2];
               G[bo + v.e[i].m2] += cg * X[ao + v.e[i].m1];
             }

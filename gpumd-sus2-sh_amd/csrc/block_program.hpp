@@ -13,8 +13,10 @@
 // W = sum_M g_P[M] C(. . | L M) into the basic adjoints: G_A += W b_B, G_B += W^T b_A
 // (G_T += g_T for a basic tensor).
 //
-// The live layer-1 tensors are grouped by degree; whole degree classes are packed into
-// batches that fit the per-lane value region.  A batch is two epochs: FWD (the batch's
+// The live layer-1 tensors are grouped by degree; whole degree classes -- or, when that makes
+// the tile small enough for two workgroups per CU, the two component halves [0, L + 1) and
+// [L + 1, 2L + 1) of the large classes of degree >= 2 (the quadratic form couples equal
+// components only) -- are packed into batches that fit the per-lane value region.  A batch is two epochs: FWD (the batch's
 // layer-1 tensors, split between the waves of the tile) and TENSOR (per quadratic-form tensor
 // of the batch: g_T, its energy share and its reverse pass, the basic adjoints summed with
 // LDS atomics).  Each wave's records of an epoch are sorted into runs of one block type, so
@@ -27,8 +29,10 @@
 //   FWD record (2 words):    v_off | a_off << 16,  b_off
 //   TENSOR record:           v_off | npart << 16,  a_off | b_off << 16,  d (Real),
 //                            npart x (partner v_off, K' (Real))
-// Offsets are LDS words (slot * 64); a Real takes 1 (FP32) or 2 (FP64) words.  Kinds
-// [0, kCgTypes) are the CG block types of cg_table.hpp; kCgTypes + L a basic tensor of degree L.
+// Offsets are LDS words (slot * 64; for a half, the slot of its first component); a Real takes
+// 1 (FP32) or 2 (FP64) words.  Kind = t | part << 6 for CG block type t of cg_table.hpp, and
+// kBlockBasicKind + 4 part + L for a basic tensor of degree L; part 0 is the whole tensor,
+// 1 and 2 its lower and upper component halves.
 #pragma once
 
 #include <cstdint>
@@ -39,6 +43,8 @@
 
 namespace sus2sh {
 
+constexpr int kBlockBasicKind = 192;
+
 struct BlockProgram {
   bool ok = false;
   std::string reason;  // why the model cannot use the block program (when !ok)
@@ -47,6 +53,7 @@ struct BlockProgram {
   int slots = 0;       // per-lane Real slots (basics and tensor values)
   int v_base = 0, cap = 0;
   int batches = 0;
+  int units = 0;       // degree classes or class halves
   int tensors = 0;     // quadratic-form tensors
   int fwd_blocks = 0;  // layer-1 tensors computed
   long long fwd_terms = 0, adj_terms = 0, bwd_terms = 0;
@@ -57,10 +64,12 @@ struct BlockProgram {
 };
 
 // Build the block program for `waves` waves per tile with coefficients of real_bytes (4 or
-// 8) and at most max_slots per-lane slots.  Never throws for an unsuitable model: returns
-// ok = false with the reason instead.  The program is checked against the flat graph on
-// seeded basic vectors before it is returned.
-BlockProgram compile_block_program(const Model& model, int waves, int real_bytes, int max_slots);
+// 8) and at most max_slots per-lane Real slots, halving large classes when that brings it to
+// target_slots (<= 0: never).  Never throws for an unsuitable model: returns ok = false with
+// the reason instead.  The program is checked against the flat graph on seeded basic vectors
+// before it is returned.
+BlockProgram compile_block_program(const Model& model, int waves, int real_bytes, int max_slots,
+                                   int target_slots = 0);
 
 // Host execution of the device words for one atom, in double (test and self-check support):
 // returns E' and writes dE'/db into grad_b (B values).

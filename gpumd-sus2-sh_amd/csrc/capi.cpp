@@ -414,8 +414,9 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot,
   });
 }
 
-int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_bytes, int n,
-                       const double* basics, double* e_prime, double* grad, int* info)
+int sus2sh_block_check(const char* mtp_path, int max_slots, int target_slots, int waves,
+                       int real_bytes, int n, const double* basics, double* e_prime,
+                       double* grad, int* info)
 {
   if (!mtp_path || n < 0 || (n > 0 && (!basics || !e_prime || !grad)) ||
       (waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16) ||
@@ -423,7 +424,8 @@ int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_
     return SUS2SH_ERR_ARGUMENT;
   return guarded(&g_create_error, [&] {
     const sus2sh::Model m = sus2sh::load_model(mtp_path);
-    const sus2sh::BlockProgram p = sus2sh::compile_block_program(m, waves, real_bytes, max_slots);
+    const sus2sh::BlockProgram p =
+      sus2sh::compile_block_program(m, waves, real_bytes, max_slots, target_slots);
     if (info) {
       info[0] = p.ok ? 1 : 0;
       info[1] = p.slots;
@@ -433,6 +435,7 @@ int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_
       info[5] = static_cast<int>(p.fwd_terms);
       info[6] = static_cast<int>(p.adj_terms);
       info[7] = static_cast<int>(p.bwd_terms);
+      info[8] = p.units;
     }
     if (!p.ok) return;
     const int B = m.basic_count;

@@ -333,8 +333,15 @@ const std::vector<int>& Engine::block_waves_list()
 BlockProgram Engine::build_block(int waves) const
 {
   const size_t rsz = opt_.use_float ? sizeof(float) : sizeof(double);
-  const int max_slots = static_cast<int>(max_lds_ / (64 * rsz));
-  BlockProgram p = compile_block_program(model_, waves, static_cast<int>(rsz), max_slots);
+  // Real slots left beside the basic adjoints (B x 64 doubles) in a workgroup's LDS, and in
+  // half of it (two workgroups per CU: large degree classes are then halved if that fits)
+  const size_t g_bytes = static_cast<size_t>(model_.basic_count) * 64 * sizeof(double);
+  const auto slots_in = [&](size_t lds) {
+    return lds > g_bytes ? static_cast<int>((lds - g_bytes) / (64 * rsz)) : 0;
+  };
+  // (halves are compiled for the FP32 kernel only)
+  BlockProgram p = compile_block_program(model_, waves, static_cast<int>(rsz), slots_in(max_lds_),
+                                         opt_.use_float ? slots_in(max_lds_ / 2) : 0);
   if (p.ok) {
     BlockTable bt{};
     bt.basics = model_.basic_count;

@@ -66,16 +66,17 @@ def lane_check(path, basics=None, max_slots=160, waves=1, nslot=0, real_bytes=0,
     return d, e, g
 
 
-def block_check(path, basics=None, max_slots=1024, waves=8, real_bytes=4):
+def block_check(path, basics=None, max_slots=1024, waves=8, real_bytes=4, target_slots=0):
     """Host-only run of the block program (the block graph kernel's device words, graph
-    variant 4) for basic vectors basics [n, B].  Returns (info dict, e_prime [n], grad [n, B]);
-    e_prime / grad are None when the model has no block program."""
+    variant 4) for basic vectors basics [n, B]; target_slots > 0 halves the large degree
+    classes when that brings the program to that many per-lane slots.  Returns (info dict,
+    e_prime [n], grad [n, B]); e_prime / grad are None when the model has no block program."""
     import ctypes
     import numpy as np
     from ._capi import check
     P = ctypes.POINTER(ctypes.c_double)
-    info = (ctypes.c_int * 8)()
-    args = (str(path).encode(), int(max_slots), int(waves), int(real_bytes))
+    info = (ctypes.c_int * 9)()
+    args = (str(path).encode(), int(max_slots), int(target_slots), int(waves), int(real_bytes))
     if basics is None:
         check(lib().sus2sh_block_check(*args, 0, None, None, None, info))
         e = g = None
@@ -86,7 +87,7 @@ def block_check(path, basics=None, max_slots=1024, waves=8, real_bytes=4):
         check(lib().sus2sh_block_check(*args, b.shape[0], b.ctypes.data_as(P),
                                        e.ctypes.data_as(P), g.ctypes.data_as(P), info))
     keys = ("ok", "slots", "batches", "tensors", "fwd_blocks", "fwd_terms", "adj_terms",
-            "bwd_terms")
+            "bwd_terms", "units")
     d = dict(zip(keys, list(info)))
     if not d["ok"]:
         e = g = None

@@ -187,7 +187,7 @@ def lib():
     L.sus2sh_lane_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
                                     P(c_double), P(c_double), P(c_double), P(c_int)]
     L.sus2sh_lane_check.restype = c_int
-    L.sus2sh_block_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, P(c_double),
+    L.sus2sh_block_check.argtypes = [c_char_p, c_int, c_int, c_int, c_int, c_int, P(c_double),
                                      P(c_double), P(c_double), P(c_int)]
     L.sus2sh_block_check.restype = c_int
     L.sus2sh_zbl_tables.argtypes = [c_char_p, c_int, P(c_char_p), P(c_int), P(c_double),

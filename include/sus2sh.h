@@ -295,13 +295,16 @@ int sus2sh_lane_check(const char* mtp_path, int max_slots, int waves, int nslot,
 
 /* Host check of the block graph kernel's program (sus2_sh_graph_variant=4; no reference
  * counterpart): compile it for the model with `waves` (1, 2, 4, 8 or 16) waves per 64-atom
- * tile, coefficients of real_bytes (4 or 8) and at most max_slots per-lane slots, and run its
- * device words on the host for n basic vectors basics[n][B] -> e_prime[n] and grad[n][B] as
- * sus2sh_lane_check does.  info[0..8) = ok, slots, batches, quadratic-form tensors, layer-1
- * blocks, forward terms, adjoint terms, reverse terms (per atom).  When the model has no
- * block program (info[0] = 0) nothing is evaluated.  Returns a status code. */
-int sus2sh_block_check(const char* mtp_path, int max_slots, int waves, int real_bytes, int n,
-                       const double* basics, double* e_prime, double* grad, int* info);
+ * tile, coefficients of real_bytes (4 or 8) and at most max_slots per-lane slots, halving the
+ * large degree classes when that brings it to target_slots (<= 0: never), and run its device
+ * words on the host for n basic vectors basics[n][B] -> e_prime[n] and grad[n][B] as
+ * sus2sh_lane_check does.  info[0..9) = ok, slots, batches, quadratic-form tensor records,
+ * layer-1 blocks, forward terms, adjoint terms, reverse terms (per atom), units (classes or
+ * class halves).  When the model has no block program (info[0] = 0) nothing is evaluated.
+ * Returns a status code. */
+int sus2sh_block_check(const char* mtp_path, int max_slots, int target_slots, int waves,
+                       int real_bytes, int n, const double* basics, double* e_prime,
+                       double* grad, int* info);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,13 @@ def _check(path, n=4, seed=0):
             assert info["ok"] == 1
             assert np.abs(e - e0).max() <= tol * scale
             assert np.abs(g - g0).max() <= tol * scale
+    # a slot target one below the whole-class program: the classes of degree >= 2 above it
+    # are halved by components when that reaches the target (else the program is unchanged)
+    hinfo, e, g = sus2sh.block_check(path, basics, target_slots=info["slots"] - 1, real_bytes=8)
+    print(f"[block] halved: {hinfo}", flush=True)
+    assert hinfo["ok"] == 1 and hinfo["units"] >= info["units"]
+    assert (hinfo["slots"] < info["slots"]) == (hinfo["units"] > info["units"])
+    assert np.abs(e - e0).max() <= 1e-11 * scale and np.abs(g - g0).max() <= 1e-11 * scale
     return info
 
 
@@ -63,12 +70,17 @@ def test_standard_graphs(make_model, L, K, body, blm):
 
 def test_c3_graph_counts(make_model):
     """The l3k3 body-5 graph of the benchmark: 136 layer-1 CG blocks (3,332 forward terms,
-    twice that in reverse), 145 quadratic-form tensors in 4 degree batches, 258 Real slots."""
+    twice that in reverse), 145 quadratic-form tensors in 4 degree batches, 258 Real slots;
+    halved to 210 slots for two FP32 workgroups per CU."""
     path = make_model("block_l3k3_c3", l_max=3, k_max=3, body_order=5, rb_size=8, max_dist=5.0,
                       seed=3003, scaling=0.5)
     info = _check(path, n=3)
     assert (info["fwd_blocks"], info["fwd_terms"], info["bwd_terms"]) == (136, 3332, 6664)
     assert (info["tensors"], info["batches"], info["slots"]) == (145, 4, 258)
+    # two FP32 workgroups per CU: 48 x 64 doubles of adjoints + 224 x 64 floats = 80 KiB; the
+    # degree-2 class (42 tensors, 210 values) is halved into 126 + 84 values
+    h = sus2sh.block_check(path, target_slots=224, real_bytes=4)[0]
+    assert (h["ok"], h["slots"], h["units"], h["batches"], h["fwd_blocks"]) == (1, 210, 8, 5, 178)
 
 
 def test_pruned_explicit_graph(tmp_path):

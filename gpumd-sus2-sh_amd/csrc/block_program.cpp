@@ -434,6 +434,9 @@ BlockProgram compile_block_program(const Model& m, int waves, int real_bytes, in
   }
 
 
+  // padding: a record loop may fetch the two header words after its last record
+  for (int k = 0; k < 8; ++k) p.words.push_back(0u);
+
   // self-check against the flat graph on two seeded basic vectors (FP32 coefficients: 1e-5)
   uint64_t seed = 0xb10cb10cULL;
   std::vector<double> bv(B), g1(B), g2(B);

@@ -29,6 +29,7 @@
 //   FWD record (2 words):    v_off | a_off << 16,  b_off
 //   TENSOR record:           v_off | npart << 16,  a_off | b_off << 16,  d (Real),
 //                            npart x (partner v_off, K' (Real))
+// The word array ends with 8 padding words.
 // Offsets are LDS words (slot * 64; for a half, the slot of its first component); a Real takes
 // 1 (FP32) or 2 (FP64) words.  Kind = t | part << 6 for CG block type t of cg_table.hpp, and
 // kBlockBasicKind + 4 part + L for a basic tensor of degree L; part 0 is the whole tensor,

@@ -1372,10 +1372,11 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (param_grad) {
     // the parameter gradient needs per-scalar sums (dE/dw_s = sum_i sigma_i s_i): the
-    // persistent row-mode kernel; with a lane program the lane kernel writes the energies
-    // and adjoints and the row kernel only runs the forward pass for the sums
-    pgo.fwd_only = lane_.ok ? 1 : 0;
-    if (lane_.ok && !launch_graph_variant<Real>(3, tile_, N, gt, s))
+    // persistent row-mode kernel; with a block or lane program that kernel writes the
+    // energies and adjoints and the row kernel only runs the forward pass for the sums
+    const int ea = block_.ok ? 4 : (lane_.ok ? 3 : -1);
+    pgo.fwd_only = ea >= 0 ? 1 : 0;
+    if (ea >= 0 && !launch_graph_variant<Real>(ea, tile_, N, gt, s))
       throw UnsupportedError("SUS2_SH: graph kernel variant does not fit in LDS.");
     if (!launch_graph<Real>(tile_, 0, N, static_cast<const Real*>(ws_.basics), ws_.type_s, gt,
                             ws_.e_s, static_cast<Real*>(ws_.grads), &pgo, s, max_lds_,
@@ -1536,6 +1537,9 @@ void Engine::set_params(const double* theta, int n)
   take(m.scal_coeffs);
   prog_ = compile_graph(m, graph_chunk_terms());
   compile_lane();
+  // the block program folds the energy weights into its records: rebuilt with its current shape
+  block_ = build_block(block_.ok ? block_.waves
+                                 : (opt_.block_waves > 0 ? opt_.block_waves : block_waves_list()[0]));
   if (opt_.use_float)
     upload_tables<float>();
   else

@@ -141,3 +141,37 @@ def test_block_variant_rejections(make_model):
         with pytest.raises(sus2sh.Sus2shError) as err:
             sus2sh.evaluate(path, cell, types, pos, options=opts)
         assert "sus2_sh_graph_variant=4" in err.value.message
+
+
+@pytest.mark.parametrize("fp32", [True, False])
+def test_block_program_follows_set_params(model_dir, fp32):
+    """The block program folds the energy weights into its records: after set_params a plain
+    evaluation (no dE/dtheta) must use the new weights -- checked against the oracle on a model
+    file written with the same parameters."""
+    from sus2sh.hipmem import DeviceArray, synchronize
+    spec = sus2sh.synthetic_model(3, 3, 5, rb_size=8, max_dist=5.0, seed=12, scaling=0.5)
+    path = sus2sh.write_mtp(spec, str(model_dir / "blk_setp_a.mtp"))
+    theta = sus2sh.model_gen.param_vector(spec)
+    rng = np.random.default_rng(5)
+    theta2 = theta * (1.0 + 0.2 * rng.standard_normal(len(theta)))
+    path2 = sus2sh.write_mtp(sus2sh.model_gen.with_params(spec, theta2),
+                             str(model_dir / "blk_setp_b.mtp"))
+    pos, types, cell = st.fcc_alloy(4, 4, 4, 3.85, [0.5, 0.5], 0.08, 57, 58)
+    n = len(types)
+    ref = Oracle(path2).compute(cell, types, pos)
+    print(f"[case] set_params then compute, fp32={fp32} N={n}", flush=True)
+    pot = sus2sh.SUS2_SH(path, n, ["Cu", "Zr", f"sus2_float={int(fp32)}", "sus2_sh_graph_variant=4"])
+    t = DeviceArray.from_numpy(np.asarray(types, dtype=np.int32))
+    x = DeviceArray.from_numpy(np.ascontiguousarray(pos, dtype=np.float64))
+    out = []
+    for th in (theta, theta2):
+        pot.set_params(th)
+        e = DeviceArray.zeros((n,), np.float64)
+        f = DeviceArray.zeros((3, n), np.float64)
+        v = DeviceArray.zeros((9, n), np.float64)
+        pot.compute(sus2sh.box_struct(cell), t, x, e, f, v)
+        synchronize()
+        out.append((e.numpy(), f.numpy(), v.numpy()))
+    pot.close()
+    _compare(ref, out[1], fp32)
+    assert np.abs(out[1][0] - out[0][0]).max() > 1e-3  # the weights did change the energies

@@ -535,7 +535,8 @@ __device__ __forceinline__ int lower_bound_key(const int* __restrict__ skey, int
 // One wave per force tile.  Candidate intervals: for every cell row (y, z) the tile touches
 // and every row offset (oy, oz) within +-R, the x range of the tile's cells widened by +-R,
 // wrapped (two pieces) or clipped like the neighbour search; each cell range becomes an atom
-// range by binary search on the sorted keys.  Lane 0 sorts and merges them.
+// range by binary search on the sorted keys.  The wave then sorts the ranges by start and
+// merges overlapping or touching ones.
 constexpr int kWinCand = 128;
 
 __global__ __launch_bounds__(64) void k_windows(int N, CellGrid g, const int* __restrict__ skey,

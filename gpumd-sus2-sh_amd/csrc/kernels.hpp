@@ -1,7 +1,8 @@
 // Kernel launch interface of the SUS2-SH engine.  Implementations live in
 // neighbor.hip (cell list), edge_kernels.hip (basic moments, forces/virial),
 // graph_kernel.hip (CG product graph forward / energy / reverse), lane_kernel.hip (the
-// same graph as a wave-uniform program, one atom per lane).
+// same graph as a wave-uniform program, one atom per lane), block_kernel.hip (the same graph
+// as tensor blocks with compile-time couplings, one atom per lane).
 #pragma once
 
 #include <hip/hip_runtime.h>

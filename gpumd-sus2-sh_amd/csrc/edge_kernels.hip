@@ -251,7 +251,9 @@ __device__ __forceinline__ int window_slot(const int* __restrict__ w_first,
 
 // WIN: neighbour forces of the tile (kWinTile atoms) are summed in LDS over the tile's
 // neighbour windows (launch_windows) and flushed with one global atomic per neighbour and
-// component; a tile without windows (count < 0) falls back to per-edge global atomics.
+// component; a tile without windows (count < 0) falls back to per-edge global atomics.  The
+// window accumulators are double in both precisions: FP64 LDS atomic adds issue at the rate
+// of other LDS operations on this GPU, FP32 ones are many times slower.
 // The FP32 windowed kernel with 8-coefficient rows is held to 168 VGPRs (3 waves per SIMD)
 // for l_max <= 4; the l_max = 5, 6 grids would spill far more at that budget.
 // GATE (two-layer mu gate, src/force/sus2_sh.cu:6077-6123, 6555-6693): the edge's radial
@@ -289,7 +291,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
   // WIN: window table (first atom, prefix size) and the [3][total] accumulators
   int* w_first = reinterpret_cast<int*>(smem_raw + pg_off);
   int* w_off = w_first + kWinMax;
-  Real* acc_w = reinterpret_cast<Real*>(smem_raw + pg_off + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16));
+  double* acc_w = reinterpret_cast<double*>(smem_raw + pg_off + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16));
   unsigned long long* acc_q = reinterpret_cast<unsigned long long*>(acc_w);  // FIX
   unsigned long long* f_q = reinterpret_cast<unsigned long long*>(f_s);      // FIX
   int w_count = -1, w_total = 0;
@@ -316,7 +318,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
         if constexpr (FIX)
           acc_q[i] = 0ull;
         else
-          acc_w[i] = Real(0);
+          acc_w[i] = 0.0;
       }
     }
   }
@@ -442,9 +444,9 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
         atomicAdd(f_q + 2 * N + jn, q2);
       }
     } else if (slot >= 0) {
-      atomicAdd(acc_w + slot, -g0);
-      atomicAdd(acc_w + w_total + slot, -g1);
-      atomicAdd(acc_w + 2 * w_total + slot, -g2);
+      atomicAdd(acc_w + slot, -static_cast<double>(g0));
+      atomicAdd(acc_w + w_total + slot, -static_cast<double>(g1));
+      atomicAdd(acc_w + 2 * w_total + slot, -static_cast<double>(g2));
     } else {
       unsafeAtomicAdd(f_s + jn, -static_cast<double>(g0));
       unsafeAtomicAdd(f_s + N + jn, -static_cast<double>(g1));
@@ -503,8 +505,8 @@ __attribute__((amdgpu_waves_per_eu((sizeof(Real) == 4 && WIN && NB == 8 && L <= 
           const unsigned long long v = acc_q[c * w_total + i];
           if (v != 0ull) atomicAdd(f_q + static_cast<size_t>(c) * N + j, v);
         } else {
-          const Real v = acc_w[c * w_total + i];
-          if (v != Real(0)) unsafeAtomicAdd(f_s + static_cast<size_t>(c) * N + j, static_cast<double>(v));
+          const double v = acc_w[c * w_total + i];
+          if (v != 0.0) unsafeAtomicAdd(f_s + static_cast<size_t>(c) * N + j, v);
         }
       }
     }
@@ -580,11 +582,11 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   const int per_block = a.threads / a.npl;
   const int blocks = (a.N + per_block - 1) / per_block;
   const int T = a.threads;
-  static_assert(kWinTile % 256 == 0, "a window tile holds whole force workgroups");
+  // windows serve workgroups that lie inside one window tile (per_block divides kWinTile)
   // the parameter gradient, the gated kernel and deterministic summation use the per-mu path
   const bool fix = a.fix_flag != nullptr;
   const int nb = (pg_rad || a.gate_m || fix) ? 0 : uniform_nb(a, true);
-  const bool use_win = win && !pg_rad && !a.gate_m;
+  const bool use_win = win && !pg_rad && !a.gate_m && kWinTile % per_block == 0;
   size_t lds = table_bytes(a, nb, true);
   if (pg_rad) {
     const int Mr = a.K * (a.L + 1);
@@ -593,7 +595,7 @@ bool launch_forces_l(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, 
   }
   if (use_win)
     lds = ((lds + 15) & ~size_t(15)) + 16 * ((sizeof(int) * (2 * kWinMax + 1) + 15) / 16) +
-          3 * (fix ? sizeof(unsigned long long) : sizeof(Real)) * static_cast<size_t>(win_cap);
+          3 * sizeof(double) * static_cast<size_t>(win_cap);
 #define SUS2SH_FORCES_LAUNCH(LL, KK, PGV, NBV, WINV, GV, FV)                                   \
   {                                                                                            \
     auto kern = &k_forces<Real, LL, KK, PGV, NBV, WINV, GV, FV>;                               \

@@ -1388,12 +1388,12 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   if (profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
   // neighbour windows of the force tiles (single structure): LDS accumulation of the
-  // neighbour forces, budget 48 KiB (FP32) / 64 KiB (FP64) of accumulators per tile
+  // neighbour forces, budget 48 KiB of 8-byte accumulators (double, or 64-bit fixed point in
+  // deterministic mode) per 128-atom tile
   const int* win = nullptr;
   int win_cap = 0;
   if (use_win) {
-    const size_t acc_bytes = opt_.deterministic ? sizeof(unsigned long long) : sizeof(Real);
-    win_cap = static_cast<int>((opt_.use_float ? 48 * 1024 : 64 * 1024) / (3 * acc_bytes));
+    win_cap = static_cast<int>(48 * 1024 / (3 * sizeof(double)));
     launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
     win = ws_.win;
   }

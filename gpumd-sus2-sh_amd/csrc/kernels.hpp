@@ -137,7 +137,9 @@ void launch_neighbors(const NeighborArgs& a, const double* pos_s, const PackedAt
 // neighbours of a tile lie in few contiguous ranges of the sorted order (the cells within
 // +-R of the tile's cells, row by row).  win[tile * kWinStride + ...] = count (-1: no window
 // set, the tile uses global atomics), total atoms, then count pairs (first atom, atoms).
-constexpr int kWinTile = 256;
+// 128-atom tiles: their windows (~1,300-1,500 atoms in a dense box with rc-sized cells) fit
+// 48 KiB of FP64 accumulators.
+constexpr int kWinTile = 128;
 constexpr int kWinMax = 64;
 constexpr int kWinStride = 2 + 2 * kWinMax;
 void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,

@@ -1007,8 +1007,8 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   const bool profile = opt_.profile && !capturing && !noalloc;
   if (cap_ == 0) cap_ = 64;
   const int n_totals = static_cast<int>(cell_sort_totals(ncell));
-  // force-tile neighbour windows (single structure, no parameter gradient)
-  const bool use_win = !batch && !param_grad;
+  // force-tile neighbour windows (no parameter gradient)
+  const bool use_win = !param_grad;
   const int win_words =
     use_win ? static_cast<int>((static_cast<size_t>(N) + kWinTile - 1) / kWinTile) * kWinStride : 0;
   if (noalloc) {
@@ -1387,14 +1387,17 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   }
   if (profile) hip_check(hipEventRecord(events_[3], s), "hipEventRecord");
 
-  // neighbour windows of the force tiles (single structure): LDS accumulation of the
+  // neighbour windows of the force tiles: LDS accumulation of the
   // neighbour forces, budget 48 KiB of 8-byte accumulators (double, or 64-bit fixed point in
   // deterministic mode) per 128-atom tile
   const int* win = nullptr;
   int win_cap = 0;
   if (use_win) {
     win_cap = static_cast<int>(48 * 1024 / (3 * sizeof(double)));
-    launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
+    if (batch)
+      launch_windows_batch(N, sid_s_.as<int>(), batch_atom_off_.as<int>(), ws_.win, win_cap, s);
+    else
+      launch_windows(N, grid, ws_.skey, ws_.win, win_cap, s);
     win = ws_.win;
   }
   if (edge_tune && opt_.force_threads <= 0) {

@@ -144,6 +144,10 @@ constexpr int kWinMax = 64;
 constexpr int kWinStride = 2 + 2 * kWinMax;
 void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, int cap,
                     hipStream_t s);
+// The same for a batch (sid_s: structure of each sorted atom, atom_off [nstruct + 1] on the
+// device): one window per tile, the sorted range of the structures the tile holds.
+void launch_windows_batch(int N, const int* sid_s, const int* atom_off, int* win, int cap,
+                          hipStream_t s);
 
 // --- ZBL pair term (zbl.hip) ---
 struct ZblTable {
@@ -211,7 +215,7 @@ bool launch_basics(const EdgeLaunch<Real>& a, Real* basics, hipStream_t s);
 // force/virial: grads[B][N] -> f_s[3][N] (atomic, double) and v_s[9][N].
 // pg_rad (optional): accumulates dE/d(radial coeff, type coeff, scal) per pair into
 // pg_rad[S*S][Mr*rb + Mr*2] (double, atomic).
-// win (optional, single structure): tile windows from launch_windows with capacity win_cap;
+// win (optional): tile windows from launch_windows(_batch) with capacity win_cap;
 // neighbour forces are then summed per tile in LDS and flushed once per neighbour.
 template <typename Real>
 bool launch_forces(const EdgeLaunch<Real>& a, const Real* grads, double* f_s, double* v_s,

@@ -780,6 +780,40 @@ void launch_windows(int N, const CellGrid& g, const int* sorted_key, int* win, i
   k_windows<<<blocks_for(N, kWinTile), 64, 0, s>>>(N, g, sorted_key, win, cap);
 }
 
+namespace {
+
+// Batch windows, one lane per tile: the structures of a batch occupy consecutive ranges of
+// the sorted order (atom_off), and every neighbour of an atom belongs to its own structure,
+// so a tile's neighbours lie in the one range spanning its first to its last structure.
+__global__ void k_windows_batch(int N, const int* __restrict__ sid_s,
+                                const int* __restrict__ atom_off, int* __restrict__ win, int cap)
+{
+  const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p0 = tile * kWinTile;
+  if (p0 >= N) return;
+  const int p1 = min(p0 + kWinTile, N) - 1;
+  const int a = atom_off[sid_s[p0]], b = atom_off[sid_s[p1] + 1];
+  int* out = win + static_cast<size_t>(tile) * kWinStride;
+  if (b - a <= cap) {
+    out[0] = 1;
+    out[1] = b - a;
+    out[2] = a;
+    out[3] = b - a;
+  } else {
+    out[0] = -1;
+  }
+}
+
+}  // namespace
+
+void launch_windows_batch(int N, const int* sid_s, const int* atom_off, int* win, int cap,
+                          hipStream_t s)
+{
+  if (N <= 0) return;
+  const int tiles = blocks_for(N, kWinTile);
+  k_windows_batch<<<blocks_for(tiles, 64), 64, 0, s>>>(N, sid_s, atom_off, win, cap);
+}
+
 void launch_scatter(int N, const int* rank, const double* e_s, const double* f_s,
                     const double* v_s, double* potential, double* force, double* virial,
                     int fixed, hipStream_t s)

@@ -6,7 +6,8 @@ partial (k, l, m) grids, forces and virials are summed with lane shuffles.  Each
 pinned with sus2_sh_basic_shape / sus2_sh_force_shape and compared with the FP64 oracle on a
 2,048-atom periodic fcc Cu-Zr box (8 force tiles of 256 atoms whose neighbour forces are
 summed in LDS windows: a workgroup of threads / npl atoms uses its tile's windows), in both
-precisions; a second case checks a batch (per-edge global atomics) and the autotuner, which
+precisions; a second case checks a batch (one window per tile: the sorted range of its
+structures) and the autotuner, which
 times all 12 x 2 candidates on the first call from 512 atoms on.
 
 Tolerances as in test_gpu_parity: FP64 1e-9 eV/atom, 1e-8 eV/A, 1e-8 eV;

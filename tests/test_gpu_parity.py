@@ -268,8 +268,8 @@ def test_lane_program_wave_splits(make_model, waves):
 def test_force_tiles_with_neighbour_windows(make_model, fp32):
     """A periodic box of 6x6x6 cells and 2,048 atoms: eight 256-atom force tiles whose
     neighbours are summed in LDS over index windows (wrapped rows included) -- against the
-    oracle, and against the same evaluation as one structure of a batch (batches keep the
-    per-edge global atomics)."""
+    oracle, and against the same evaluation as one structure of a batch (a batch tile's window
+    is the sorted range of the structures it holds)."""
     path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])
     pos, types, cell = st.fcc_alloy(8, 8, 8, 3.85, [0.5, 0.5], 0.08, 31, 32)
     ref = Oracle(path).compute(cell, types, pos)
@@ -312,3 +312,38 @@ def test_graph_kernel_variants(make_model, variant, tile):
             continue
         got = _run(path, cell, types, pos, options=opts)
         _compare(ref, got, bool(fp32))
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_batch_windows_mixed_sizes(make_model, fp32):
+    """A batch of small cells (a 128-atom force tile spans several of them: one window, the
+    sorted range of its structures) and a 2,916-atom cell (its range exceeds the 2,048-atom
+    window capacity: its tiles fall back to per-edge atomics), each against the oracle."""
+    import torch
+    path = make_model("l3k3_fcc", **CASES["l3k3_fcc"]["model"])
+    structs = [st.fcc_alloy(2, 2, 2, 3.85, [0.5, 0.5], 0.08, 61, 62),
+               st.fcc_alloy(3, 3, 3, 3.85, [0.5, 0.5], 0.08, 63, 64),
+               st.fcc_alloy(9, 9, 9, 3.85, [0.5, 0.5], 0.08, 65, 66),
+               st.fcc_alloy(4, 4, 4, 3.85, [0.5, 0.5], 0.08, 67, 68),
+               st.fcc_alloy(3, 3, 2, 3.85, [0.5, 0.5], 0.08, 69, 70)]
+    sizes = [len(t) for _, t, _ in structs]
+    assert sizes[2] == 2916 and max(s for k, s in enumerate(sizes) if k != 2) <= 256
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(offs[-1])
+    print(f"[case] batch windows, sizes {sizes}, fp32={fp32}", flush=True)
+    pot = sus2sh.SUS2_SH(path, n, ["Cu", "Zr", f"sus2_float={int(fp32)}"])
+    dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a, dtype=dt), device="cuda")
+    e = torch.zeros(n, dtype=torch.float64, device="cuda")
+    f = torch.zeros(3, n, dtype=torch.float64, device="cuda")
+    v = torch.zeros(9, n, dtype=torch.float64, device="cuda")
+    sus2sh.compute_batch(pot, [c for _, _, c in structs], offs,
+                         dev(np.concatenate([t for _, t, _ in structs]), np.int32),
+                         dev(np.concatenate([p for p, _, _ in structs], axis=1), np.float64),
+                         e, f, v)
+    torch.cuda.synchronize()
+    pot.close()
+    e, f, v = e.cpu().numpy(), f.cpu().numpy(), v.cpu().numpy()
+    orc = Oracle(path)
+    for k, (pos, types, cell) in enumerate(structs):
+        sl = slice(offs[k], offs[k + 1])
+        _compare(orc.compute(cell, types, pos), (e[sl], f[:, sl], v[:, sl]), fp32)

@@ -212,6 +212,29 @@ __device__ __forceinline__ int wrap_cell(int c, int n, int& shift)
 // A candidate is one 32-byte (x, y, z, type) record of the packed copy the cell sort writes:
 // two 16-byte loads from one cache line instead of three gathers from three arrays and a
 // fourth for the species of each neighbour (the kernel is bound by these gathers' latency).
+template <typename Real>
+__device__ __forceinline__ void test_candidate(const PackedAtom& pq, int q, int p, bool self_image,
+                                               double xp, double yp, double zp, double shx,
+                                               double shy, double shz, double rc2, bool valid,
+                                               int cap, int N, int& c, Edge<Real>* __restrict__ nl)
+{
+  if (self_image && q == p) return;
+  const double dx = pq.x - xp + shx;
+  const double dy = pq.y - yp + shy;
+  const double dz = pq.z - zp + shz;
+  if (dx * dx + dy * dy + dz * dz < rc2) {
+    if (valid && c < cap) {
+      Edge<Real> e;
+      e.dx = static_cast<Real>(dx);
+      e.dy = static_cast<Real>(dy);
+      e.dz = static_cast<Real>(dz);
+      e.j = q | (static_cast<int>(pq.t) << kEdgeTypeShift);
+      nl[static_cast<size_t>(c) * N + p] = e;
+    }
+    ++c;
+  }
+}
+
 template <typename Real, bool BATCH>
 __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double* __restrict__ pos,
                                                    const PackedAtom* __restrict__ packed,
@@ -262,24 +285,20 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
         const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
         const bool self_image = (sx | sy | sz) == 0;
         const int qe = cend[cc];
-        for (int q = cbeg[cc]; q < qe; ++q) {
-          if (self_image && q == p) continue;
-          const PackedAtom pq = packed[q];
-          const double dx = pq.x - xp + shx;
-          const double dy = pq.y - yp + shy;
-          const double dz = pq.z - zp + shz;
-          if (dx * dx + dy * dy + dz * dz < a.rc2) {
-            if (valid && c < a.cap) {
-              Edge<Real> e;
-              e.dx = static_cast<Real>(dx);
-              e.dy = static_cast<Real>(dy);
-              e.dz = static_cast<Real>(dz);
-              e.j = q | (static_cast<int>(pq.t) << kEdgeTypeShift);
-              nl[static_cast<size_t>(c) * N + p] = e;
-            }
-            ++c;
-          }
+        int q = cbeg[cc];
+        // four candidates' loads in flight together, then their tests
+        for (; q + 4 <= qe; q += 4) {
+          PackedAtom c4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c4[u] = packed[q + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            test_candidate<Real>(c4[u], q + u, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
+                                 valid, a.cap, N, c, nl);
         }
+        for (; q < qe; ++q)
+          test_candidate<Real>(packed[q], q, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
+                               valid, a.cap, N, c, nl);
       }
     }
   }
@@ -313,28 +332,6 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
 constexpr int kStageCap = 256;  // staged candidate atoms per band (8 KiB)
 constexpr int kStageTab = 128;  // staged cells per band
 
-template <typename Real>
-__device__ __forceinline__ void test_candidate(const PackedAtom& pq, int q, int p, bool self_image,
-                                               double xp, double yp, double zp, double shx,
-                                               double shy, double shz, double rc2, bool valid,
-                                               int cap, int N, int& c, Edge<Real>* __restrict__ nl)
-{
-  if (self_image && q == p) return;
-  const double dx = pq.x - xp + shx;
-  const double dy = pq.y - yp + shy;
-  const double dz = pq.z - zp + shz;
-  if (dx * dx + dy * dy + dz * dz < rc2) {
-    if (valid && c < cap) {
-      Edge<Real> e;
-      e.dx = static_cast<Real>(dx);
-      e.dy = static_cast<Real>(dy);
-      e.dz = static_cast<Real>(dz);
-      e.j = q | (static_cast<int>(pq.t) << kEdgeTypeShift);
-      nl[static_cast<size_t>(c) * N + p] = e;
-    }
-    ++c;
-  }
-}
 
 template <typename Real>
 __global__ __launch_bounds__(64) void k_neighbors_staged(NeighborArgs a,

@@ -178,7 +178,7 @@ __global__ void k_rank(int N, int S, const int* __restrict__ key, const int* __r
                        int* __restrict__ bad_type, const int* __restrict__ sid,
                        int* __restrict__ sid_s, int* __restrict__ rank)
 {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= N) return;
   const int i = arrival[q];
   const int k = key[i];
@@ -488,7 +488,7 @@ __global__ void k_scatter(int N, const int* __restrict__ rank, const double* __r
                           double* __restrict__ pot, double* __restrict__ force,
                           double* __restrict__ vir, int fixed)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int p = rank[i];
   pot[i] += e_s[p];

@@ -1072,6 +1072,7 @@ void Engine::run(int nstruct, const sus2sh_box* boxes, const int* atom_offsets, 
   na.grids = batch ? batch_grids_.as<CellGrid>() : nullptr;
   na.cell_off = batch ? batch_cell_off_.as<int>() : nullptr;
   na.sid_s = batch ? sid_s_.as<int>() : nullptr;
+  na.atom_off = batch ? batch_atom_off_.as<int>() : nullptr;
   na.rc2 = model_.max_dist * model_.max_dist;
   if (deferred) {
     na.cap = ws_.cap;

@@ -86,6 +86,7 @@ struct NeighborArgs {
   const CellGrid* grids;      // batch: per-structure grids (nullptr for a single structure)
   const int* cell_off;        // batch: first cell key of each structure
   const int* sid_s;           // batch: structure of each sorted atom
+  const int* atom_off;        // batch: first atom of each structure (sorted order keeps them)
   double rc2;
   int cap;
 };

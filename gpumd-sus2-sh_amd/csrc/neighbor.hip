@@ -235,6 +235,37 @@ __device__ __forceinline__ void test_candidate(const PackedAtom& pq, int q, int 
   }
 }
 
+// Batch: a workgroup's candidates are the atoms of the structures its atoms belong to, one
+// contiguous sorted range; up to kBatchStage of them (structures of <= 256 atoms around a
+// 256-atom workgroup: at most 766) are copied into LDS once, so that the candidate loads of
+// the search -- a serial chain per lane, hundreds of them for a small triclinic cell's
+// images -- are LDS reads instead of L2 round trips.
+constexpr int kBatchStage = 768;  // 24 KiB
+
+// The candidates [qb, qe) of one cell, candidate q at src[q - qoff]: four records' loads in
+// flight together, then their tests.
+template <typename Real>
+__device__ __forceinline__ void scan_cell(const PackedAtom* __restrict__ src, int qoff, int qb,
+                                          int qe, int p, bool self_image, double xp, double yp,
+                                          double zp, double shx, double shy, double shz,
+                                          double rc2, bool valid, int cap, int N, int& c,
+                                          Edge<Real>* __restrict__ nl)
+{
+  int q = qb;
+  for (; q + 4 <= qe; q += 4) {
+    PackedAtom c4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c4[u] = src[q + u - qoff];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      test_candidate<Real>(c4[u], q + u, p, self_image, xp, yp, zp, shx, shy, shz, rc2, valid,
+                           cap, N, c, nl);
+  }
+  for (; q < qe; ++q)
+    test_candidate<Real>(src[q - qoff], q, p, self_image, xp, yp, zp, shx, shy, shz, rc2, valid,
+                         cap, N, c, nl);
+}
+
 template <typename Real, bool BATCH>
 __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double* __restrict__ pos,
                                                    const PackedAtom* __restrict__ packed,
@@ -257,6 +288,20 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
   const int cy = (key / g.n[0]) % g.n[1];
   const int cz = key / (g.n[0] * g.n[1]);
   const double xp = pos[p], yp = pos[N + p], zp = pos[2 * N + p];
+  __shared__ PackedAtom s_atoms[BATCH ? kBatchStage : 1];
+  bool staged = false;
+  int qoff = 0;
+  if constexpr (BATCH) {
+    const int pf = xcd_block() * static_cast<int>(blockDim.x);
+    const int pl = min(pf + static_cast<int>(blockDim.x), N) - 1;
+    const int lo = a.atom_off[a.sid_s[pf]], hi = a.atom_off[a.sid_s[pl] + 1];
+    staged = hi - lo <= kBatchStage;  // workgroup-uniform
+    if (staged) {
+      for (int t = threadIdx.x; t < hi - lo; t += blockDim.x) s_atoms[t] = packed[lo + t];
+      __syncthreads();
+      qoff = lo;
+    }
+  }
   int c = 0;
   for (int oz = -g.R[2]; oz <= g.R[2]; ++oz) {
     int sz = 0, z = cz + oz;
@@ -284,21 +329,13 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
         const double shy = sx * g.h[3] + sy * g.h[4] + sz * g.h[5];
         const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
         const bool self_image = (sx | sy | sz) == 0;
-        const int qe = cend[cc];
-        int q = cbeg[cc];
-        // four candidates' loads in flight together, then their tests
-        for (; q + 4 <= qe; q += 4) {
-          PackedAtom c4[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) c4[u] = packed[q + u];
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            test_candidate<Real>(c4[u], q + u, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
-                                 valid, a.cap, N, c, nl);
-        }
-        for (; q < qe; ++q)
-          test_candidate<Real>(packed[q], q, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
-                               valid, a.cap, N, c, nl);
+        const int qb = cbeg[cc], qe = cend[cc];
+        if (BATCH && staged)
+          scan_cell<Real>(s_atoms, qoff, qb, qe, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
+                          valid, a.cap, N, c, nl);
+        else
+          scan_cell<Real>(packed, 0, qb, qe, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
+                          valid, a.cap, N, c, nl);
       }
     }
   }

@@ -241,6 +241,9 @@ __device__ __forceinline__ void test_candidate(const PackedAtom& pq, int q, int 
 // the search -- a serial chain per lane, hundreds of them for a small triclinic cell's
 // images -- are LDS reads instead of L2 round trips.
 constexpr int kBatchStage = 768;  // 24 KiB
+// The cell ranges of those structures, the search's other serial loads, are staged too when
+// they number at most kBatchCells.
+constexpr int kBatchCells = 256;  // 2 KiB
 
 // The candidates [qb, qe) of one cell, candidate q at src[q - qoff]: four records' loads in
 // flight together, then their tests.
@@ -289,18 +292,29 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
   const int cz = key / (g.n[0] * g.n[1]);
   const double xp = pos[p], yp = pos[N + p], zp = pos[2 * N + p];
   __shared__ PackedAtom s_atoms[BATCH ? kBatchStage : 1];
-  bool staged = false;
-  int qoff = 0;
+  __shared__ int s_cb[BATCH ? kBatchCells : 1], s_ce[BATCH ? kBatchCells : 1];
+  bool staged = false, cells_staged = false;
+  int qoff = 0, coff = 0;
   if constexpr (BATCH) {
     const int pf = xcd_block() * static_cast<int>(blockDim.x);
     const int pl = min(pf + static_cast<int>(blockDim.x), N) - 1;
-    const int lo = a.atom_off[a.sid_s[pf]], hi = a.atom_off[a.sid_s[pl] + 1];
+    const int s0 = a.sid_s[pf], s1 = a.sid_s[pl];
+    const int lo = a.atom_off[s0], hi = a.atom_off[s1 + 1];
+    const int clo = a.cell_off[s0], nc = a.cell_off[s1 + 1] - clo;
     staged = hi - lo <= kBatchStage;  // workgroup-uniform
+    cells_staged = nc <= kBatchCells;
     if (staged) {
       for (int t = threadIdx.x; t < hi - lo; t += blockDim.x) s_atoms[t] = packed[lo + t];
-      __syncthreads();
       qoff = lo;
     }
+    if (cells_staged) {
+      for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+        s_cb[t] = cbeg[clo + t];
+        s_ce[t] = cend[clo + t];
+      }
+      coff = clo;
+    }
+    if (staged || cells_staged) __syncthreads();
   }
   int c = 0;
   for (int oz = -g.R[2]; oz <= g.R[2]; ++oz) {
@@ -329,7 +343,14 @@ __global__ __launch_bounds__(256) void k_neighbors(NeighborArgs a, const double*
         const double shy = sx * g.h[3] + sy * g.h[4] + sz * g.h[5];
         const double shz = sx * g.h[6] + sy * g.h[7] + sz * g.h[8];
         const bool self_image = (sx | sy | sz) == 0;
-        const int qb = cbeg[cc], qe = cend[cc];
+        int qb, qe;
+        if (BATCH && cells_staged) {
+          qb = s_cb[cc - coff];
+          qe = s_ce[cc - coff];
+        } else {
+          qb = cbeg[cc];
+          qe = cend[cc];
+        }
         if (BATCH && staged)
           scan_cell<Real>(s_atoms, qoff, qb, qe, p, self_image, xp, yp, zp, shx, shy, shz, a.rc2,
                           valid, a.cap, N, c, nl);
